@@ -1,0 +1,238 @@
+"""blacktracer_amd -- MI355X-native implementation of BlackTracer's
+data-parallel core, exposed through the C ABI of include/bt_abi.h.
+
+The Python classes mirror the reference's host interface
+(annemiekie/BlackTracer, BlackTracer2/BlackTracer/):
+
+    BlackHole(a)                                     BlackHole.h:8-15
+    Camera(theta, phi, r, br, btheta, bphi)          Camera.h:40-61
+    Grid(max_level, start_level, angle, cam, bh)     Grid.h:605-624
+    Grid.integration_wrapper(theta, phi)             Grid.h:514-556
+    Viewer, StarProcessor, make_image                Viewer.h, StarProcessor.h,
+                                                     MakeImageKernel.cu:1206
+
+All work runs on the HIP device through libbtamd.so; there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (BT_PREC_FP64, BT_PREC_MIXED, BtError, BtGridParams, BtIntegParams, BtPose,
+                   BtRenderFlags, BtTraceStats, check)
+
+__all__ = ["Context", "BlackHole", "Camera", "Grid", "IntegParams", "trace_rays", "BtError",
+           "BT_PREC_FP64", "BT_PREC_MIXED", "default_context"]
+
+
+class Context:
+    """A bt_context: one HIP device, its stream, work buffers and timers."""
+
+    def __init__(self, device=0):
+        L = _lib.load()
+        h = C.c_void_p()
+        check(L.bt_context_create(device, C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def sync(self):
+        check(_lib.load().bt_context_sync(self.handle))
+
+    def timing(self, enable=True):
+        check(_lib.load().bt_timing_enable(self.handle, 1 if enable else 0))
+
+    def timer(self, kernel):
+        n = C.c_int64()
+        ms = C.c_double()
+        check(_lib.load().bt_timer_get(self.handle, kernel.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def reset_timers(self):
+        check(_lib.load().bt_timer_reset(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().bt_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default = None
+
+
+def default_context():
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+class BlackHole:
+    """BlackHole(afactor) (BlackHole.h:8-15).  The spin is carried by the
+    pose of each call instead of a process global."""
+
+    def __init__(self, afactor):
+        self.a = float(afactor)
+
+
+class Camera:
+    """Camera(theta, phi, r, br, btheta, bphi) on the circular equatorial
+    orbit (Camera.h:40-52), or Camera(theta, phi, r, speed=...) (Camera.h:27-38)."""
+
+    def __init__(self, theta, phi, r, br=0.0, btheta=0.0, bphi=1.0, speed=None, black_hole=None):
+        self.theta, self.phi, self.r = float(theta), float(phi), float(r)
+        self.br, self.btheta, self.bphi = float(br), float(btheta), float(bphi)
+        self.user_speed = speed is not None
+        self._speed_in = float(speed) if speed is not None else 0.0
+        self.a = black_hole.a if black_hole is not None else None
+        if self.a is not None:
+            self._init_forms()
+
+    def pose(self, a=None):
+        a = self.a if a is None else a
+        if a is None:
+            raise ValueError("Camera needs a BlackHole (spin) for its pose")
+        return BtPose(a=a, theta=self.theta, phi=self.phi, r=self.r, br=self.br,
+                      btheta=self.btheta, bphi=self.bphi, speed=self._speed_in,
+                      user_speed=1 if self.user_speed else 0)
+
+    def _init_forms(self):
+        out = (C.c_double * 6)()
+        cam7 = (C.c_float * 7)()
+        check(_lib.load().bt_camera_init(C.byref(self.pose()), out, cam7))
+        self.speed, self.alpha, self.w, self.wbar, self.Delta, self.ro = list(out)
+        self.param_array = np.array(list(cam7), dtype=np.float32)
+
+    def bind(self, black_hole):
+        self.a = black_hole.a
+        self._init_forms()
+        return self
+
+    def getParamArray(self):  # noqa: N802  (reference name, Camera.h:67)
+        return self.param_array.copy()
+
+
+def IntegParams(precision=BT_PREC_FP64, **kw):  # noqa: N802
+    """bt_integ_params with the reference defaults (Metric.h:307-318), overridable."""
+    p = BtIntegParams()
+    _lib.load().bt_integ_params_default(C.byref(p))
+    p.precision = precision
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def baseline_params(precision=BT_PREC_FP64):
+    """The BASELINE wording of the grid build: rel tol 1e-8, escape at
+    r > 1000M, capture at r < 1.01 r+ (no a-priori capture test)."""
+    return IntegParams(precision, eps=1e-8, max_steps=10000, apriori_capture=0, r_escape=1000.0,
+                       r_capture_factor=1.01)
+
+
+def _stats_dict(s):
+    return {n: getattr(s, n) for n, _ in BtTraceStats._fields_}
+
+
+def trace_rays(camera, theta, phi, params=None, ctx=None, return_stats=False):
+    """Grid::integration_wrapper on the device: returns (theta', phi', step)."""
+    ctx = ctx or default_context()
+    th = np.ascontiguousarray(theta, dtype=np.float64).copy()
+    ph = np.ascontiguousarray(phi, dtype=np.float64).copy()
+    st = np.zeros(th.shape, dtype=np.int32)
+    stats = BtTraceStats()
+    params = params or IntegParams()
+    check(_lib.load().bt_trace_rays(ctx.handle, C.byref(camera.pose()), C.byref(params), th.size,
+                                    th.ctypes.data_as(_lib.DP), ph.ctypes.data_as(_lib.DP),
+                                    st.ctypes.data_as(_lib.I32P), C.byref(stats)))
+    if return_stats:
+        return th, ph, st, _stats_dict(stats)
+    return th, ph, st
+
+
+class Grid:
+    """Grid(maxLevel, startLevel, angle, camera, blackhole) (Grid.h:605-624),
+    built and kept resident on the device.  ``cel``/``steps``/``levels``
+    are dense host copies made on first access."""
+
+    def __init__(self, max_level, start_level, angle, camera, black_hole, params=None,
+                 refine_thresh=0.015, fix_tvertices=True, ctx=None):
+        self.ctx = ctx or default_context()
+        L = _lib.load()
+        gp = BtGridParams()
+        L.bt_grid_params_default(C.byref(gp))
+        gp.refine_thresh = refine_thresh
+        gp.fix_tvertices = 1 if fix_tvertices else 0
+        self.params = params or IntegParams()
+        h = C.c_void_p()
+        check(L.bt_grid_build(self.ctx.handle, C.byref(camera.pose(black_hole.a)),
+                              C.byref(self.params), max_level, start_level, 1 if angle else 0,
+                              C.byref(gp), C.byref(h)))
+        self.handle = h
+        vals = [C.c_int() for _ in range(5)]
+        rays = C.c_int64()
+        check(L.bt_grid_info(h, *[C.byref(v) for v in vals], C.byref(rays)))
+        self.N, self.M, self.MAXLEVEL, self.STARTLVL, self.equafactor = [v.value for v in vals]
+        self.rays_traced = rays.value
+        self._cel = self._steps = self._levels = None
+
+    def stats(self):
+        s = BtTraceStats()
+        check(_lib.load().bt_grid_stats(self.handle, C.byref(s)))
+        return _stats_dict(s)
+
+    def _export(self):
+        N, M = self.N, self.M
+        cel = np.empty((N, M, 2), dtype=np.float64)
+        steps = np.empty((N, M), dtype=np.int32)
+        lv = np.empty((N, M), dtype=np.int8)
+        check(_lib.load().bt_grid_export(self.handle, cel.ctypes.data_as(_lib.DP),
+                                         steps.ctypes.data_as(_lib.I32P),
+                                         lv.ctypes.data_as(_lib.I8P)))
+        self._cel, self._steps, self._levels = cel, steps, lv
+
+    @property
+    def cel(self):
+        if self._cel is None:
+            self._export()
+        return self._cel
+
+    @property
+    def steps(self):
+        if self._steps is None:
+            self._export()
+        return self._steps
+
+    @property
+    def levels(self):
+        if self._levels is None:
+            self._export()
+        return self._levels
+
+    def psh(self):
+        """saveAsGpuHash (Grid.h:642-654): (hash_table, offset_table, hash_pos_tag, hw, ow)."""
+        L = _lib.load()
+        hw, ow = C.c_int(), C.c_int()
+        check(L.bt_grid_export_psh(self.handle, C.byref(hw), C.byref(ow), None, None, None))
+        ht = np.zeros(2 * hw.value * hw.value, dtype=np.float32)
+        ot = np.zeros(2 * ow.value * ow.value, dtype=np.int32)
+        tag = np.zeros(2 * hw.value * hw.value, dtype=np.int32)
+        check(L.bt_grid_export_psh(self.handle, C.byref(hw), C.byref(ow),
+                                   ht.ctypes.data_as(_lib.FP), ot.ctypes.data_as(_lib.I32P),
+                                   tag.ctypes.data_as(_lib.I32P)))
+        return ht, ot, tag, hw.value, ow.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().bt_grid_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,123 @@
+"""ctypes bindings of libbtamd.so (the C ABI declared in include/bt_abi.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (make -C
+blacktracer_amd/csrc).  There is no fallback: if the shared object is missing
+or a HIP device is absent, the calls raise.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbtamd.so")
+
+DP = C.POINTER(C.c_double)
+FP = C.POINTER(C.c_float)
+I32P = C.POINTER(C.c_int32)
+I8P = C.POINTER(C.c_int8)
+U8P = C.POINTER(C.c_uint8)
+I64P = C.POINTER(C.c_int64)
+
+BT_OK = 0
+BT_ERR_INVALID = -1
+BT_ERR_DEVICE = -2
+BT_ERR_HIP = -3
+BT_ERR_NOMEM = -4
+BT_ERR_BUDGET = -5
+BT_ERR_NOTFOUND = -6
+BT_ERR_STATE = -7
+
+BT_PREC_FP64 = 0
+BT_PREC_MIXED = 1
+
+
+class BtPose(C.Structure):
+    _fields_ = [("a", C.c_double), ("theta", C.c_double), ("phi", C.c_double), ("r", C.c_double),
+                ("br", C.c_double), ("btheta", C.c_double), ("bphi", C.c_double),
+                ("speed", C.c_double), ("user_speed", C.c_int)]
+
+
+class BtIntegParams(C.Structure):
+    _fields_ = [("eps", C.c_double), ("h1", C.c_double), ("z_end", C.c_double),
+                ("max_steps", C.c_int), ("apriori_capture", C.c_int),
+                ("r_escape", C.c_double), ("r_capture_factor", C.c_double),
+                ("newton_cap", C.c_int), ("precision", C.c_int),
+                ("crit_eps", C.c_double), ("hmin_fp64", C.c_double)]
+
+
+class BtTraceStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("rays", "integrated", "attempts", "attempts_fp64",
+                                         "newton", "golden", "fallback_rays")]
+
+
+class BtGridParams(C.Structure):
+    _fields_ = [("refine_thresh", C.c_double), ("fix_tvertices", C.c_int)]
+
+
+class BtRenderFlags(C.Structure):
+    _fields_ = [("star", C.c_int), ("map", C.c_int), ("lr", C.c_int), ("hor", C.c_float),
+                ("ver", C.c_float), ("offset", C.c_float)]
+
+
+# name -> (restype, argtypes); every symbol here is declared in include/bt_abi.h
+SIGNATURES = {
+    "bt_last_error": (C.c_char_p, []),
+    "bt_abi_version": (C.c_int, []),
+    "bt_context_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "bt_context_destroy": (None, [C.c_void_p]),
+    "bt_context_sync": (C.c_int, [C.c_void_p]),
+    "bt_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "bt_timer_get": (C.c_int, [C.c_void_p, C.c_char_p, I64P, DP]),
+    "bt_timer_reset": (C.c_int, [C.c_void_p]),
+    "bt_camera_init": (C.c_int, [C.POINTER(BtPose), DP, FP]),
+    "bt_integ_params_default": (None, [C.POINTER(BtIntegParams)]),
+    "bt_trace_rays": (C.c_int, [C.c_void_p, C.POINTER(BtPose), C.POINTER(BtIntegParams), C.c_int64,
+                                DP, DP, I32P, C.POINTER(BtTraceStats)]),
+    "bt_trace_rays_device": (C.c_int, [C.c_void_p, C.POINTER(BtPose), C.POINTER(BtIntegParams),
+                                       C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.POINTER(BtTraceStats)]),
+    "bt_grid_params_default": (None, [C.POINTER(BtGridParams)]),
+    "bt_grid_build": (C.c_int, [C.c_void_p, C.POINTER(BtPose), C.POINTER(BtIntegParams), C.c_int,
+                                C.c_int, C.c_int, C.POINTER(BtGridParams), C.POINTER(C.c_void_p)]),
+    "bt_grid_info": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int)] * 5 + [I64P]),
+    "bt_grid_stats": (C.c_int, [C.c_void_p, C.POINTER(BtTraceStats)]),
+    "bt_grid_export": (C.c_int, [C.c_void_p, DP, I32P, I8P]),
+    "bt_grid_export_psh": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), FP, I32P,
+                                     I32P]),
+    "bt_grid_free": (None, [C.c_void_p]),
+    "bt_render_flags_default": (None, [C.POINTER(BtRenderFlags)]),
+    "bt_make_image": (C.c_int, [C.c_void_p, FP, I32P, C.c_int, FP, FP, C.c_int, C.c_int, C.c_int,
+                                C.c_int, U8P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, FP,
+                                FP, C.c_float, C.c_int, C.c_int, C.c_int, I32P, FP, I32P, I32P,
+                                C.c_int, C.c_int, C.POINTER(BtRenderFlags), U8P]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libbtamd.so and bind every declared symbol; raises if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"blacktracer_amd native library missing: {LIB_PATH}; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class BtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"blacktracer_amd error {code}: {msg}")
+        self.code = code
+
+
+def check(rc):
+    if rc != BT_OK:
+        msg = load().bt_last_error()
+        raise BtError(rc, msg.decode() if msg else "")
+    return rc

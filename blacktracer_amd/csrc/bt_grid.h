@@ -1,0 +1,33 @@
+// bt_grid.h -- the device-resident adaptive grid (one camera pose).
+#pragma once
+
+#include "bt_internal.h"
+
+namespace bt {
+
+// Dense row-major grid arrays in HBM (cell (i, j) at i*M + j).
+struct GridK {
+    double* cel;        // 2 per cell: theta', phi'
+    int32_t* steps;
+    int8_t* level;
+    uint32_t* present;  // bit per cell
+    int N, M, eq, max_level, start_level;
+};
+
+int grid_alloc(Context* c, bt_grid* g);
+int grid_build(Context* c, bt_grid* g, const bt_grid_params* gp);
+int grid_fix_tvertices(Context* c, const GridK& G);
+void grid_release(bt_grid* g);
+
+}  // namespace bt
+
+struct bt_grid {
+    bt::Context* ctx = nullptr;
+    int N = 0, M = 0, eq = 0, max_level = 0, start_level = 0;
+    int64_t rays = 0;
+    size_t bytes = 0;
+    bt::GridK k{};
+    bt::PoseK pose{};
+    bt::ParamsK prm{};
+    bt_trace_stats stats{};
+};

@@ -1,0 +1,472 @@
+// grid.hip -- device-resident adaptive grid build: the drop-in for
+// Grid::Grid(maxLevel, startLevel, angle, cam, bh) (Grid.h:605-624).
+//
+// The grid lives in HBM as dense row-major arrays (cell (i, j) at i*M + j):
+//   cel     double2  celestial (theta', phi'); (-2,-2) = never traced
+//   steps   int32    step index of the last Cash-Karp attempt
+//   level   int8     level of the block whose key is (i, j), -1 = none
+//   present bitmask  the reference's CamToCel key set
+// Per refinement level (Grid.h:465-505) one kernel evaluates refineCheck for
+// every block of the level, claims the five new points of a refined block
+// with an atomicOr on the bitmask (the dedupe of fillVector, Grid.h:453-459)
+// and appends the four children; the ray kernels then trace the claimed
+// points straight into the grid.  The refinement set does not depend on
+// the order in which blocks are visited, so it equals the reference's.
+//
+// fixTvertices (Grid.h:103-177) is order dependent (the reference iterates
+// an unordered_map).  The pinned order is level ascending, then block key
+// ascending.  Within one level, fixes on different horizontal lines touch
+// disjoint points, and so do fixes on different vertical lines except that
+// a meridian c and its antipode c + M/2 meet through the pole branches of
+// checkAdjacentBlock (Grid.h:146-153).  So one thread walks each horizontal
+// line, and one thread each meridian pair, in key order -- which reproduces
+// the sequential pinned order exactly.
+//
+// Compiled with -ffp-contract=off (refinement compares and Hermite
+// midpoints round as in the reference).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bt_grid.h"
+
+namespace bt {
+namespace {
+
+constexpr double PI = 3.141592653589793238;
+constexpr double PI2 = 3.141592653589793238 * 2.;
+
+__device__ __forceinline__ bool present_at(const GridK& G, uint32_t i, uint32_t j) {
+    if (i >= (uint32_t)G.N || j >= (uint32_t)G.M) return false;
+    uint32_t idx = i * (uint32_t)G.M + j;
+    return (G.present[idx >> 5] >> (idx & 31)) & 1u;
+}
+
+__device__ __forceinline__ void set_present(const GridK& G, uint32_t idx) {
+    atomicOr(&G.present[idx >> 5], 1u << (idx & 31));
+}
+
+__global__ void init_grid(GridK G, int64_t nm, int64_t nwords) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = t; i < nm; i += stride) {
+        G.cel[2 * i] = -2.0;
+        G.cel[2 * i + 1] = -2.0;
+        G.steps[i] = 0;
+        G.level[i] = -1;
+    }
+    for (int64_t i = t; i < nwords; i += stride) G.present[i] = 0u;
+}
+
+__global__ void zero_u32(unsigned* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+__global__ void pole_keys(uint32_t* keys, uint32_t south) {
+    if (threadIdx.x == 0) {
+        keys[0] = 0u;
+        keys[1] = south;
+    }
+}
+
+// raytrace(), Grid.h:303-315: copy the pole values along the pole rows at
+// the start-level spacing and queue the pole-row blocks.
+__global__ void pole_rows(GridK G, int gap, uint32_t* blocks) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t ncols = (uint32_t)G.M / (uint32_t)gap;
+    if (t >= ncols) return;
+    uint32_t j = t * (uint32_t)gap;
+    uint32_t M = (uint32_t)G.M;
+    G.cel[2 * j] = G.cel[0];
+    G.cel[2 * j + 1] = G.cel[1];
+    G.steps[j] = G.steps[0];
+    set_present(G, j);
+    blocks[t] = j;
+    if (G.eq) {
+        uint32_t s = (uint32_t)(G.N - 1) * M;
+        G.cel[2 * (s + j)] = G.cel[2 * s];
+        G.cel[2 * (s + j) + 1] = G.cel[2 * s + 1];
+        G.steps[s + j] = G.steps[0];   // the north pole's count (Grid.h:312)
+        set_present(G, s + j);
+    }
+}
+
+// integrateFirst(gap), Grid.h:325-338.
+__global__ void start_lattice(GridK G, int gap, uint32_t nrows, uint32_t ncols, uint32_t* trace,
+                              uint32_t* blocks, int mark_level) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrows * ncols) return;
+    uint32_t i = (uint32_t)gap * (t / ncols + 1);
+    uint32_t j = (uint32_t)gap * (t % ncols);
+    uint32_t idx = i * (uint32_t)G.M + j;
+    trace[t] = idx;
+    set_present(G, idx);
+    if (i != (uint32_t)(G.N - 1)) {
+        if (mark_level >= 0)
+            G.level[idx] = (int8_t)mark_level;
+        else
+            blocks[t] = idx;   // the excluded row is the last one, so t stays dense
+    }
+}
+
+__device__ __forceinline__ void claim(const GridK& G, uint32_t i, uint32_t j, uint32_t* trace,
+                                      unsigned* ntrace) {
+    uint32_t idx = i * (uint32_t)G.M + j;
+    uint32_t bit = 1u << (idx & 31);
+    uint32_t old = atomicOr(&G.present[idx >> 5], bit);
+    if (!(old & bit)) trace[atomicAdd(ntrace, 1u)] = idx;
+}
+
+// adaptiveBlockIntegration's loop body (Grid.h:476-497) with refineCheck
+// (Grid.h:410-445) for one level.
+__global__ void refine_level(GridK G, const uint32_t* blocks, unsigned nb, int level,
+                             double thresh, uint32_t* next_blocks, uint32_t* trace,
+                             unsigned* counters) {
+    unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nb) return;
+    const uint32_t M = (uint32_t)G.M;
+    uint32_t key = blocks[q];
+    uint32_t gap = 1u << (G.max_level - level);
+    uint32_t i = key / M;
+    uint32_t j = key % M;
+    uint32_t k = i + gap / 2;
+    uint32_t l = j + gap / 2;
+    j = j % M;
+    // refineCheck(i, j, gap, level)
+    uint32_t kk = i + gap;
+    uint32_t ll = (j + gap) % M;
+    const double* c = G.cel;
+    double th1 = c[2 * (i * M + j)], th2 = c[2 * (kk * M + j)];
+    double th3 = c[2 * (i * M + ll)], th4 = c[2 * (kk * M + ll)];
+    double ph1 = c[2 * (i * M + j) + 1], ph2 = c[2 * (kk * M + j) + 1];
+    double ph3 = c[2 * (i * M + ll) + 1], ph4 = c[2 * (kk * M + ll) + 1];
+    double diag = (th1 - th4) * (th1 - th4) + (ph1 - ph4) * (ph1 - ph4);
+    double diag2 = (th2 - th3) * (th2 - th3) + (ph2 - ph3) * (ph2 - ph3);
+    double mx = (diag < diag2) ? diag2 : diag;   // std::max
+    bool refine = (level < 6 && mx > 1E-10) || (mx > thresh);
+    if (!refine) {
+        G.level[i * M + j] = (int8_t)level;
+        return;
+    }
+    claim(G, k, j, trace, &counters[0]);
+    claim(G, k, l, trace, &counters[0]);
+    claim(G, i, l, trace, &counters[0]);
+    claim(G, i + gap, l, trace, &counters[0]);
+    claim(G, k, (j + gap) % M, trace, &counters[0]);
+    unsigned base = atomicAdd(&counters[1], 4u);
+    next_blocks[base + 0] = i * M + j;
+    next_blocks[base + 1] = k * M + j;
+    next_blocks[base + 2] = k * M + l;
+    next_blocks[base + 3] = i * M + l;
+}
+
+// Grid.h:503-504
+__global__ void final_level(GridK G, const uint32_t* blocks, unsigned nb, int level) {
+    unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nb) G.level[blocks[q]] = (int8_t)level;
+}
+
+// ---------------------------------------------------------------- T-vertices
+
+// Grid.h:183-204 with tension = bias = 0, on one component.
+__device__ __forceinline__ double hermite1(double v, double x0, double x1, double x2, double x3) {
+    double const v2 = v * v;
+    double const v3 = v * v2;
+    double const aa = (1.0 + 0.0) * (1.0 - 0.0) / 2.0;
+    double const bb = (1.0 - 0.0) * (1.0 - 0.0) / 2.0;
+    double const u0 = 2.0 * v3 - 3.0 * v2 + 1.0;
+    double const u1 = v3 - 2.0 * v2 + v;
+    double const u2 = v3 - v2;
+    double const u3 = -2.0 * v3 + 3.0 * v2;
+    double m0 = (x1 - x0) * aa + (x2 - x1) * bb;
+    double m1 = (x2 - x1) * aa + (x3 - x2) * bb;
+    return x1 * u0 + m0 * u1 + m1 * u2 + x2 * u3;
+}
+
+// Metric.h:333-364 with factor 5 on n phi values.
+__device__ __forceinline__ void fix_2pi(double* p, int n) {
+    const double hi = PI2 * (1. - 1. / (double)5.f);
+    const double lo = PI2 * (1. / (double)5.f);
+    bool cross = false;
+    for (int q = 0; q < n; q++)
+        if (p[q] > hi) cross = true;
+    if (!cross) return;
+    for (int q = 0; q < n; q++)
+        if (p[q] < lo) p[q] += PI2;
+}
+
+struct Edge {
+    uint32_t ii, jj, ii2, jj2;
+    int level, gap;
+};
+
+// checkAdjacentBlock (Grid.h:132-177): the reference's pre-order recursion
+// run from an explicit stack.
+__device__ void check_adjacent(const GridK& G, Edge root, int udlr) {
+    const uint32_t M = (uint32_t)G.M, N = (uint32_t)G.N;
+    double* cel = G.cel;
+    Edge st[40];
+    int sp = 0;
+    st[sp++] = root;
+    while (sp > 0) {
+        Edge f = st[--sp];
+        uint32_t i = f.ii + udlr * f.gap / 2;
+        uint32_t j = f.jj + (1 - udlr) * f.gap / 2;
+        if (!present_at(G, i, j)) continue;
+        uint32_t jprev = (f.jj - (1 - udlr) * f.gap + M) % M;
+        uint32_t jnext = (f.jj + (1 - udlr) * 2 * f.gap) % M;
+        uint32_t iprev = f.ii - udlr * f.gap;
+        uint32_t inext = f.ii + 2 * udlr * f.gap;
+        bool half = false;
+        if (f.ii == 0) {
+            jprev = (f.jj + M / 2) % M;
+            iprev = (uint32_t)f.gap;
+        } else if (inext > N - 1) {
+            inext = f.ii;
+            if (G.eq)
+                jnext = (f.jj + M / 2) % M;
+            else
+                half = true;
+        }
+        const uint32_t mid = i * M + j;
+        bool succes = false;
+        if (present_at(G, iprev, jprev) && present_at(G, inext, jnext)) {
+            uint32_t ip = iprev * M + jprev, in = inext * M + jnext;
+            uint32_t i1 = f.ii * M + f.jj, i2 = f.ii2 * M + f.jj2;
+            double t0 = cel[2 * ip], p0 = cel[2 * ip + 1];
+            double t1 = cel[2 * i1], p1 = cel[2 * i1 + 1];
+            double t2 = cel[2 * i2], p2 = cel[2 * i2 + 1];
+            double t3 = cel[2 * in], p3 = cel[2 * in + 1];
+            if ((t0 != -1 || p0 != -1) && (t3 != -1 || p3 != -1)) {
+                succes = true;
+                if (half) t3 = PI - t3;
+                double ph[4] = {p0, p1, p2, p3};
+                fix_2pi(ph, 4);
+                cel[2 * mid] = hermite1(0.5, t0, t1, t2, t3);
+                cel[2 * mid + 1] = hermite1(0.5, ph[0], ph[1], ph[2], ph[3]);
+            }
+        }
+        if (!succes) {
+            uint32_t i1 = f.ii * M + f.jj, i2 = f.ii2 * M + f.jj2;
+            double t0 = cel[2 * i1], t1 = cel[2 * i2];
+            double ph[2] = {cel[2 * i1 + 1], cel[2 * i2 + 1]};
+            fix_2pi(ph, 2);
+            cel[2 * mid] = 1. / 2. * (t1 + t0);
+            cel[2 * mid + 1] = 1. / 2. * (ph[1] + ph[0]);
+        }
+        if (f.level + 1 == G.max_level) continue;
+        if (sp + 2 > 40) continue;   // unreachable: depth <= max_level
+        st[sp++] = Edge{i, j, f.ii2, f.jj2, f.level + 1, f.gap / 2};
+        st[sp++] = Edge{f.ii, f.jj, i, j, f.level + 1, f.gap / 2};
+    }
+}
+
+__device__ __forceinline__ bool block_fixable(const GridK& G, uint32_t i, uint32_t j, int L) {
+    uint32_t idx = i * (uint32_t)G.M + j;
+    return G.level[idx] == L && !(G.cel[2 * idx + 1] < 0);
+}
+
+// fixTvertices (Grid.h:103-119) for every block of level L, in key order
+// along each line (see the header comment).
+__global__ void fix_level(GridK G, int L, unsigned nrows, unsigned npairs) {
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t M = (uint32_t)G.M, N = (uint32_t)G.N;
+    const int gap = 1 << (G.max_level - L);
+    const uint32_t ug = (uint32_t)gap;
+    if (t < nrows) {
+        uint32_t y = t * ug;
+        if (y >= ug) {
+            // bottom edges (k_j, k_l) of blocks keyed on row y - gap
+            for (uint32_t j = 0; j < M; j += ug)
+                if (block_fixable(G, y - ug, j, L))
+                    check_adjacent(G, Edge{y, j, y, (j + ug) % M, L, gap}, 0);
+        }
+        if (y + ug <= N - 1) {
+            // top edges (ij, i_l) of blocks keyed on row y
+            for (uint32_t j = 0; j < M; j += ug)
+                if (block_fixable(G, y, j, L))
+                    check_adjacent(G, Edge{y, j, y, (j + ug) % M, L, gap}, 0);
+        }
+    } else if (t - nrows < npairs) {
+        uint32_t c = (t - nrows) * ug;
+        uint32_t c2 = c + M / 2;
+        uint32_t cand[4] = {c, (c - ug + M) % M, c2, (c2 - ug + M) % M};
+        // ascending j (4-element sort network)
+#define BT_SWAP(x, y)                   \
+    if (cand[x] > cand[y]) {            \
+        uint32_t tmp_ = cand[x];        \
+        cand[x] = cand[y];              \
+        cand[y] = tmp_;                 \
+    }
+        BT_SWAP(0, 1) BT_SWAP(2, 3) BT_SWAP(0, 2) BT_SWAP(1, 3) BT_SWAP(1, 2)
+#undef BT_SWAP
+        for (uint32_t i = 0; i + ug <= N - 1; i += ug) {
+            for (int s = 0; s < 4; s++) {
+                uint32_t j = cand[s];
+                if (!block_fixable(G, i, j, L)) continue;
+                if (j == c || j == c2) {
+                    // left edge (ij, k_j)
+                    check_adjacent(G, Edge{i, j, i + ug, j, L, gap}, 1);
+                } else {
+                    // right edge (i_l, k_l)
+                    uint32_t l = (j + ug) % M;
+                    check_adjacent(G, Edge{i, l, i + ug, l, L, gap}, 1);
+                }
+            }
+        }
+    }
+}
+
+unsigned blocks_for(int64_t n, int tpb) { return (unsigned)((n + tpb - 1) / tpb); }
+
+}  // namespace
+
+int grid_fix_tvertices(Context* c, const GridK& G) {
+    for (int L = G.start_level; L < G.max_level; L++) {
+        int gap = 1 << (G.max_level - L);
+        unsigned nrows = (unsigned)((G.N - 1) / gap + 1);
+        unsigned npairs = (unsigned)((G.M / 2) / gap);
+        unsigned n = nrows + npairs;
+        KTimer t(c, "fix_tvertices");
+        fix_level<<<blocks_for(n, 64), 64, 0, c->stream>>>(G, L, nrows, npairs);
+        BT_HIP(hipGetLastError());
+    }
+    return BT_OK;
+}
+
+int grid_build(Context* c, bt_grid* g, const bt_grid_params* gp) {
+    const int N = g->N, M = g->M, eq = g->eq;
+    const int max_level = g->max_level, start_level = g->start_level;
+    const int64_t NM = (int64_t)N * M;
+    GridK& G = g->k;
+    int rc;
+    void* p;
+    const double thresh = gp ? gp->refine_thresh : 0.015;
+    const int fix = gp ? gp->fix_tvertices : 1;
+
+    if ((rc = c->buffer("grid_blocks_a", sizeof(uint32_t) * (NM + 64), &p))) return rc;
+    uint32_t* blocks = (uint32_t*)p;
+    if ((rc = c->buffer("grid_blocks_b", sizeof(uint32_t) * (NM + 64), &p))) return rc;
+    uint32_t* next = (uint32_t*)p;
+    if ((rc = c->buffer("grid_trace", sizeof(uint32_t) * (NM + 64), &p))) return rc;
+    uint32_t* trace_keys = (uint32_t*)p;
+    if ((rc = c->buffer("grid_ctr", 64, &p))) return rc;
+    unsigned* ctr = (unsigned*)p;
+    if ((rc = c->buffer("grid_stats", sizeof(StatsK), &p))) return rc;
+    StatsK* d_stats = (StatsK*)p;
+    BT_HIP(hipMemsetAsync(d_stats, 0, sizeof(StatsK), c->stream));
+
+    const int64_t nwords = (NM + 31) / 32;
+    init_grid<<<(unsigned)std::min<int64_t>(blocks_for(NM, 256), 4096), 256, 0, c->stream>>>(G, NM, nwords);
+    BT_HIP(hipGetLastError());
+
+    RayIO io;
+    std::memset(&io, 0, sizeof(io));
+    io.N = N;
+    io.M = M;
+    io.eq = eq;
+    io.out_th = G.cel;
+    io.out_ph = G.cel + 1;
+    io.ostride = 2;
+    io.out_step = G.steps;
+    g->rays = 0;
+
+    // raytrace(): the pole rays (Grid.h:295-301)
+    pole_keys<<<1, 64, 0, c->stream>>>(trace_keys, (uint32_t)(N - 1) * (uint32_t)M);
+    BT_HIP(hipGetLastError());
+    io.keys = trace_keys;
+    if ((rc = trace(c, g->pose, g->prm, 1 + eq, io, d_stats))) return rc;
+    g->rays += 1 + eq;
+    // (stream order: the pole keys are consumed before the buffer is reused)
+    const int gap0 = 1 << (max_level - start_level);
+    const unsigned npole = (unsigned)(M / gap0);
+    pole_rows<<<blocks_for(npole, 256), 256, 0, c->stream>>>(G, gap0, blocks);
+    BT_HIP(hipGetLastError());
+
+    // integrateFirst(gap)
+    unsigned nrows = 0;
+    for (int i = gap0; i < N - eq; i += gap0) nrows++;
+    const unsigned ncols = (unsigned)(M / gap0);
+    const int64_t nlat = (int64_t)nrows * ncols;
+    const bool uniform = (max_level == start_level);
+    start_lattice<<<blocks_for(nlat, 256), 256, 0, c->stream>>>(
+        G, gap0, nrows, ncols, trace_keys, blocks + npole, uniform ? start_level : -1);
+    BT_HIP(hipGetLastError());
+    if ((rc = trace(c, g->pose, g->prm, nlat, io, d_stats))) return rc;
+    g->rays += nlat;
+    unsigned nb = npole;
+    if (!uniform) {
+        unsigned lat_blocks = nrows * ncols;
+        if (!eq) lat_blocks -= ncols;   // row N-1 is traced but is no block
+        nb += lat_blocks;
+    }
+
+    // adaptiveBlockIntegration(startLevel)
+    int level = start_level;
+    bool early = false;
+    while (level < max_level) {
+        if (nb == 0) {
+            early = true;
+            break;
+        }
+        zero_u32<<<1, 64, 0, c->stream>>>(ctr, 2);
+        {
+            KTimer t(c, "refine_level");
+            refine_level<<<blocks_for(nb, 256), 256, 0, c->stream>>>(G, blocks, nb, level, thresh,
+                                                                      next, trace_keys, ctr);
+        }
+        BT_HIP(hipGetLastError());
+        unsigned h_ctr[2];
+        BT_HIP(hipMemcpyAsync(h_ctr, ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, c->stream));
+        BT_HIP(hipStreamSynchronize(c->stream));
+        if ((rc = trace(c, g->pose, g->prm, h_ctr[0], io, d_stats))) return rc;
+        g->rays += h_ctr[0];
+        level++;
+        std::swap(blocks, next);
+        nb = h_ctr[1];
+    }
+    if (!early && nb > 0) {
+        final_level<<<blocks_for(nb, 256), 256, 0, c->stream>>>(G, blocks, nb, level);
+        BT_HIP(hipGetLastError());
+    }
+    if (fix) {
+        if ((rc = grid_fix_tvertices(c, G))) return rc;
+    }
+    StatsK s;
+    BT_HIP(hipMemcpyAsync(&s, d_stats, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    BT_HIP(hipStreamSynchronize(c->stream));
+    stats_to_abi(s, &g->stats);
+    return BT_OK;
+}
+
+int grid_alloc(Context* c, bt_grid* g) {
+    const int64_t NM = (int64_t)g->N * g->M;
+    const int64_t nwords = (NM + 31) / 32;
+    GridK& G = g->k;
+    G.N = g->N;
+    G.M = g->M;
+    G.eq = g->eq;
+    G.max_level = g->max_level;
+    G.start_level = g->start_level;
+    BT_HIP(hipMalloc((void**)&G.cel, sizeof(double) * 2 * NM));
+    BT_HIP(hipMalloc((void**)&G.steps, sizeof(int32_t) * NM));
+    BT_HIP(hipMalloc((void**)&G.level, sizeof(int8_t) * NM));
+    BT_HIP(hipMalloc((void**)&G.present, sizeof(uint32_t) * nwords));
+    g->bytes = sizeof(double) * 2 * NM + sizeof(int32_t) * NM + NM + sizeof(uint32_t) * nwords;
+    (void)c;
+    return BT_OK;
+}
+
+void grid_release(bt_grid* g) {
+    GridK& G = g->k;
+    if (G.cel) hipFree(G.cel);
+    if (G.steps) hipFree(G.steps);
+    if (G.level) hipFree(G.level);
+    if (G.present) hipFree(G.present);
+    G.cel = nullptr;
+    G.steps = nullptr;
+    G.level = nullptr;
+    G.present = nullptr;
+}
+
+}  // namespace bt

@@ -1,0 +1,258 @@
+// kerr_dev.h -- device-side Kerr geodesic building blocks in FP64:
+// initial conditions, a-priori capture test, Hamiltonian right-hand side
+// and one Cash-Karp attempt with the reference's step control.
+//
+// Included only by translation units compiled with -ffp-contract=off, so
+// that every expression rounds exactly as the reference's host code does
+// (Metric.h is compiled with /fp:precise).  Expression order below is kept
+// operation for operation; do not reassociate.
+#pragma once
+
+#include "bt_internal.h"
+
+namespace bt {
+namespace kerr {
+
+constexpr double PI = 3.141592653589793238;   // Const.h:2
+constexpr double PI2 = 3.141592653589793238 * 2.;
+
+// Metric.h:27-33
+constexpr double SAFETY = 0.9;
+constexpr double PGROW = -0.2;
+constexpr double PSHRNK = -0.25;
+constexpr double ERRCON = 1.89e-4;
+constexpr double TINY = 1.0e-30;
+constexpr double ADAPTIVE = 5.0;
+
+// Cash-Karp embedded RK4(5) tableau, Metric.h:41-49.
+constexpr double b21 = 0.2, b31 = 3.0 / 40.0, b32 = 9.0 / 40.0, b41 = 0.3, b42 = -0.9,
+                 b43 = 1.2, b51 = -11.0 / 54.0, b52 = 2.5, b53 = -70.0 / 27.0,
+                 b54 = 35.0 / 27.0, b61 = 1631.0 / 55296.0, b62 = 175.0 / 512.0,
+                 b63 = 575.0 / 13824.0, b64 = 44275.0 / 110592.0, b65 = 253.0 / 4096.0,
+                 c1 = 37.0 / 378.0, c3 = 250.0 / 621.0, c4 = 125.0 / 594.0,
+                 c6 = 512.0 / 1771.0, dc5 = -277.00 / 14336.0;
+constexpr double dc1 = 37.0 / 378.0 - 2825.0 / 27648.0, dc3 = 250.0 / 621.0 - 18575.0 / 48384.0,
+                 dc4 = 125.0 / 594.0 - 13525.0 / 55296.0, dc6 = 512.0 / 1771.0 - 0.25;
+
+__device__ __forceinline__ double sq(double x) { return x * x; }
+__device__ __forceinline__ double sq3(double x) { return x * x * x; }
+
+// R(r) of Metric.h:96-98 with P and Delta of Metric.h:64-66,92-94.
+__device__ __forceinline__ double Rfun(double a, double asq, double r, double b, double q) {
+    double P = sq(r) + asq - a * b;
+    double D = sq(r) - 2. * r + asq;
+    return sq(P) - D * (sq((b - a)) + q);
+}
+
+// Metric.h:133-143
+__device__ __forceinline__ double b0(double a, double asq, double r0) {
+    return -(sq3(r0) - 3. * sq(r0) + asq * r0 + asq) / (a * (r0 - 1.));
+}
+__device__ __forceinline__ double b0diff(double a, double asq, double r0) {
+    return (asq + a - 2. * r0 * (sq(r0) - 3. * r0 + 3.)) / (a * sq(r0 - 1.));
+}
+__device__ __forceinline__ double q0(double asq, double r0) {
+    return -sq3(r0) * (sq3(r0) - 6. * sq(r0) + 9. * r0 - 4. * asq) / (asq * sq(r0 - 1.));
+}
+
+// Grid.h:524-547: camera-sky direction -> (pR, pTheta, b, q).
+__device__ __forceinline__ void initial_conditions(const PoseK& P, double theta, double phi,
+                                                   double& pR, double& pTheta, double& b,
+                                                   double& q) {
+    double st, ct, sp, cp;
+    sincos(theta, &st, &ct);
+    sincos(phi, &sp, &cp);
+    double xCam = st * cp;
+    double yCam = st * sp;
+    double zCam = ct;
+    double yFido = (-yCam + P.speed) / (1 - P.speed * yCam);
+    double xFido = P.neg_s * xCam / (1 - P.speed * yCam);
+    double zFido = P.neg_s * zCam / (1 - P.speed * yCam);
+    double rFido = xFido * P.bphi / P.k + P.br * yFido + P.brbt_k * zFido;
+    double thetaFido = P.btheta * yFido - P.k * zFido;
+    double phiFido = -xFido * P.br / P.k + P.bphi * yFido + P.bpbt_k * zFido;
+    double eF = 1. / (P.alpha + P.ww * phiFido);
+    pR = eF * P.ro * rFido / P.sqrtD;
+    pTheta = eF * P.ro * thetaFido;
+    double pPhi = eF * P.wbar * phiFido;
+    b = pPhi;
+    q = pTheta * pTheta + P.cos2S * (b * b / P.sin2S - P.asq);
+}
+
+// Metric.h:145-167 (Newton loop bounded by newton_cap).
+__device__ __forceinline__ bool check_bq(const PoseK& P, double bV, double qV, int cap,
+                                         unsigned& iters) {
+    const double error = 0.0000001;
+    double r0V = 2.0;
+    double bcheck = 100;
+    int it = 0;
+    while (fabs(bV - bcheck) > error) {
+        if (it >= cap) break;
+        bcheck = b0(P.a, P.asq, r0V);
+        double bdiffcheck = b0diff(P.a, P.asq, r0V);
+        double rnew = r0V - (bcheck - bV) / bdiffcheck;
+        r0V = (rnew < 1) ? 1.0001 : rnew;
+        it++;
+    }
+    iters += it;
+    double qb = q0(P.asq, r0V);
+    return (P.b1 >= bV) || (P.b2 <= bV) || (qV >= qb);
+}
+
+// Metric.h:110-131
+__device__ __forceinline__ bool check_rup(const PoseK& P, double rV, double bV, double qV,
+                                          unsigned& iters) {
+    if (P.a_zero) return false;
+    double ax = rV, bb = 4 * rV;
+    const double tol = 0.00001;
+    const double gr = P.gr;
+    double c = bb - (bb - ax) / gr;
+    double d = ax + (bb - ax) / gr;
+    int it = 0;
+    while (fabs(c - d) > tol) {
+        if (Rfun(P.a, P.asq, c, bV, qV) < Rfun(P.a, P.asq, d, bV, qV))
+            bb = d;
+        else
+            ax = c;
+        c = bb - (bb - ax) / gr;
+        d = ax + (bb - ax) / gr;
+        it++;
+    }
+    iters += it;
+    double mn = (ax + bb) / 2;
+    return Rfun(P.a, P.asq, mn, bV, qV) >= 0;
+}
+
+// Metric.h:169-174: true when the ray escapes (is integrated).
+__device__ __forceinline__ bool check_celest(const PoseK& P, double pR, double bV, double qV,
+                                             int cap, unsigned& newton, unsigned& golden) {
+    bool check1 = check_bq(P, bV, qV, cap, newton);
+    bool check2 = !check1 && (pR < 0);
+    bool check3 = check1 && check_rup(P, P.r, bV, qV, golden);
+    return check2 || check3;
+}
+
+// Metric.h:198-226.  v = (r, theta, phi, pR, pTheta).
+__device__ __forceinline__ void derivs(double a, double asq, const double* v, double* o,
+                                       double b, double q) {
+    const double r = v[0], pr = v[3], pt = v[4];
+    double sinv, cosv;
+    sincos(v[1], &sinv, &cosv);
+    double cossq = sq(cosv);
+    double sinsq = sq(sinv);
+    double bsq = sq(b);
+    double delta = sq(r) - 2. * r + asq;
+    double rosq = sq(r) + asq * cossq;
+    double P = sq(r) + asq - a * b;
+    double prsq = sq(pr);
+    double pthetasq = sq(pt);
+    double R = sq(P) - delta * (sq((b - a)) + q);
+    double partR = (q + sq(a - b));
+    double btheta = q - cossq * (bsq / sinsq - asq);
+    double rosqsq = sq(2 * rosq);
+    double sqrosqdel = (sq(rosq) * delta);
+    double asqcossin = asq * cosv * sinv;
+    double rtwo = 2 * r - 2;
+
+    o[0] = delta / rosq * pr;
+    o[1] = 1. / rosq * pt;
+    o[2] = (2 * a * P - (2 * a - 2 * b) * delta + (2 * b * cossq * delta) / sinsq) / (rosq * 2 * delta);
+    o[3] = (rtwo * btheta - rtwo * partR + 4 * r * P) / (rosq * (2 * delta)) - (prsq * rtwo) / (2 * rosq)
+        + (4 * pthetasq * r) / rosqsq - ((4 * r - 4) * (btheta * (delta) + R)) / (rosq * sq(2 * delta))
+        + (4 * prsq * r * (delta)) / rosqsq - (r * (btheta * delta + R)) / sqrosqdel;
+    o[4] = ((2 * cosv * sinv * (bsq / sinsq - asq) + (2 * bsq * sq3(cosv)) / sq3(sinv)) * delta) /
+        (rosq * 2 * delta) - (4 * asqcossin * pthetasq) / rosqsq - (4 * asqcossin * prsq * delta) /
+        rosqsq + (asqcossin * (btheta * delta + R)) / sqrosqdel;
+}
+
+// odeint1's loop body (Metric.h:292-297): derivs, scale, rkqs (Metric.h:253-270)
+// with rkck (Metric.h:228-251).  Updates var/z/h in place.
+__device__ __forceinline__ void attempt(double a, double asq, double* var, double& z, double& h,
+                                        double eps, double b, double q) {
+    double dvdz[5], scal[5], k2[5], k3[5], k4[5], k5[5], k6[5], t[5];
+    derivs(a, asq, var, dvdz, b, q);
+#pragma unroll
+    for (int i = 0; i < 5; i++) scal[i] = fabs(var[i]) + fabs(dvdz[i] * h) + TINY;
+    const double hh = h;
+#pragma unroll
+    for (int i = 0; i < 5; i++) t[i] = var[i] + b21 * hh * dvdz[i];
+    derivs(a, asq, t, k2, b, q);
+#pragma unroll
+    for (int i = 0; i < 5; i++) t[i] = var[i] + hh * (b31 * dvdz[i] + b32 * k2[i]);
+    derivs(a, asq, t, k3, b, q);
+#pragma unroll
+    for (int i = 0; i < 5; i++) t[i] = var[i] + hh * (b41 * dvdz[i] + b42 * k2[i] + b43 * k3[i]);
+    derivs(a, asq, t, k4, b, q);
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        t[i] = var[i] + hh * (b51 * dvdz[i] + b52 * k2[i] + b53 * k3[i] + b54 * k4[i]);
+    derivs(a, asq, t, k5, b, q);
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        t[i] = var[i] + hh * (b61 * dvdz[i] + b62 * k2[i] + b63 * k3[i] + b64 * k4[i] + b65 * k5[i]);
+    derivs(a, asq, t, k6, b, q);
+    double errmax = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        t[i] = var[i] + hh * (c1 * dvdz[i] + c3 * k3[i] + c4 * k4[i] + c6 * k6[i]);
+        double err = hh * (dc1 * dvdz[i] + dc3 * k3[i] + dc4 * k4[i] + dc5 * k5[i] + dc6 * k6[i]);
+        errmax = fmax(errmax, fabs(err / scal[i]));
+    }
+    errmax /= eps;
+    if (errmax <= 1.0) {
+        z += h;
+#pragma unroll
+        for (int i = 0; i < 5; i++) var[i] = t[i];
+        if (errmax > ERRCON)
+            h = SAFETY * h * pow(errmax, PGROW);
+        else
+            h = ADAPTIVE * h;
+    } else {
+        h = fmin(SAFETY * h * pow(errmax, PSHRNK), 0.1 * h);
+    }
+}
+
+// Metric.h:176-185 (the while loops are kept: repeated addition of 2*PI
+// rounds differently from a single fmod).
+__device__ __forceinline__ void wrap_to_pi(double& thetaW, double& phiW) {
+    thetaW = fmod(thetaW, PI2);
+    while (thetaW < 0) thetaW += PI2;
+    if (thetaW > PI) {
+        thetaW -= 2 * (thetaW - PI);
+        phiW += PI;
+    }
+    while (phiW < 0) phiW += PI2;
+    phiW = fmod(phiW, PI2);
+}
+
+// Camera-sky direction of ray k (Grid.h:386-390 for grid keys).
+__device__ __forceinline__ void ray_direction(const RayIO& io, int64_t k, double& th, double& ph) {
+    if (io.keys) {
+        uint32_t key = io.keys[k];
+        uint32_t i = key / (uint32_t)io.M, j = key % (uint32_t)io.M;
+        th = (double)i / (io.N - 1) * PI / (2 - io.eq);
+        ph = (double)j / io.M * PI2;
+    } else {
+        th = io.in_th[k];
+        ph = io.in_ph[k];
+    }
+}
+
+__device__ __forceinline__ void ray_store(const RayIO& io, int64_t k, double th, double ph, int step) {
+    int64_t o = io.keys ? (int64_t)io.keys[k] : k;
+    io.out_th[o * io.ostride] = th;
+    io.out_ph[o * io.ostride] = ph;
+    io.out_step[o] = step;
+}
+
+// ---- wavefront helpers (wave64) ----
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+}  // namespace kerr
+}  // namespace bt

@@ -1,0 +1,173 @@
+/*
+ * bt_abi.h -- C ABI of blacktracer_amd, the MI355X (gfx950) implementation
+ * of BlackTracer's data-parallel core: the Kerr null-geodesic grid build
+ * (Path A) and the per-pixel render pass (Path B).
+ *
+ * Every entry point returns an int status (BT_OK == 0), never prints,
+ * never exits and never resets the device.  Host buffers are owned by the
+ * caller and borrowed for the duration of the call; device memory is owned
+ * by a bt_context (one per device; calls on one context are serialised by
+ * the context's mutex, calls on different contexts run concurrently).
+ * Spin lives in the pose, not in process globals.
+ *
+ * Reference interfaces replaced (annemiekie/BlackTracer,
+ * BlackTracer2/BlackTracer/):
+ *   bt_camera_init      Camera::Camera(...) + BlackHole::setA   Camera.h:27-61, BlackHole.h:12-15
+ *   bt_trace_rays       Grid::integration_wrapper               Grid.h:514-556
+ *   bt_grid_build       Grid::Grid(maxLevel, startLevel, ...)   Grid.h:605-624
+ *   bt_grid_export_psh  Grid::saveAsGpuHash / PSHOffsetTable    Grid.h:642-654, PSHOffsetTable.h:350-398
+ *   bt_make_image       makeImage(...)                          MakeImageKernel.cu:1206-1236, kernel.cuh:28-33
+ */
+#ifndef BT_ABI_H
+#define BT_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BT_ABI_VERSION 1
+
+/* ---- status codes ---- */
+#define BT_OK              0
+#define BT_ERR_INVALID    -1  /* bad argument (null pointer, size, level range) */
+#define BT_ERR_DEVICE     -2  /* no HIP device / device index out of range */
+#define BT_ERR_HIP        -3  /* a HIP runtime call failed */
+#define BT_ERR_NOMEM      -4  /* device or host allocation failed */
+#define BT_ERR_BUDGET     -5  /* request does not fit the HBM admission budget */
+#define BT_ERR_NOTFOUND   -6  /* handle or cache entry not found */
+#define BT_ERR_STATE      -7  /* call not valid in the object's state */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* bt_last_error(void);
+int         bt_abi_version(void);
+
+/* ---- context (device memory pools, stream, timers) ---- */
+typedef struct bt_context bt_context;
+int  bt_context_create(int device, bt_context** out);
+void bt_context_destroy(bt_context* ctx);
+/* Blocks until all work queued on the context's stream is done. */
+int  bt_context_sync(bt_context* ctx);
+/* Per-kernel timing with HIP events recorded on the context's stream.
+ * enable != 0 switches it on; bt_timer_get returns launches and total
+ * milliseconds for a kernel name (e.g. "rk_f64"). */
+int  bt_timing_enable(bt_context* ctx, int enable);
+int  bt_timer_get(bt_context* ctx, const char* kernel, int64_t* launches, double* total_ms);
+int  bt_timer_reset(bt_context* ctx);
+
+/* ---- pose: Camera(theta, phi, r, br, btheta, bphi) + BlackHole(a) ---- */
+typedef struct {
+    double a;                  /* spin a/M (BlackHole.h:12) */
+    double theta, phi, r;      /* camera Boyer-Lindquist position */
+    double br, btheta, bphi;   /* direction of motion in the FIDO frame */
+    double speed;              /* used only when user_speed != 0 */
+    int    user_speed;         /* 0: speed = calcSpeed(r, theta) (circular orbit) */
+} bt_pose;
+
+/* Derived camera quantities, Camera::initforms (Camera.h:54-61):
+ * out6 = {speed, alpha, w, wbar, Delta, ro}.  cam7 (nullable) receives
+ * getParamArray() (Camera.h:67-78): {speed, alpha, w, wbar, br, btheta, bphi}. */
+int bt_camera_init(const bt_pose* pose, double* out6, float* cam7);
+
+/* ---- integrator parameters ---- */
+#define BT_PREC_FP64  0   /* plain FP64 path: the drop-in */
+#define BT_PREC_MIXED 1   /* FP32 Cash-Karp bulk, FP64 set-up/fallback/accumulation */
+
+typedef struct {
+    double eps;               /* Metric.h:312 (1e-5) */
+    double h1;                /* Metric.h:313 (0.01) */
+    double z_end;             /* Metric.h:311 (-1e7) */
+    int    max_steps;         /* Metric.h:31 (1000) */
+    int    apriori_capture;   /* 1: checkCelest (Metric.h:169) decides capture */
+    double r_escape;          /* >0: stop as escaped once r > r_escape */
+    double r_capture_factor;  /* >0: stop as captured once r < f * r_horizon */
+    int    newton_cap;        /* cap on checkB_Q's Newton loop (reference: none) */
+    int    precision;         /* BT_PREC_FP64 / BT_PREC_MIXED */
+    double crit_eps;          /* mixed: FP64 for rays with |q - q0(b)| < crit_eps */
+    double hmin_fp64;         /* mixed: switch a ray to FP64 once |h| < hmin_fp64 */
+} bt_integ_params;
+
+void bt_integ_params_default(bt_integ_params* p);
+
+/* Per-call work counters (algorithmic-work tally and fallback fraction). */
+typedef struct {
+    int64_t rays;          /* rays handled */
+    int64_t integrated;    /* rays that entered the Cash-Karp loop */
+    int64_t attempts;      /* Cash-Karp attempts, all precisions */
+    int64_t attempts_fp64; /* attempts done in FP64 */
+    int64_t newton;        /* checkB_Q Newton iterations */
+    int64_t golden;        /* findMinGoldSec iterations */
+    int64_t fallback_rays; /* mixed: rays finished in FP64 */
+} bt_trace_stats;
+
+/* Grid::integration_wrapper (Grid.h:514-556): camera-sky (theta, phi) in,
+ * celestial (theta', phi') out in place, (-1,-1) for captured rays, plus
+ * the step index of the last Cash-Karp attempt.  Host buffers of n. */
+int bt_trace_rays(bt_context* ctx, const bt_pose* pose, const bt_integ_params* prm,
+                  int64_t n, double* theta, double* phi, int32_t* step,
+                  bt_trace_stats* stats);
+/* Same on device buffers, enqueued on the context's stream (asynchronous;
+ * stats, when non-null, forces a synchronisation). */
+int bt_trace_rays_device(bt_context* ctx, const bt_pose* pose, const bt_integ_params* prm,
+                         int64_t n, double* d_theta, double* d_phi, int32_t* d_step,
+                         bt_trace_stats* stats);
+
+/* ---- adaptive grid: Grid(maxLevel, startLevel, angle, cam, bh) ---- */
+typedef struct bt_grid bt_grid;
+
+typedef struct {
+    double refine_thresh;   /* PRECCELEST, Grid.h:24 (0.015) */
+    int    fix_tvertices;   /* 1: Grid.h:620-622, pinned order level asc, key asc */
+} bt_grid_params;
+
+void bt_grid_params_default(bt_grid_params* p);
+
+int  bt_grid_build(bt_context* ctx, const bt_pose* pose, const bt_integ_params* prm,
+                   int max_level, int start_level, int angle, const bt_grid_params* gp,
+                   bt_grid** out);
+int  bt_grid_info(const bt_grid* g, int* N, int* M, int* max_level, int* start_level,
+                  int* equafactor, int64_t* rays_traced);
+int  bt_grid_stats(const bt_grid* g, bt_trace_stats* stats);
+/* Host copies of the dense grid, row-major (i*M + j):
+ * cel N*M*2 doubles ((-2,-2) = not traced, (-1,-1) = captured),
+ * steps N*M int32, levels N*M int8 (-1 = not a block key). Any may be null. */
+int  bt_grid_export(const bt_grid* g, double* cel, int32_t* steps, int8_t* levels);
+/* Perfect spatial hash of the traced points (PSHOffsetTable layout):
+ * call with null buffers to get the widths, then with buffers of
+ * 2*hw*hw floats / 2*ow*ow ints / 2*hw*hw ints. */
+int  bt_grid_export_psh(const bt_grid* g, int* hash_width, int* offset_width,
+                        float* hash_table, int32_t* offset_table, int32_t* hash_pos_tag);
+void bt_grid_free(bt_grid* g);
+
+/* ---- render: makeImage ---- */
+typedef struct {
+    int   star;        /* star pass (MakeImageKernel.cu:955) */
+    int   map;         /* sky-map pass (:956) */
+    int   lr;          /* lensing + redshift (:957) */
+    float hor, ver;    /* view offsets (:959-960) */
+    float offset;      /* celestial phi offset of the frame (:1413) */
+} bt_render_flags;
+
+void bt_render_flags_default(bt_render_flags* f);
+
+/* makeImage (MakeImageKernel.cu:1206) with cv::Mat csImage replaced by a
+ * BGR8 buffer and the PNG written by the reference replaced by out_bgra
+ * (N*M*4 bytes, B,G,R,255 per pixel).  Arrays of int2/float2 are passed as
+ * interleaved pairs.  G must be 1 (multi-grid blending is not built). */
+int bt_make_image(bt_context* ctx, const float* stars, const int32_t* star_tree,
+                  int star_size, const float* cam_param, const float* magnitude,
+                  int tree_level, int M, int N, int step, const uint8_t* sky_bgr,
+                  int sky_rows, int sky_cols, int G, float grid_start, float grid_step,
+                  const float* hit, const float* viewthing, float view_angle, int GM,
+                  int GN, int gridlvl, const int32_t* offset_tables,
+                  const float* hash_tables, const int32_t* hash_pos_tag,
+                  const int32_t* table_sizes, int otsize, int htsize,
+                  const bt_render_flags* flags, uint8_t* out_bgra);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BT_ABI_H */

@@ -1,0 +1,127 @@
+/*
+ * bt_oracle.h -- CPU oracle for the BlackTracer hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is a plain-C restatement of the
+ * reference algorithm (annemiekie/BlackTracer, BlackTracer2/BlackTracer/).
+ * It is the checker that the HIP product is compared against, and the
+ * CPU baseline that bench.py times.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it; the product never does.
+ *
+ * Path A (grid build, FP64) follows Metric.h, Camera.h, BlackHole.h and
+ * Grid.h operation for operation, built with -ffp-contract=off.
+ * Path B (render, FP32) follows kernel.cuh and MakeImageKernel.cu.
+ *
+ * Pinning status of every function is listed in the header of
+ * bt_oracle.c and bt_oracle_render.c.
+ */
+#ifndef BT_ORACLE_H
+#define BT_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Pose: BlackHole(a) + Camera(theta, phi, r, br, btheta, bphi)
+ * (BlackHole.h:12-15, Camera.h:40-61).  All derived quantities are
+ * computed by bto_pose_init exactly as Camera::initforms does. */
+typedef struct {
+    double a, asq;
+    double theta, phi, r;
+    double speed, br, btheta, bphi;
+    double alpha, w, wbar, Delta, ro;
+} bto_pose;
+
+/* Integration parameters.  Reference defaults (Metric.h:307-318,27-33):
+ * eps 1e-5, h1 0.01, z_end -1e7, max_steps 1000, a-priori capture on,
+ * no radius stops. */
+typedef struct {
+    double eps;
+    double h1;
+    double z_end;
+    int    max_steps;
+    int    apriori_capture;   /* 1: checkCelest decides capture (reference) */
+    double r_escape;          /* >0: stop as escaped once r > r_escape */
+    double r_capture_factor;  /* >0: stop as captured once r < f * r_horizon */
+    int    newton_cap;        /* iteration cap on checkB_Q's Newton loop */
+} bto_params;
+
+/* Per-ray counters used for the algorithmic-work tally. */
+typedef struct {
+    int64_t attempts;   /* Cash-Karp attempts (derivs + rkqs) */
+    int64_t newton;     /* checkB_Q Newton iterations */
+    int64_t golden;     /* findMinGoldSec iterations */
+    int64_t rays;       /* rays handled */
+    int64_t integrated; /* rays that entered odeint1 */
+} bto_stats;
+
+enum { BTO_ESCAPED = 0, BTO_CAPTURED = 1, BTO_MAXSTEP = 2 };
+
+void   bto_default_params(bto_params* p);
+void   bto_pose_init(bto_pose* p, double a, double theta, double phi, double r,
+                     double br, double btheta, double bphi);
+void   bto_pose_init_speed(bto_pose* p, double a, double theta, double phi,
+                           double r, double speed);
+
+/* Metric.h closed forms (exposed for pinning tests). */
+double bto_calc_speed(double a, double r, double theta);
+double bto_b0(double a, double r0);
+double bto_q0(double a, double r0);
+int    bto_check_bq(double a, double b, double q, int newton_cap, int64_t* iters);
+int    bto_check_rup(double a, double r, double theta, double b, double q, int64_t* iters);
+int    bto_check_celest(double a, double pr, double r, double theta, double b,
+                        double q, int newton_cap, int64_t* newton, int64_t* golden);
+void   bto_derivs(double a, const double* var, double* out, double b, double q);
+void   bto_rkck(double a, const double* var, const double* dvdz, double h,
+                double* out, double* err, double b, double q);
+void   bto_wrap_to_pi(double* theta, double* phi);
+
+/* Initial conditions of one camera-sky direction (Grid.h:524-547):
+ * out[0..3] = pR, pTheta, b, q. */
+void   bto_initial_conditions(const bto_pose* pose, double theta, double phi,
+                              double* out);
+
+/* One ray: integration_wrapper's loop body (Grid.h:524-554).
+ * Returns BTO_ESCAPED/BTO_CAPTURED/BTO_MAXSTEP; (theta, phi) in/out
+ * follow the reference's in-place convention: (-1,-1) when captured. */
+int    bto_trace_ray(const bto_pose* pose, const bto_params* prm,
+                     double* theta, double* phi, int* step, bto_stats* st);
+
+/* Batch: integration_wrapper (Grid.h:514-556), optional OpenMP. */
+void   bto_trace_rays(const bto_pose* pose, const bto_params* prm, int n,
+                      double* theta, double* phi, int* step, int nthreads,
+                      bto_stats* st);
+
+/* Adaptive grid: Grid(maxLevel, startLevel, angle, cam, bh) (Grid.h:605-624)
+ * on dense arrays.  cel is N*M*2 doubles ((-2,-2) = not traced),
+ * steps N*M ints, level N*M int8 (-1 = no block key here).
+ * refine_thresh: PRECCELEST (0.015, Grid.h:24).
+ * fix_tvertices: 1 to run fixTvertices in the pinned order
+ * (level ascending, then key ascending). */
+typedef struct {
+    int N, M, max_level, start_level, equafactor;
+    int64_t rays_traced;
+    double* cel;
+    int*    steps;
+    int8_t* level;
+    uint8_t* present;   /* 1 where CamToCel holds the key */
+} bto_grid;
+
+int    bto_grid_dims(int max_level, int angle, int* N, int* M);
+int    bto_grid_build(const bto_pose* pose, const bto_params* prm, int max_level,
+                      int start_level, int angle, double refine_thresh,
+                      int fix_tvertices, int nthreads, bto_grid* g, bto_stats* st);
+void   bto_grid_fix_tvertices(bto_grid* g);
+void   bto_grid_free(bto_grid* g);
+
+/* Catmull-Rom (tension 0, bias 0) Hermite of Grid.h:183-204 on (theta, phi). */
+void   bto_hermite2(double v, const double* x0, const double* x1, const double* x2,
+                    const double* x3, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

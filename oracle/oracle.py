@@ -1,0 +1,189 @@
+"""ctypes bindings for the CPU oracle (test infrastructure only).
+
+Loaded by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline
+leg -- never by the blacktracer_amd product.  The oracle is a plain-C
+restatement of the reference (annemiekie/BlackTracer) built from
+oracle/bt_oracle*.c; `load_ref()` loads the reference's own Metric.h /
+Camera.h / BlackHole.h compiled into oracle/_ref/libbtref.so.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+DP = C.POINTER(C.c_double)
+FP = C.POINTER(C.c_float)
+IP = C.POINTER(C.c_int)
+I8P = C.POINTER(C.c_int8)
+U8P = C.POINTER(C.c_uint8)
+
+
+class Pose(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "a", "asq", "theta", "phi", "r", "speed", "br", "btheta", "bphi",
+        "alpha", "w", "wbar", "Delta", "ro")]
+
+
+class Params(C.Structure):
+    _fields_ = [("eps", C.c_double), ("h1", C.c_double), ("z_end", C.c_double),
+                ("max_steps", C.c_int), ("apriori_capture", C.c_int),
+                ("r_escape", C.c_double), ("r_capture_factor", C.c_double),
+                ("newton_cap", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("attempts", "newton", "golden", "rays", "integrated")]
+
+
+class Grid(C.Structure):
+    _fields_ = [("N", C.c_int), ("M", C.c_int), ("max_level", C.c_int),
+                ("start_level", C.c_int), ("equafactor", C.c_int),
+                ("rays_traced", C.c_int64), ("cel", DP), ("steps", IP),
+                ("level", I8P), ("present", U8P)]
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "libbtoracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library not built: {path} (run make -C oracle)")
+        L = C.CDLL(path)
+        L.bto_default_params.argtypes = [C.POINTER(Params)]
+        L.bto_pose_init.argtypes = [C.POINTER(Pose)] + [C.c_double] * 7
+        L.bto_pose_init_speed.argtypes = [C.POINTER(Pose)] + [C.c_double] * 5
+        for n in ("bto_calc_speed",):
+            getattr(L, n).argtypes = [C.c_double] * 3
+            getattr(L, n).restype = C.c_double
+        for n in ("bto_b0", "bto_q0"):
+            getattr(L, n).argtypes = [C.c_double] * 2
+            getattr(L, n).restype = C.c_double
+        L.bto_check_bq.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int64)]
+        L.bto_check_rup.argtypes = [C.c_double] * 5 + [C.POINTER(C.c_int64)]
+        L.bto_check_celest.argtypes = [C.c_double] * 6 + [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.bto_derivs.argtypes = [C.c_double, DP, DP, C.c_double, C.c_double]
+        L.bto_rkck.argtypes = [C.c_double, DP, DP, C.c_double, DP, DP, C.c_double, C.c_double]
+        L.bto_wrap_to_pi.argtypes = [DP, DP]
+        L.bto_initial_conditions.argtypes = [C.POINTER(Pose), C.c_double, C.c_double, DP]
+        L.bto_trace_ray.argtypes = [C.POINTER(Pose), C.POINTER(Params), DP, DP, IP, C.POINTER(Stats)]
+        L.bto_trace_rays.argtypes = [C.POINTER(Pose), C.POINTER(Params), C.c_int, DP, DP, IP,
+                                     C.c_int, C.POINTER(Stats)]
+        L.bto_grid_dims.argtypes = [C.c_int, C.c_int, IP, IP]
+        L.bto_grid_build.argtypes = [C.POINTER(Pose), C.POINTER(Params), C.c_int, C.c_int, C.c_int,
+                                     C.c_double, C.c_int, C.c_int, C.POINTER(Grid), C.POINTER(Stats)]
+        L.bto_grid_fix_tvertices.argtypes = [C.POINTER(Grid)]
+        L.bto_grid_free.argtypes = [C.POINTER(Grid)]
+        L.bto_hermite2.argtypes = [C.c_double, DP, DP, DP, DP, DP]
+        _LIB = L
+    return _LIB
+
+
+def load_ref():
+    """The reference's own Metric.h/Camera.h/BlackHole.h, or None."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(HERE, "_ref", "libbtref.so")
+        if not os.path.exists(path):
+            return None
+        R = C.CDLL(path)
+        R.btref_pose.argtypes = [C.c_double] * 7 + [DP]
+        R.btref_check_celest.argtypes = [C.c_double] * 6
+        R.btref_check_bq.argtypes = [C.c_double] * 3
+        R.btref_check_rup.argtypes = [C.c_double] * 5
+        for n in ("btref_b0", "btref_q0"):
+            getattr(R, n).argtypes = [C.c_double] * 2
+            getattr(R, n).restype = C.c_double
+        R.btref_calc_speed.argtypes = [C.c_double] * 3
+        R.btref_calc_speed.restype = C.c_double
+        R.btref_derivs.argtypes = [C.c_double, DP, DP, C.c_double, C.c_double]
+        R.btref_rkck.argtypes = [C.c_double, DP, DP, C.c_double, DP, DP, C.c_double, C.c_double]
+        R.btref_integrate.argtypes = [C.c_double] * 8 + [DP, DP, IP]
+        R.btref_wrap_to_pi.argtypes = [DP, DP]
+        R.btref_wrap_to_pi_f.argtypes = [FP, FP]
+        _REF = R
+    return _REF
+
+
+def params(eps=1e-5, h1=0.01, z_end=-1e7, max_steps=1000, apriori_capture=1,
+           r_escape=0.0, r_capture_factor=0.0, newton_cap=100000):
+    p = Params()
+    p.eps, p.h1, p.z_end, p.max_steps = eps, h1, z_end, max_steps
+    p.apriori_capture, p.r_escape, p.r_capture_factor = apriori_capture, r_escape, r_capture_factor
+    p.newton_cap = newton_cap
+    return p
+
+
+def baseline_params():
+    """The BASELINE wording: rel tol 1e-8, stop r>1000M or r<1.01 r+."""
+    return params(eps=1e-8, max_steps=10000, apriori_capture=0, r_escape=1000.0,
+                  r_capture_factor=1.01)
+
+
+def pose(a, theta, phi, r, br=0.0, btheta=0.0, bphi=1.0, speed=None):
+    p = Pose()
+    if speed is None:
+        lib().bto_pose_init(C.byref(p), a, theta, phi, r, br, btheta, bphi)
+    else:
+        lib().bto_pose_init_speed(C.byref(p), a, theta, phi, r, speed)
+    return p
+
+
+def trace_rays(ps, prm, theta, phi, nthreads=0):
+    """integration_wrapper on copies of theta/phi; returns (theta', phi', step, stats)."""
+    th = np.ascontiguousarray(theta, dtype=np.float64).copy()
+    ph = np.ascontiguousarray(phi, dtype=np.float64).copy()
+    st = np.zeros(th.shape, dtype=np.int32)
+    s = Stats()
+    lib().bto_trace_rays(C.byref(ps), C.byref(prm), th.size, _ptr(th, DP), _ptr(ph, DP),
+                         _ptr(st, IP), nthreads, C.byref(s))
+    return th, ph, st, s
+
+
+def initial_conditions(ps, theta, phi):
+    out = np.zeros(4)
+    lib().bto_initial_conditions(C.byref(ps), theta, phi, _ptr(out, DP))
+    return out
+
+
+def grid_dims(max_level, angle=True):
+    n, m = C.c_int(), C.c_int()
+    lib().bto_grid_dims(max_level, 1 if angle else 0, C.byref(n), C.byref(m))
+    return n.value, m.value
+
+
+class OracleGrid:
+    """Dense view of Grid(maxLevel, startLevel, angle, cam, bh)."""
+
+    def __init__(self, ps, prm, max_level, start_level, angle=True, thresh=0.015,
+                 fix_tvertices=True, nthreads=0):
+        self.g = Grid()
+        self.stats = Stats()
+        lib().bto_grid_build(C.byref(ps), C.byref(prm), max_level, start_level,
+                             1 if angle else 0, thresh, 1 if fix_tvertices else 0,
+                             nthreads, C.byref(self.g), C.byref(self.stats))
+        N, M = self.g.N, self.g.M
+        self.N, self.M = N, M
+        self.cel = np.ctypeslib.as_array(self.g.cel, shape=(N, M, 2)).copy()
+        self.steps = np.ctypeslib.as_array(self.g.steps, shape=(N, M)).copy()
+        self.level = np.ctypeslib.as_array(self.g.level, shape=(N, M)).copy()
+        self.present = np.ctypeslib.as_array(self.g.present, shape=(N, M)).copy()
+        self.rays_traced = self.g.rays_traced
+
+    def fix_tvertices(self):
+        lib().bto_grid_fix_tvertices(C.byref(self.g))
+        N, M = self.N, self.M
+        self.cel = np.ctypeslib.as_array(self.g.cel, shape=(N, M, 2)).copy()
+
+    def __del__(self):
+        try:
+            lib().bto_grid_free(C.byref(self.g))
+        except Exception:
+            pass

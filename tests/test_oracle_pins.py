@@ -1,0 +1,189 @@
+"""Pins the CPU oracle with mathematics that does not restate it:
+closed forms of Kerr photon orbits and circular-orbit speed, the null
+constraint of the initial conditions, the Cash-Karp error order, Catmull-Rom
+exactness, and structural invariants of the adaptive grid.  A plausible
+mistake (dropped term, wrong sign, transposed index) fails one of these.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+L = O.lib()
+
+
+def carter_R(a, r, b, q):
+    """Carter's radial potential R(r) = (r^2 + a^2 - a b)^2 - Delta [(b - a)^2 + q] (M = 1)."""
+    delta = r * r - 2 * r + a * a
+    return (r * r + a * a - a * b) ** 2 - delta * ((b - a) ** 2 + q)
+
+
+@pytest.mark.parametrize("a", [0.1, 0.5, 0.9, 0.999])
+def test_spherical_photon_orbits_are_double_roots(a):
+    """Bardeen (1973): (b0(r0), q0(r0)) makes r0 a double root of R."""
+    r_plus = 2 * (1 + math.cos(2 / 3 * math.acos(-a)))
+    r_minus = 2 * (1 + math.cos(2 / 3 * math.acos(a)))
+    for r0 in np.linspace(r_plus + 1e-3, r_minus - 1e-3, 25):
+        b, q = L.bto_b0(a, r0), L.bto_q0(a, r0)
+        scale = r0 ** 4
+        h = 1e-5
+        dR = (carter_R(a, r0 + h, b, q) - carter_R(a, r0 - h, b, q)) / (2 * h)
+        assert abs(carter_R(a, r0, b, q)) < 1e-9 * scale
+        assert abs(dR) < 1e-6 * scale
+
+
+@pytest.mark.parametrize("a", [0.1, 0.5, 0.9, 0.999])
+def test_equatorial_photon_orbits_have_zero_q(a):
+    for sgn in (-1, 1):
+        r_ph = 2 * (1 + math.cos(2 / 3 * math.acos(sgn * a)))
+        assert abs(L.bto_q0(a, r_ph)) < 1e-9
+
+
+@pytest.mark.parametrize("a,r", [(0.999, 5.0), (0.5, 3.0), (0.9, 20.0), (0.2, 8.0)])
+def test_circular_orbit_speed_closed_form(a, r):
+    """Bardeen-Press-Teukolsky: prograde equatorial orbit speed seen by a ZAMO,
+    v = (r^2 - 2 a sqrt(r) + a^2) / (sqrt(Delta) (r^{3/2} + a))."""
+    delta = r * r - 2 * r + a * a
+    v = (r * r - 2 * a * math.sqrt(r) + a * a) / (math.sqrt(delta) * (r ** 1.5 + a))
+    assert L.bto_calc_speed(a, r, math.pi / 2) == pytest.approx(v, rel=1e-12)
+
+
+@pytest.mark.parametrize("pose", [(0.999, math.pi / 2, 0.0, 5.0), (0.5, 1.1, 0.2, 3.0),
+                                  (0.9, 0.5, 1.0, 20.0)])
+def test_initial_conditions_are_null(pose):
+    """The photon built from a camera direction satisfies H = 0:
+    Delta^2 pR^2 = R(r) at the camera (the theta part holds by the definition
+    of q).  sqrtf in Grid.h:529-542 limits the agreement to ~1e-7."""
+    a, th, ph, r = pose
+    ps = O.pose(a, th, ph, r)
+    rng = np.random.default_rng(1)
+    delta = r * r - 2 * r + a * a
+    for t, p in zip(np.arccos(1 - 2 * rng.random(400)), 2 * math.pi * rng.random(400)):
+        pR, pT, b, q = O.initial_conditions(ps, t, p)
+        R = carter_R(a, r, b, q)
+        assert abs(delta ** 2 * pR * pR - R) <= 1e-6 * max(abs(R), r ** 4)
+        theta_pot = q - math.cos(th) ** 2 * (b * b / math.sin(th) ** 2 - a * a)
+        assert theta_pot == pytest.approx(pT * pT, rel=1e-9, abs=1e-9)
+
+
+def test_cash_karp_error_is_fifth_order():
+    """The embedded error estimate of Cash-Karp RK4(5) scales as h^5."""
+    a, b, q = 0.9, 3.0, 12.0
+    var = np.array([8.0, 1.2, 0.4, -0.7, 1.5])
+    dv = np.zeros(5)
+    L.bto_derivs(a, O._ptr(var, O.DP), O._ptr(dv, O.DP), b, q)
+    errs = []
+    for h in (-0.4, -0.2):
+        y, e = np.zeros(5), np.zeros(5)
+        L.bto_rkck(a, O._ptr(var, O.DP), O._ptr(dv, O.DP), h, O._ptr(y, O.DP), O._ptr(e, O.DP), b, q)
+        errs.append(np.max(np.abs(e)))
+    assert 25 < errs[0] / errs[1] < 40
+
+
+def test_catmull_rom_midpoint_is_exact_for_cubics():
+    for poly in (lambda k: 3 * k + 1, lambda k: k * k - 2 * k, lambda k: 0.5 * k ** 3 - k):
+        xs = [np.array([poly(k), poly(2 * k)], dtype=np.float64) for k in range(4)]
+        out = np.zeros(2)
+        L.bto_hermite2(0.5, *[O._ptr(x, O.DP) for x in xs], O._ptr(out, O.DP))
+        assert out[0] == pytest.approx(poly(1.5), abs=1e-12)
+        assert out[1] == pytest.approx(poly(3.0), abs=1e-12)
+
+
+def test_wrap_to_pi_preserves_direction():
+    import ctypes as C
+    rng = np.random.default_rng(2)
+    for t, p in zip(rng.uniform(-30, 30, 500), rng.uniform(-60, 60, 500)):
+        a, b = C.c_double(t), C.c_double(p)
+        L.bto_wrap_to_pi(C.byref(a), C.byref(b))
+        assert 0 <= a.value <= math.pi and 0 <= b.value < 2 * math.pi
+        d0 = np.array([math.sin(t) * math.cos(p), math.sin(t) * math.sin(p), math.cos(t)])
+        d1 = np.array([math.sin(a.value) * math.cos(b.value), math.sin(a.value) * math.sin(b.value),
+                       math.cos(a.value)])
+        assert np.allclose(d0, d1, atol=1e-9)
+
+
+def _blocks(g):
+    keys = np.argwhere(g.level >= 0)
+    return keys, g.level[g.level >= 0]
+
+
+def test_uniform_grid_covers_lattice():
+    ps = O.pose(0.999, math.pi / 2, 0.0, 5.0)
+    g = O.OracleGrid(ps, O.params(), 5, 5, True, nthreads=4)
+    N, M = g.N, g.M
+    assert (N, M) == (33, 64)
+    assert g.rays_traced == 2 + (N - 2) * M
+    assert g.present.all()
+    assert (g.level[:-1] == 5).all() and (g.level[-1] == -1).all()
+
+
+@pytest.mark.parametrize("pose,lv", [((0.999, math.pi / 2, 0.0, 5.0), 8), ((0.6, 1.0, 0.0, 4.0), 7)])
+def test_adaptive_grid_invariants(pose, lv):
+    """Blocks tile the sphere, every block's corners were traced, unrefined
+    blocks fail refineCheck and refined parents pass it (Grid.h:410-445)."""
+    ps = O.pose(*pose)
+    g = O.OracleGrid(ps, O.params(), lv, 1, True, fix_tvertices=False, nthreads=4)
+    N, M = g.N, g.M
+    keys, lvls = _blocks(g)
+    gaps = 2 ** (lv - lvls.astype(np.int64))
+    assert np.sum(gaps * gaps) == (N - 1) * M
+    cover = np.zeros((N - 1, M), dtype=np.int32)
+    th, phv = g.cel[..., 0], g.cel[..., 1]
+    for (i, j), L_, gp in zip(keys, lvls, gaps):
+        cover[i:i + gp, j:j + gp] += 1
+        k, l = i + gp, (j + gp) % M
+        assert g.present[i, j] and g.present[k, j] and g.present[i, l] and g.present[k, l]
+        d1 = (th[i, j] - th[k, l]) ** 2 + (phv[i, j] - phv[k, l]) ** 2
+        d2 = (th[k, j] - th[i, l]) ** 2 + (phv[k, j] - phv[i, l]) ** 2
+        mx = max(d1, d2)
+        refine = (L_ < 6 and mx > 1e-10) or mx > 0.015
+        if L_ < lv:
+            assert not refine
+        if L_ > 1:
+            # the parent block (one level up) must have been refined
+            pg = 2 * gp
+            pi_, pj = (i // pg) * pg, (j // pg) * pg
+            pk, pl = pi_ + pg, (pj + pg) % M
+            p1 = (th[pi_, pj] - th[pk, pl]) ** 2 + (phv[pi_, pj] - phv[pk, pl]) ** 2
+            p2 = (th[pk, pj] - th[pi_, pl]) ** 2 + (phv[pk, pj] - phv[pi_, pl]) ** 2
+            pm = max(p1, p2)
+            assert (L_ - 1 < 6 and pm > 1e-10) or pm > 0.015
+    assert (cover == 1).all()
+
+
+def test_fix_tvertices_reproduces_affine_field():
+    """Catmull-Rom midpoints and edge averages reproduce an affine field, so
+    on the identity map (celestial = camera direction) fixTvertices must
+    change nothing away from the pole rows, where the reference deliberately
+    reads across the pole (Grid.h:146-149)."""
+    ps = O.pose(0.999, math.pi / 2, 0.0, 5.0)
+    g = O.OracleGrid(ps, O.params(), 8, 1, True, fix_tvertices=False, nthreads=4)
+    N, M = g.N, g.M
+    ii, jj = np.meshgrid(np.arange(N), np.arange(M), indexing="ij")
+    ident_t = ii / (N - 1) * math.pi
+    ident_p = jj / M * 2 * math.pi
+    cel = np.ctypeslib.as_array(g.g.cel, shape=(N, M, 2))
+    cel[..., 0] = np.where(g.present == 1, ident_t, -2)
+    cel[..., 1] = np.where(g.present == 1, ident_p, -2)
+    g.fix_tvertices()
+    out = g.cel
+    band = (ii > N // 8) & (ii < N - N // 8) & (g.present == 1)
+    dt = np.abs(out[..., 0] - ident_t)[band]
+    dp = np.abs(out[..., 1] - ident_p)[band]
+    dp = np.minimum(dp, np.abs(dp - 2 * math.pi))
+    assert dt.max() < 1e-12 and dp.max() < 1e-12
+    assert band.sum() > 10000
+
+
+def test_fix_tvertices_moves_real_tvertices():
+    """On the traced field the fix must rewrite some T-vertices, and only
+    points that were traced."""
+    ps = O.pose(0.999, math.pi / 2, 0.0, 5.0)
+    g0 = O.OracleGrid(ps, O.params(), 8, 1, True, fix_tvertices=False, nthreads=4)
+    g1 = O.OracleGrid(ps, O.params(), 8, 1, True, fix_tvertices=True, nthreads=4)
+    changed = np.any(g0.cel != g1.cel, axis=-1)
+    assert changed.sum() > 100
+    assert (g0.present[changed] == 1).all()
+    assert np.array_equal(g0.level, g1.level)
