@@ -236,3 +236,144 @@ class Grid:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- render
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def RenderFlags(star=True, map=True, lr=True, hor=0.0, ver=0.0, offset=0.0):  # noqa: N802
+    """bt_render_flags (MakeImageKernel.cu:955-960, :1413)."""
+    return BtRenderFlags(star=int(star), map=int(map), lr=int(lr), hor=hor, ver=ver, offset=offset)
+
+
+def viewer_matrix(view_angle, off_up, off_right, width, height, sphere=False):
+    """Viewer(viewangle, offUp, offRight, w, h, sphere).viewMatrix, (h+1, w+1, 2) float32."""
+    out = np.zeros((height + 1, width + 1, 2), dtype=np.float32)
+    check(_lib.load().bt_viewer_matrix(view_angle, off_up, off_right, width, height,
+                                       1 if sphere else 0, out.ctypes.data_as(_lib.FP)))
+    return out
+
+
+def star_tree(stars4, tree_level):
+    """StarProcessor::makeTree: (tree, star_pos (S,2), star_mag (S,2))."""
+    s4 = _f32(stars4)
+    S = s4.shape[0]
+    tree = np.zeros((1 << (tree_level + 1)) - 1, dtype=np.int32)
+    pos = np.zeros((S, 2), dtype=np.float32)
+    mag = np.zeros((S, 2), dtype=np.float32)
+    check(_lib.load().bt_star_tree(s4.ctypes.data_as(_lib.FP), S, tree_level,
+                                   tree.ctypes.data_as(_lib.I32P), pos.ctypes.data_as(_lib.FP),
+                                   mag.ctypes.data_as(_lib.FP)))
+    return tree, pos, mag
+
+
+class Scene:
+    """Device-resident stars (star tree, StarProcessor.h) and summed-area sky
+    table (MakeImageKernel.cu:1312-1331)."""
+
+    def __init__(self, stars4, sky_bgr, tree_level=8, ctx=None):
+        self.ctx = ctx or default_context()
+        s4 = _f32(stars4)
+        sky = np.ascontiguousarray(sky_bgr, dtype=np.uint8)
+        self.rows, self.cols = sky.shape[:2]
+        self.tree_level = tree_level
+        h = C.c_void_p()
+        check(_lib.load().bt_scene_create(self.ctx.handle, s4.ctypes.data_as(_lib.FP), s4.shape[0],
+                                          tree_level, sky.ctypes.data_as(_lib.U8P), self.rows,
+                                          self.cols, C.byref(h)))
+        self.handle = h
+
+    def table(self):
+        t = np.zeros((self.rows, self.cols, 4), dtype=np.float32)
+        check(_lib.load().bt_scene_export_table(self.handle, t.ctypes.data_as(_lib.FP)))
+        return t
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().bt_scene_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class View:
+    """Device-resident Viewer matrix of one resolution."""
+
+    def __init__(self, width, height, view_angle=np.pi / 2, off_up=0.0, off_right=0.25 * np.pi / 4,
+                 sphere=False, ctx=None):
+        self.ctx = ctx or default_context()
+        self.width, self.height = width, height
+        h = C.c_void_p()
+        check(_lib.load().bt_view_create(self.ctx.handle, view_angle, off_up, off_right, width,
+                                         height, 1 if sphere else 0, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().bt_view_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def render_frame(grid, scene, view, flags=None, cam7=None, step=1, stages=False, ctx=None):
+    """One frame (BGRA uint8, (h, w, 4)) of a device-resident grid."""
+    ctx = ctx or grid.ctx
+    W, H = view.width, view.height
+    out = np.zeros((H, W, 4), dtype=np.uint8)
+    flags = flags or RenderFlags()
+    c7 = None if cam7 is None else _f32(cam7).ctypes.data_as(_lib.FP)
+    L = _lib.load()
+    if not stages:
+        check(L.bt_render_frame(ctx.handle, grid.handle, scene.handle, view.handle, c7,
+                                C.byref(flags), step, out.ctypes.data_as(_lib.U8P), None))
+        return out
+    thphi = np.zeros((H + 1, W + 1, 2), dtype=np.float32)
+    gap = np.zeros((H + 1, W + 1), dtype=np.int32)
+    bh = np.zeros((H, W), dtype=np.uint8)
+    area = np.zeros((H, W), dtype=np.float32)
+    star = np.zeros((H, W, 3), dtype=np.float32)
+    check(L.bt_render_frame_stages(ctx.handle, grid.handle, scene.handle, view.handle, c7,
+                                   C.byref(flags), step, out.ctypes.data_as(_lib.U8P), None,
+                                   thphi.ctypes.data_as(_lib.FP), gap.ctypes.data_as(_lib.I32P),
+                                   bh.ctypes.data_as(_lib.U8P), area.ctypes.data_as(_lib.FP),
+                                   star.ctypes.data_as(_lib.FP)))
+    return out, dict(thphi=thphi, gap=gap, bh=bh, area=area, star=star)
+
+
+def make_image(stars, star_tree_arr, cam_param, magnitude, tree_level, M, N, step, sky_bgr,
+               viewthing, GM, GN, gridlvl, offset_table, hash_table, hash_pos_tag, hw, ow,
+               flags=None, ctx=None):
+    """makeImage (MakeImageKernel.cu:1206) with G = 1; returns the BGRA frame."""
+    ctx = ctx or default_context()
+    out = np.zeros((N, M, 4), dtype=np.uint8)
+    sky = np.ascontiguousarray(sky_bgr, dtype=np.uint8)
+    ts = np.array([hw, ow], dtype=np.int32)
+    ot = np.ascontiguousarray(offset_table, dtype=np.int32)
+    ht = _f32(hash_table)
+    tag = np.ascontiguousarray(hash_pos_tag, dtype=np.int32)
+    st = _f32(stars)
+    mg = _f32(magnitude)
+    tr = np.ascontiguousarray(star_tree_arr, dtype=np.int32)
+    flags = flags or RenderFlags()
+    check(_lib.load().bt_make_image(
+        ctx.handle, st.ctypes.data_as(_lib.FP), tr.ctypes.data_as(_lib.I32P), st.size // 2,
+        _f32(cam_param).ctypes.data_as(_lib.FP), mg.ctypes.data_as(_lib.FP), tree_level, M, N,
+        step, sky.ctypes.data_as(_lib.U8P), sky.shape[0], sky.shape[1], 1, 0.0, 0.0, None,
+        _f32(viewthing).ctypes.data_as(_lib.FP), 0.0, GM, GN, gridlvl,
+        ot.ctypes.data_as(_lib.I32P), ht.ctypes.data_as(_lib.FP), tag.ctypes.data_as(_lib.I32P),
+        ts.ctypes.data_as(_lib.I32P), ot.size // 2, ht.size // 2, C.byref(flags),
+        out.ctypes.data_as(_lib.U8P)))
+    return out

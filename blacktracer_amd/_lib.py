@@ -89,6 +89,20 @@ SIGNATURES = {
                                 C.c_int, U8P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, FP,
                                 FP, C.c_float, C.c_int, C.c_int, C.c_int, I32P, FP, I32P, I32P,
                                 C.c_int, C.c_int, C.POINTER(BtRenderFlags), U8P]),
+    "bt_viewer_matrix": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, FP]),
+    "bt_star_tree": (C.c_int, [FP, C.c_int, C.c_int, I32P, FP, FP]),
+    "bt_scene_create": (C.c_int, [C.c_void_p, FP, C.c_int, C.c_int, U8P, C.c_int, C.c_int,
+                                  C.POINTER(C.c_void_p)]),
+    "bt_scene_export_table": (C.c_int, [C.c_void_p, FP]),
+    "bt_scene_free": (None, [C.c_void_p]),
+    "bt_view_create": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 C.c_int, C.POINTER(C.c_void_p)]),
+    "bt_view_free": (None, [C.c_void_p]),
+    "bt_render_frame": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, FP,
+                                  C.POINTER(BtRenderFlags), C.c_int, U8P, C.c_void_p]),
+    "bt_render_frame_stages": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, FP,
+                                         C.POINTER(BtRenderFlags), C.c_int, U8P, C.c_void_p, FP,
+                                         I32P, U8P, FP, FP]),
 }
 
 _lib = None

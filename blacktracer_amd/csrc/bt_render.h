@@ -1,0 +1,51 @@
+// bt_render.h -- device-resident inputs of the render pass and its launcher.
+#pragma once
+
+#include "bt_internal.h"
+
+namespace bt {
+
+// Dense FP32 celestial grid (GN rows x GM columns of float2).
+struct GridF {
+    const float* grid;
+    int GN, GM, gridlvl;
+};
+
+// Stars (StarProcessor layout) and the summed-area sky table.
+struct SceneK {
+    const int32_t* tree;
+    int tree_level;
+    const float* star_pos;   // 2 per star: theta, phi
+    const float* star_mag;   // 2 per star: magnitude, B-V
+    int S;
+    const float* table;      // rows * cols float4
+    int rows, cols;
+};
+
+// Pixel-corner camera-sky directions ((H+1) x (W+1) float2).
+struct ViewK {
+    const float* view;
+    int W, H;
+};
+
+// Optional device copies of intermediates (for parity tests).
+struct FrameDebug {
+    float* thphi;
+    int* gap;
+    uint8_t* bh;
+    float* area;
+    float* star;
+};
+
+int render_sum_table(Context* c, const uint8_t* d_sky, int rows, int cols, float* d_table);
+int render_grid_from_psh(Context* c, const float* d_ht, const int32_t* d_ot, const int32_t* d_tag, int hw,
+                         int ow, int GN, int GM, float* d_grid, int* GN1_out);
+int render_grid_from_cel(Context* c, const double* d_cel, int GN, int GM, float* d_grid, int* GN1_out);
+int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, const float* cam7,
+                 const bt_render_flags& f, int step, uint8_t* d_out, FrameDebug* dbg);
+
+// Host-side inputs (Viewer.h, StarProcessor.h).
+int viewer_matrix(double view_angle, double off_up, double off_right, int w, int h, int sphere, float* out);
+int star_tree(const float* stars4, int S, int tree_level, int32_t* tree, float* pos, float* mag);
+
+}  // namespace bt

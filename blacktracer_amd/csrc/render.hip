@@ -1,2 +1,861 @@
-// render.hip -- render kernels (Path B).
-#include "bt_internal.h"
+// render.hip -- the per-pixel render pass (Path B): the drop-in for the
+// G == 1 frame of cudaPrep (MakeImageKernel.cu:1402-1529).
+//
+// Stages (one kernel each, all on the context stream):
+//   pix_interp   per pixel corner: quadtree descent of the dense grid and
+//                bicubic Hermite interpolation of the celestial angle
+//                (pixInterpolation, interpolatePix, kernel.cuh:736-968)
+//   bh_area      shadow flag and celestial solid angle per pixel
+//                (findBlackPixels + findArea)
+//   smooth_h/v   box-filtered solid angle (smoothAreaH/V)
+//   star_map     per pixel: star-tree search, point-in-quad, sub-pixel
+//                position, blackbody colour and the 3x3 Gaussian taps
+//                (distortStarMap); taps are stored tap-major so the gather
+//                of star_sum reads coalesced rows
+//   star_sum     gather of the neighbours' taps + makePix
+//   env_map      summed-area sky sampling, lensing and redshift
+//                (distortEnvironmentMap)
+//   composite    addStarsAndBackground
+// Pixel (i, j) is processed by thread x = j, y = i so that a wavefront
+// (16 x 4 pixels of a 16 x 16 block) reads consecutive row addresses.
+// Compiled with -ffp-contract=off: results equal the CPU oracle bit for bit.
+#include "bt_render.h"
+#include "render_dev.h"
+
+namespace bt {
+namespace {
+
+using namespace rdr;
+
+__constant__ int kTempRGB[1173] = {
+#include "blackbody_rgb.inc"
+};
+
+struct Cam7 {
+    float v[7];
+};
+
+__device__ __forceinline__ F2 gload(const float* grid, int GN, int GM, int i, int j) {
+    if (i < 0 || i >= GN || j < 0 || j >= GM) return F2{-2.f, -2.f};
+    const float2 v = reinterpret_cast<const float2*>(grid)[(size_t)i * GM + j];
+    return F2{v.x, v.y};
+}
+
+struct GridV {
+    const float* g;
+    int GN, GM;
+    __device__ __forceinline__ F2 at(int i, int j) const { return gload(g, GN, GM, i, j); }
+};
+
+// kernel.cuh:774-788
+__device__ F2 interpolate_linear(float percDown, float percRight, const F2* c) {
+    float phi[4] = {c[0].y, c[1].y, c[2].y, c[3].y};
+    float theta[4] = {c[0].x, c[1].x, c[2].x, c[3].x};
+    pi_check(phi, 0.2f);
+    float leftT = theta[0] + percDown * (theta[2] - theta[0]);
+    float leftP = phi[0] + percDown * (phi[2] - phi[0]);
+    float rightT = theta[1] + percDown * (theta[3] - theta[1]);
+    float rightP = phi[1] + percDown * (phi[3] - phi[1]);
+    float upT = theta[0] + percRight * (theta[1] - theta[0]);
+    float upP = phi[0] + percRight * (phi[1] - phi[0]);
+    float downT = theta[2] + percRight * (theta[3] - theta[2]);
+    float downP = phi[2] + percRight * (phi[3] - phi[2]);
+    return intersection(upT, upP, downT, downP, leftT, leftP, rightT, rightP);
+}
+
+// The reference's findPoint may recurse once into interpolateHermite
+// (count 0 -> 1) and never deeper; the two levels are written out here.
+__device__ F2 interpolate_hermite_l1(const GridV& g, int i, int j, int gap, float percDown,
+                                     float percRight, F2* c);
+
+// kernel.cuh:817-889 for recursion depth `count`.
+template <int COUNT>
+__device__ F2 find_point(const GridV& g, int i, int j, int offver, int offhor, int gap) {
+    const int GM = g.GM, GN = g.GN;
+    F2 gridpt = g.at(i, j);
+    if (gridpt.x == -2 && gridpt.y == -2) {
+        int j2 = (j + offhor * gap + GM) % GM;
+        int i2 = i + offver * gap;
+        F2 ij2 = g.at(i2, j2);
+        if (ij2.x == -1 && ij2.y == -1) return F2{-1, -1};
+        if (ij2.x != -2 && ij2.y != -2) {
+            int j0 = (j - offhor * gap + GM) % GM;
+            int i0 = (i - offver * gap);
+            F2 ij0 = g.at(i0, j0);
+            if (ij0.x < 0) return F2{-1, -1};
+            int jprev = (j - 3 * offhor * gap + GM) % GM;
+            int jnext = (j + 3 * offhor * gap + GM) % GM;
+            int iprev = i - offver * 3 * gap;
+            int inext = i + offver * 3 * gap;
+            if (offver != 0) {
+                if (i2 == 0) {
+                    jnext = (j0 + GM / 2) % GM;
+                    inext = i0;
+                } else if (i0 == GN - 1) {
+                    jprev = (j0 + GM / 2) % GM;
+                    iprev = i2;
+                } else if (i2 == GN - 1) {
+                    inext = i0;
+                    jnext = (j0 + GM / 2) % GM;
+                }
+            }
+            F2 ijprev = g.at(iprev, jprev);
+            F2 ijnext = g.at(inext, jnext);
+            if (ijprev.x > -2 && ijnext.x > -2) {
+                F2 pt[4] = {ijprev, ij0, ij2, ijnext};
+                if (pt[0].x != -1 && pt[3].x != -1) {
+                    pi_check_tot(pt, 0.2f, 4);
+                    return hermite(0.5f, pt[0], pt[1], pt[2], pt[3]);
+                }
+            }
+            F2 pt[2] = {ij2, ij0};
+            pi_check_tot(pt, 0.2f, 2);
+            return F2{.5f * (pt[0].x + pt[1].x), .5f * (pt[0].y + pt[1].y)};
+        }
+        if (COUNT > 0) return F2{-1.f, -1.f};
+        int j0 = (j + gap) % GM;
+        int j1 = (j - gap + GM) % GM;
+        if (i - gap < 0) return F2{-1, -1};
+        F2 cc[12];
+        cc[0] = g.at(i + gap, j0);
+        cc[1] = g.at(i - gap, j0);
+        cc[2] = g.at(i - gap, j1);
+        cc[3] = g.at(i + gap, j1);
+        for (int q = 0; q < 4; q++)
+            if (cc[q].x == -1 || cc[q].x == -2) return F2{-1, -1};
+        return interpolate_hermite_l1(g, i - gap, j1, 2 * gap, .5f, .5f, cc);
+    }
+    return gridpt;
+}
+
+// kernel.cuh:891-938 for recursion depth COUNT.
+template <int COUNT>
+__device__ F2 interpolate_hermite(const GridV& g, int i, int j, int gap, float percDown,
+                                  float percRight, F2* c) {
+    const int GM = g.GM, GN = g.GN;
+    int k = i + gap;
+    int l = (j + gap) % GM;
+    int imin1 = i - gap;
+    int kplus1 = k + gap;
+    int jmin1 = (j - gap + GM) % GM;
+    int lplus1 = (l + gap) % GM;
+    int jx = j, jy = j, lx = l, ly = l;
+    if (i == 0) {
+        jx = (j + GM / 2) % GM;
+        lx = (jx + gap) % GM;
+        imin1 = k;
+    } else if (k == GN - 1) {
+        jy = (j + GM / 2) % GM;
+        ly = (jy + gap) % GM;
+        kplus1 = i;
+    }
+    c[4] = find_point<COUNT>(g, i, jmin1, 0, -1, gap);
+    c[5] = find_point<COUNT>(g, i, lplus1, 0, 1, gap);
+    c[6] = find_point<COUNT>(g, k, jmin1, 0, -1, gap);
+    c[7] = find_point<COUNT>(g, k, lplus1, 0, 1, gap);
+    c[8] = find_point<COUNT>(g, imin1, jx, -1, 0, gap);
+    c[9] = find_point<COUNT>(g, imin1, lx, -1, 0, gap);
+    c[10] = find_point<COUNT>(g, kplus1, jy, 1, 0, gap);
+    c[11] = find_point<COUNT>(g, kplus1, ly, 1, 0, gap);
+    for (int q = 4; q < 12; q++)
+        if (c[q].x == -1) return interpolate_linear(percDown, percRight, c);
+    pi_check_tot(c, 0.2f, 12);
+    F2 up = hermite(percRight, c[4], c[0], c[1], c[5]);
+    F2 down = hermite(percRight, c[6], c[2], c[3], c[7]);
+    F2 upup = F2{c[8].x + (c[9].x - c[8].x) * percRight, c[8].y + (c[9].y - c[8].y) * percRight};
+    F2 downdown = F2{c[10].x + (c[11].x - c[10].x) * percRight, c[10].y + (c[11].y - c[10].y) * percRight};
+    return hermite(percDown, upup, up, down, downdown);
+}
+
+__device__ F2 interpolate_hermite_l1(const GridV& g, int i, int j, int gap, float percDown,
+                                     float percRight, F2* c) {
+    return interpolate_hermite<1>(g, i, j, gap, percDown, percRight, c);
+}
+
+// MakeImageKernel.cu:328-351 with findBlock (kernel.cuh:736-752) and
+// interpolateSpline (kernel.cuh:940-968).
+__device__ F2 interpolate_pix(const GridV& g, float theta, float phi, int gridlvl, int& gapsave) {
+    const int GM = g.GM;
+    int half = (phi < BT_FPI) ? 0 : 1;
+    int a = 0;
+    int b = half * GM / 2;
+    int gap = GM / 2;
+    for (int s = 0; s < gridlvl + 1; s++) {
+        int ngap = gap / 2;
+        int k = a + ngap;
+        int l = b + ngap;
+        if (gap <= 1 || g.at(k, l).x == -2.f) break;
+        float thHalf = BT_FPI2 * k / (1.f * GM);
+        float phHalf = BT_FPI2 * l / (1.f * GM);
+        if (thHalf <= theta) a = k;
+        if (phHalf <= phi) b = l;
+        gap = ngap;
+    }
+    gapsave = gap;
+    int k = a + gap;
+    int l = b + gap;
+    float factor = BT_FPI2 / (1.f * GM);
+    float thetaUp = factor * a, phiLeft = factor * b, thetaDown = factor * k, phiRight = factor * l;
+    l = l % GM;
+    F2 c[12];
+    c[0] = g.at(a, b);
+    c[1] = g.at(a, l);
+    c[2] = g.at(k, b);
+    c[3] = g.at(k, l);
+    F2 r;
+    bool done = false;
+    if (thetaUp == theta) {
+        if (phiLeft == phi) {
+            r = c[0];
+            done = true;
+        } else if (phiRight == phi) {
+            r = c[1];
+            done = true;
+        } else if (a == 0.f) {
+            r = c[0];
+            done = true;
+        }
+    } else if (thetaDown == theta) {
+        if (phiLeft == phi) {
+            r = c[2];
+            done = true;
+        } else if (phiRight == phi) {
+            r = c[3];
+            done = true;
+        }
+    }
+    if (!done) {
+        bool captured = false;
+        for (int q = 0; q < 4; q++)
+            if (c[q].x == -1 && c[q].y == -1) captured = true;
+        if (captured) {
+            r = F2{-1.f, -1.f};
+        } else {
+            float percDown = (theta - thetaUp) / (thetaDown - thetaUp);
+            float percRight = (phi - phiLeft) / (phiRight - phiLeft);
+            r = interpolate_hermite<0>(g, a, b, gap, percDown, percRight, c);
+        }
+    }
+    if (!(r.x == -1 && r.y == -1)) wrap_to_pi(r.x, r.y);
+    return r;
+}
+
+// ------------------------------------------------------------------ kernels
+
+__global__ __launch_bounds__(256) void pix_interp(GridV g, int gridlvl, const float* view, int W, int H,
+                                                  float hor, float ver, float* thphi, int* gap) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > H || j > W) return;
+    size_t c = (size_t)i * (W + 1) + j;
+    float theta = view[2 * c] + ver;
+    float phi = fmodf(view[2 * c + 1] + hor + BT_FPI2, BT_FPI2);
+    int gs = 0;
+    F2 r = interpolate_pix(g, theta, phi, gridlvl, gs);
+    thphi[2 * c] = r.x;
+    thphi[2 * c + 1] = r.y;
+    gap[c] = gs;
+}
+
+// kernel.cuh:652-672; false when a corner is negative (shadow).
+__device__ __forceinline__ bool corners(const float* thphi, int W, int ind, float* t, float* p,
+                                        bool& picheck, float offset) {
+    const int M1 = W + 1;
+    const float2* tp = reinterpret_cast<const float2*>(thphi);
+    float2 c0 = tp[ind + M1], c1 = tp[ind], c2 = tp[ind + 1], c3 = tp[ind + M1 + 1];
+    t[0] = c0.x; t[1] = c1.x; t[2] = c2.x; t[3] = c3.x;
+    p[0] = c0.y; p[1] = c1.y; p[2] = c2.y; p[3] = c3.y;
+    for (int q = 0; q < 4; q++) {
+        if (p[q] < 0 || t[q] < 0) return false;
+        p[q] = fmodf(p[q] + offset, BT_FPI2);
+    }
+    picheck = pi_check(p, .2f);
+    return true;
+}
+
+__global__ __launch_bounds__(256) void bh_area(const float* thphi, int W, int H, uint8_t* bh, float* area) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= H || j >= W) return;
+    float t[4], p[4];
+    bool pc = false;
+    bool ok = corners(thphi, W, i * (W + 1) + j, t, p, pc, 0.0f);
+    size_t ij = (size_t)i * W + j;
+    bh[ij] = ok ? 0 : 1;
+    area[ij] = ok ? calc_area3(t[0], t[1], t[2], p[0], p[1], p[2]) + calc_area3(t[0], t[2], t[3], p[0], p[2], p[3])
+                  : 0.f;
+}
+
+__device__ __forceinline__ int filter_size(const int* gap, int W, int i, int j) {
+    const int M1 = W + 1;
+    int fs = gap[i * M1 + j];
+    fs = max(fs, gap[i * M1 + M1 + j]);
+    fs = max(fs, gap[i * M1 + j + 1]);
+    fs = max(fs, gap[i * M1 + M1 + j + 1]);
+    return fs / 2;
+}
+
+// MakeImageKernel.cu:570-591
+__global__ __launch_bounds__(256) void smooth_h(const float* area, const uint8_t* bh, const int* gap, int W,
+                                                int H, float* area_s) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= H || j >= W) return;
+    size_t ij = (size_t)i * W + j;
+    if (bh[ij] == 1) {
+        area_s[ij] = 0.f;
+        return;
+    }
+    int fs = filter_size(gap, W, i, j);
+    float sum = 0.f;
+    int count = 0;
+    for (int h = -fs; h <= fs; h++) {
+        int jj = ((j + h) % W + W) % W;
+        if (bh[(size_t)i * W + jj] == 0) {
+            sum += area[(size_t)i * W + jj];
+            count++;
+        }
+    }
+    area_s[ij] = sum / (1.f * count);
+}
+
+// MakeImageKernel.cu:593-616
+__global__ __launch_bounds__(256) void smooth_v(const float* area_s, const uint8_t* bh, const int* gap, int W,
+                                                int H, float* area) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= H || j >= W) return;
+    size_t ij = (size_t)i * W + j;
+    if (bh[ij] == 1) return;
+    int fs = filter_size(gap, W, i, j);
+    int minv = max(0, i - fs);
+    int maxv = min(H - 1, i + fs);
+    float sum = 0.f;
+    int count = 0;
+    for (int h = minv; h <= maxv; h++)
+        if (bh[(size_t)h * W + j] == 0) {
+            sum += area_s[(size_t)h * W + j];
+            count++;
+        }
+    area[ij] = sum / (1.f * count);
+}
+
+// kernel.cuh:674-692
+__device__ void lensing_redshift(const float* t, const float* p, int W, int ind, const Cam7& cam,
+                                 const float* view, float& frac, float& redshft, float solidAngle) {
+    const int M1 = W + 1;
+    if (solidAngle == 0.f)
+        solidAngle = calc_area3(t[0], t[1], t[2], p[0], p[1], p[2]) + calc_area3(t[0], t[2], t[3], p[0], p[2], p[3]);
+    const float2* v = reinterpret_cast<const float2*>(view);
+    float2 v0 = v[ind], v1 = v[ind + 1], v2 = v[ind + M1 + 1], v3 = v[ind + M1];
+    float ver4[4] = {v0.x, v1.x, v2.x, v3.x};
+    float hor4[4] = {v0.y, v1.y, v2.y, v3.y};
+    float pixArea = calc_area4(ver4, hor4);
+    frac = pixArea / solidAngle;
+    float thetaCam = (ver4[0] + ver4[1] + ver4[2] + ver4[3]) * .25f;
+    float phiCam = (hor4[0] + hor4[1] + hor4[2] + hor4[3]) * .25f;
+    redshft = redshift(thetaCam, phiCam, cam.v);
+}
+
+// Per-pixel star accumulation: the 3x3 taps live in registers.
+template <int STEP>
+struct Taps {
+    static constexpr int FW = 2 * STEP + 1;
+    float v[FW * FW][3];
+};
+
+// distortStarMap (MakeImageKernel.cu:206-326) for step = STEP, with the
+// star-tree search (MakeImageKernel.cu:27-96) visiting nodes in the
+// reference's list order.
+template <int STEP>
+__global__ __launch_bounds__(256) void star_map(const float* thphi, const uint8_t* bh, SceneK s, Cam7 cam,
+                                                int W, int H, float offset, int lr, const float* area,
+                                                const float* view, float* taps) {
+    constexpr int FW = 2 * STEP + 1;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= H || j >= W) return;
+    const size_t NP = (size_t)W * H;
+    const size_t ij = (size_t)i * W + j;
+    float acc[FW * FW][3];
+#pragma unroll
+    for (int u = 0; u < FW * FW; u++) acc[u][0] = acc[u][1] = acc[u][2] = 0.f;
+
+    if (bh[ij] == 0) {
+        float t[4], p[4];
+        bool picheck = false;
+        const int ind = i * (W + 1) + j;
+        corners(thphi, W, ind, t, p, picheck, offset);
+        const float mx[2] = {fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3])), fmaxf(fmaxf(p[0], p[1]), fmaxf(p[2], p[3]))};
+        const float mn[2] = {fminf(fminf(t[0], t[1]), fminf(t[2], t[3])), fminf(fminf(p[0], p[1]), fminf(p[2], p[3]))};
+        float orient = (t[1] - t[0]) * (p[1] + p[0]) + (t[2] - t[1]) * (p[2] + p[1]) + (t[3] - t[2]) * (p[3] + p[2]) +
+                       (t[0] - t[3]) * (p[0] + p[3]);
+        int sgn = orient < 0 ? -1 : 1;
+        float redshft = 1.f, frac = 1.f;
+        if (lr) lensing_redshift(t, p, W, ind, cam, view, frac, redshft, area[ij]);
+        float red = 4.f * f_log10(redshft);
+        const float maxDistSq = (STEP + .5f) * (STEP + .5f);
+        const int32_t* tree = s.tree;
+        const int treeLevel = s.tree_level;
+
+        auto visit = [&](int node) {
+            int startN = 0;
+            if (node != 0 && ((node + 1) & node) != 0) startN = tree[node - 1];
+            const int endN = tree[node];
+            for (int q = startN; q < endN; q++) {
+                const float2 sp2 = reinterpret_cast<const float2*>(s.star_pos)[q];
+                float st = sp2.x, sp = sp2.y;
+                bool in = star_in_polygon(t, p, st, sp, sgn);
+                if (picheck && !in && sp < BT_FPI2 * .2f) {
+                    sp += BT_FPI2;
+                    in = star_in_polygon(t, p, st, sp, sgn);
+                }
+                if (!in) continue;
+                interpolate_star(t[0], t[1], t[2], t[3], p[0], p[1], p[2], p[3], st, sp, sgn);
+                const float2 mg = reinterpret_cast<const float2*>(s.star_mag)[q];
+                float part = mg.x + red;
+                float temp = 46.f / redshft * ((1.f / ((0.92f * mg.y) + 1.7f)) + (1.f / ((0.92f * mg.y) + 0.62f))) - 10.f;
+                // clamp to the table's last entry (the reference's 1170 reads past it)
+                int index = max(0, min((int)floorf(temp), 390));
+                float hh, ss, pp;
+                rgb_to_hsp((float)kTempRGB[3 * index] / 255.f, (float)kTempRGB[3 * index + 1] / 255.f,
+                           (float)kTempRGB[3 * index + 2] / 255.f, hh, ss, pp);
+#pragma unroll
+                for (int u = 0; u < FW; u++)
+#pragma unroll
+                    for (int v = 0; v < FW; v++) {
+                        float dt = -STEP + u + .5f - st;
+                        float dp = -STEP + v + .5f - sp;
+                        float dist = dt * dt + dp * dp;
+                        if (dist > maxDistSq) continue;
+                        float appMag = part - 2.5f * f_log10(frac * gaussian(dist));
+                        float brightness = 100 * f_exp10(-.4f * appMag);
+                        float rr, gg, bb;
+                        hsp_to_rgb(hh, ss, pp * brightness, rr, gg, bb);
+                        rr *= 255.f;
+                        gg *= 255.f;
+                        bb *= 255.f;
+                        acc[FW * u + v][0] += rr * rr;
+                        acc[FW * u + v][1] += gg * gg;
+                        acc[FW * u + v][2] += bb * bb;
+                    }
+            }
+        };
+
+        // searchTree (MakeImageKernel.cu:27-96) with picheck = 0
+        float nodeStart[2] = {0.f, 0.f};
+        float nodeSize[2] = {BT_FPI, BT_FPI2};
+        int node = 0;
+        unsigned bitMask = 1u << treeLevel;
+        int level = 0;
+        int lvl = 0;
+        while (bitMask != 0) {
+            bitMask &= ~(1u << (treeLevel - level));
+            for (lvl = level + 1; lvl <= treeLevel; lvl++) {
+                int star_n = tree[node];
+                if (node != 0 && ((node + 1) & node) != 0) star_n -= tree[node - 1];
+                int tp = lvl & 1;
+                float x_overlap = fmaxf(0.f, fminf(mx[0], nodeStart[0] + nodeSize[0]) - fmaxf(mn[0], nodeStart[0]));
+                float y_overlap = fmaxf(0.f, fminf(mx[1], nodeStart[1] + nodeSize[1]) - fmaxf(mn[1], nodeStart[1]));
+                float overlapArea = x_overlap * y_overlap;
+                bool size = overlapArea / (nodeSize[0] * nodeSize[1]) > 0.8f;
+                nodeSize[tp] = nodeSize[tp] * .5f;
+                if (star_n == 0) {
+                    node = node * 2 + 1;
+                    break;
+                }
+                float check = nodeStart[tp] + nodeSize[tp];
+                bool lu = mn[tp] < check;
+                bool rd = mx[tp] >= check;
+                if (lvl == treeLevel || (rd && lu && size)) {
+                    if (rd) visit(node * 2 + 2);
+                    if (lu) visit(node * 2 + 1);
+                    node = node * 2 + 1;
+                    break;
+                } else {
+                    node = node * 2 + 1;
+                    if (rd) bitMask |= 1u << (treeLevel - lvl);
+                    if (!lu) break;
+                }
+            }
+            level = treeLevel - (__ffs((int)bitMask)) + 1;
+            if (level >= 0) {
+                int diff = lvl - level;
+                for (int q = 0; q < diff; q++) {
+                    int tp = (lvl - q) & 1;
+                    if (!(node & 1)) nodeStart[tp] -= nodeSize[tp];
+                    nodeSize[tp] = nodeSize[tp] * 2.f;
+                    node = (node - 1) / 2;
+                }
+                node++;
+                int tp = level & 1;
+                nodeStart[tp] += nodeSize[tp];
+            }
+        }
+    }
+    // tap-major layout: taps[(tap * 3 + c) * NP + ij]
+#pragma unroll
+    for (int u = 0; u < FW * FW; u++) {
+        taps[(size_t)(u * 3 + 0) * NP + ij] = acc[u][0];
+        taps[(size_t)(u * 3 + 1) * NP + ij] = acc[u][1];
+        taps[(size_t)(u * 3 + 2) * NP + ij] = acc[u][2];
+    }
+}
+
+// sumStarLight (MakeImageKernel.cu:753-773, bounds-checked) + makePix (:802-835)
+template <int STEP>
+__global__ __launch_bounds__(256) void star_sum(const float* taps, int W, int H, uint8_t* img, float* o_star) {
+    constexpr int FW = 2 * STEP + 1;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= H || j >= W) return;
+    const size_t NP = (size_t)W * H;
+    float br = 0.f, bg = 0.f, bb = 0.f;
+    int start = max(0, STEP - i);
+    int stop = min(2 * STEP, STEP + H - i - 1);
+    const float factor = 20.f;
+    for (int u = start; u <= stop; u++)
+        for (int v = 0; v <= 2 * STEP; v++) {
+            size_t src = (size_t)(i + u - STEP) * W + ((j + v - STEP + W) % W);
+            int tap = FW * FW - (FW * u + v + 1);
+            br += factor * taps[(size_t)(tap * 3 + 0) * NP + src];
+            bg += factor * taps[(size_t)(tap * 3 + 1) * NP + src];
+            bb += factor * taps[(size_t)(tap * 3 + 2) * NP + src];
+        }
+    const size_t ij = (size_t)i * W + j;
+    if (o_star) {
+        o_star[3 * ij] = br;
+        o_star[3 * ij + 1] = bg;
+        o_star[3 * ij + 2] = bb;
+    }
+    float r = sqrtf(br), g = sqrtf(bg), b = sqrtf(bb);
+    float mxc = fmaxf(fmaxf(r, g), b);
+    if (mxc > 255.f) {
+        float hh, ss, pp;
+        rgb_to_hsp(r / 255.f, g / 255.f, b / 255.f, hh, ss, pp);
+        pp = 1.0f;
+        hsp_to_rgb(hh, ss, pp, r, g, b);
+        r *= 255;
+        g *= 255;
+        b *= 255;
+    }
+    uchar4 o;
+    o.x = (uint8_t)min(255, (int)(b + 0.f));
+    o.y = (uint8_t)min(255, (int)(g + 0.f));
+    o.z = (uint8_t)min(255, (int)(r + 0.f));
+    o.w = 255;
+    reinterpret_cast<uchar4*>(img)[ij] = o;
+}
+
+// distortEnvironmentMap (MakeImageKernel.cu:631-751)
+__global__ __launch_bounds__(256) void env_map(const float* thphi, const uint8_t* bh, SceneK s, Cam7 cam, int W,
+                                               int H, float offset, int lr, const float* area,
+                                               const float* view, uint8_t* img) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= H || j >= W) return;
+    const size_t ij = (size_t)i * W + j;
+    const int rows = s.rows, cols = s.cols;
+    const float4* table = reinterpret_cast<const float4*>(s.table);
+    float cx = 0.f, cy = 0.f, cz = 0.f, cw = 0.f;
+    if (bh[ij] == 0) {
+        float t[4], p[4];
+        int ind = i * (W + 1) + j;
+        bool picheck = false;
+        corners(thphi, W, ind, t, p, picheck, offset);
+        if (ind > 0) {
+            float pixSize = BT_FPI / (float)rows;
+            float phMax = fmaxf(fmaxf(p[0], p[1]), fmaxf(p[2], p[3]));
+            float phMin = fminf(fminf(p[0], p[1]), fminf(p[2], p[3]));
+            int pixMax = (int)(phMax / pixSize);
+            int pixMin = (int)(phMin / pixSize);
+            int pixNum = pixMax - pixMin + 1;
+            if (pixNum > 1 && pixMax * pixSize > phMax) pixNum -= 1;
+            float thMax = fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3]));
+            float thMin = fminf(fminf(t[0], t[1]), fminf(t[2], t[3]));
+            if (pixNum == 1) {
+                int rmax = min(max((int)(thMax / pixSize), 0), rows - 1);
+                int rmin = (int)(thMin / pixSize) - 1;
+                float4 mxc = table[(size_t)rmax * cols + pixMin];
+                float4 mnc = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (rmin >= 0) mnc = table[(size_t)min(rmin, rows - 1) * cols + pixMin];
+                cx += mxc.x - mnc.x;
+                cy += mxc.y - mnc.y;
+                cz += mxc.z - mnc.z;
+                cw += mxc.w - mnc.w;
+            } else {
+                float max1 = -100, min1 = cols + 1, max2 = -100, min2 = cols + 1;
+                if (pixNum < cols) {
+                    for (int sidx = 0; sidx <= pixNum; sidx++) {
+                        float cp = (pixMin + sidx) * pixSize;
+                        for (int q = 0; q < 4; q++) {
+                            float ap = p[q], bp = p[(q + 1) % 4];
+                            float at = t[q], bt = t[(q + 1) % 4];
+                            if ((cp - ap) * (cp - bp) <= 0) {
+                                float ct = at + (bt - at) / (bp - ap) * (cp - ap);
+                                min2 = ct < min2 ? ct : min2;
+                                max2 = ct > max2 ? ct : max2;
+                            }
+                        }
+                        if (sidx > 0) {
+                            float max_ = fmaxf(max1, max2);
+                            float min_ = fminf(min1, min2);
+                            int rmax = min(max((int)(max_ / pixSize), 0), rows - 1);
+                            int col = (pixMin + sidx) % cols;
+                            float4 mxc = table[(size_t)rmax * cols + col];
+                            int index2 = ((int)(min_ / pixSize) - 1) * cols + col;
+                            float4 mnc = make_float4(0.f, 0.f, 0.f, 0.f);
+                            if (index2 > 0) {
+                                if (index2 >= rows * cols) index2 = (rows - 1) * cols + col;
+                                mnc = table[index2];
+                            }
+                            cx += mxc.x - mnc.x;
+                            cy += mxc.y - mnc.y;
+                            cz += mxc.z - mnc.z;
+                            cw += mxc.w - mnc.w;
+                        }
+                        max1 = max2;
+                        min1 = min2;
+                    }
+                } else {
+                    int thMaxPix = min(max((int)(thMax / pixSize), 0), rows - 1) * cols;
+                    int thMinPix = min(max((int)(thMin / pixSize), 0), rows - 1) * cols;
+                    for (int q = 0; q < pixNum; q++) {
+                        float4 mxc = table[thMaxPix + (pixMin + q) % cols];
+                        float4 mnc = table[thMinPix + (pixMin + q) % cols];
+                        cx += mxc.x - mnc.x;
+                        cy += mxc.y - mnc.y;
+                        cz += mxc.z - mnc.z;
+                        cw += mxc.w - mnc.w;
+                    }
+                }
+            }
+            cx = fminf(255.f, f_pow(cx / cw, 1.f / 2.2f));
+            cy = fminf(255.f, f_pow(cy / cw, 1.f / 2.2f));
+            cz = fminf(255.f, f_pow(cz / cw, 1.f / 2.2f));
+            float Hh, Ss, Pp;
+            rgb_to_hsp(cz / 255.f, cy / 255.f, cx / 255.f, Hh, Ss, Pp);
+            if (lr) {
+                float redshft, frac;
+                lensing_redshift(t, p, W, ind, cam, view, frac, redshft, area[ij]);
+                Pp *= frac;
+                Pp = f_pow(Pp, redshft);
+            }
+            hsp_to_rgb(Hh, Ss, fminf(1.f, Pp), cz, cy, cx);
+        }
+    }
+    uchar4 o;
+    o.x = (uint8_t)min(255, (int)(cx * 255));
+    o.y = (uint8_t)min(255, (int)(cy * 255));
+    o.z = (uint8_t)min(255, (int)(cz * 255));
+    o.w = 255;
+    reinterpret_cast<uchar4*>(img)[ij] = o;
+}
+
+// addStarsAndBackground (MakeImageKernel.cu:837-853)
+__global__ __launch_bounds__(256) void composite(const uint8_t* img_star, const uint8_t* img_map, int64_t n,
+                                                 int star, int map, uint8_t* out) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    uchar4 o;
+    if (star && map) {
+        uchar4 s = reinterpret_cast<const uchar4*>(img_star)[q];
+        uchar4 b = reinterpret_cast<const uchar4*>(img_map)[q];
+        float p = 1.f;
+        float sx = (float)s.x * s.x, sy = (float)s.y * s.y, sz = (float)s.z * s.z;
+        float bx = (float)b.x * b.x, by = (float)b.y * b.y, bz = (float)b.z * b.z;
+        o.x = (uint8_t)min((int)sqrtf(p * sx + (2.f - p) * bx), 255);
+        o.y = (uint8_t)min((int)sqrtf(p * sy + (2.f - p) * by), 255);
+        o.z = (uint8_t)min((int)sqrtf(p * sz + (2.f - p) * bz), 255);
+        o.w = 255;
+    } else if (map) {
+        o = reinterpret_cast<const uchar4*>(img_map)[q];
+    } else if (star) {
+        o = reinterpret_cast<const uchar4*>(img_star)[q];
+    } else {
+        o = make_uchar4(0, 0, 0, 255);
+    }
+    reinterpret_cast<uchar4*>(out)[q] = o;
+}
+
+// Summed-area sky table (MakeImageKernel.cu:1312-1331): one thread per
+// column, prefix sums down the rows in the reference's order.
+__global__ __launch_bounds__(64) void sum_table(const uint8_t* sky, int rows, int cols, float* table) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= cols) return;
+    float pxsz = BT_FPI2 / (1.f * cols);
+    float prev0 = 0.f, prev1 = 0.f, prev2 = 0.f, prev3 = 0.f;
+    float phi[4] = {pxsz * q, pxsz * q, pxsz * (q + 1), pxsz * (q + 1)};
+    for (int p = 0; p < rows; p++) {
+        float theta[4] = {pxsz * (p + 1), pxsz * p, pxsz * p, pxsz * (p + 1)};
+        float area = calc_area4(theta, phi);
+        const uint8_t* px = sky + 3 * ((size_t)p * cols + q);
+        prev0 += f_pow((float)px[0], 2.2f) * area;
+        prev1 += f_pow((float)px[1], 2.2f) * area;
+        prev2 += f_pow((float)px[2], 2.2f) * area;
+        prev3 += area;
+        reinterpret_cast<float4*>(table)[(size_t)p * cols + q] = make_float4(prev0, prev1, prev2, prev3);
+    }
+}
+
+// makeGrid (MakeImageKernel.cu:369-384) with hashLookup (kernel.cuh:705-733), G == 1.
+__global__ void make_grid(const float* ht, const int32_t* ot, const int32_t* tag, int hw, int ow, int GN, int GM,
+                          int sym, int GN1, float* grid) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= GN || j >= GM) return;
+    int ix = (i + ow) % ow, iy = (j + ow) % ow;
+    int ax = (i + hw) % hw + ot[2 * (ix * ow + iy)];
+    int ay = (j + hw) % hw + ot[2 * (ix * ow + iy) + 1];
+    int hx = (ax + hw) % hw, hy = (ay + hw) % hw;
+    int sidx = hx * hw + hy;
+    float vx = -2.f, vy = -2.f;
+    if (tag[2 * sidx] == i && tag[2 * sidx + 1] == j) {
+        vx = ht[2 * sidx];
+        vy = ht[2 * sidx + 1];
+    }
+    float2* g2 = reinterpret_cast<float2*>(grid);
+    g2[(size_t)i * GM + j] = make_float2(vx, vy);
+    if (sym == 1) {
+        if (vx != -1 && vx != -2) vx = BT_FPI - vx;
+        int x = GN1 - i - 1;
+        g2[(size_t)x * GM + j] = make_float2(vx, vy);
+    }
+}
+
+// Dense FP32 grid from a device-built FP64 grid: the values saveAsGpuHash
+// would store (Grid.h:645-650), without the hash round trip.
+__global__ void grid_to_f32(const double* cel, int GN, int GM, int sym, int GN1, float* grid) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= GN || j >= GM) return;
+    size_t o = (size_t)i * GM + j;
+    float vx = (float)cel[2 * o], vy = (float)cel[2 * o + 1];
+    float2* g2 = reinterpret_cast<float2*>(grid);
+    g2[o] = make_float2(vx, vy);
+    if (sym == 1) {
+        if (vx != -1 && vx != -2) vx = BT_FPI - vx;
+        int x = GN1 - i - 1;
+        g2[(size_t)x * GM + j] = make_float2(vx, vy);
+    }
+}
+
+dim3 grid2d(int W, int H) { return dim3((unsigned)((W + 15) / 16), (unsigned)((H + 15) / 16)); }
+
+}  // namespace
+
+int render_sum_table(Context* c, const uint8_t* d_sky, int rows, int cols, float* d_table) {
+    KTimer t(c, "sum_table");
+    sum_table<<<(unsigned)((cols + 63) / 64), 64, 0, c->stream>>>(d_sky, rows, cols, d_table);
+    BT_HIP(hipGetLastError());
+    return BT_OK;
+}
+
+int render_grid_from_psh(Context* c, const float* d_ht, const int32_t* d_ot, const int32_t* d_tag, int hw,
+                         int ow, int GN, int GM, float* d_grid, int* GN1_out) {
+    int sym = (float)GM / (float)GN > 3 ? 1 : 0;
+    int GN1 = sym ? 2 * GN - 1 : GN;
+    KTimer t(c, "make_grid");
+    make_grid<<<grid2d(GM, GN), dim3(16, 16), 0, c->stream>>>(d_ht, d_ot, d_tag, hw, ow, GN, GM, sym, GN1, d_grid);
+    BT_HIP(hipGetLastError());
+    *GN1_out = GN1;
+    return BT_OK;
+}
+
+int render_grid_from_cel(Context* c, const double* d_cel, int GN, int GM, float* d_grid, int* GN1_out) {
+    int sym = (float)GM / (float)GN > 3 ? 1 : 0;
+    int GN1 = sym ? 2 * GN - 1 : GN;
+    KTimer t(c, "grid_to_f32");
+    grid_to_f32<<<grid2d(GM, GN), dim3(16, 16), 0, c->stream>>>(d_cel, GN, GM, sym, GN1, d_grid);
+    BT_HIP(hipGetLastError());
+    *GN1_out = GN1;
+    return BT_OK;
+}
+
+int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, const float* cam7,
+                 const bt_render_flags& f, int step, uint8_t* d_out, FrameDebug* dbg) {
+    const int W = v.W, H = v.H;
+    const size_t NP = (size_t)W * H, NC = (size_t)(W + 1) * (H + 1);
+    if (step != 1 && step != 2 && step != 3)
+        return fail(BT_ERR_INVALID, "render: star splat step must be 1, 2 or 3");
+    void* p;
+    int rc;
+    if ((rc = c->buffer("r_thphi", sizeof(float) * 2 * NC, &p))) return rc;
+    float* thphi = (float*)p;
+    if ((rc = c->buffer("r_gap", sizeof(int) * NC, &p))) return rc;
+    int* gap = (int*)p;
+    if ((rc = c->buffer("r_bh", NP, &p))) return rc;
+    uint8_t* bh = (uint8_t*)p;
+    if ((rc = c->buffer("r_area", sizeof(float) * NP, &p))) return rc;
+    float* area = (float*)p;
+    if ((rc = c->buffer("r_area_s", sizeof(float) * NP, &p))) return rc;
+    float* area_s = (float*)p;
+    if ((rc = c->buffer("r_img_star", 4 * NP, &p))) return rc;
+    uint8_t* img_star = (uint8_t*)p;
+    if ((rc = c->buffer("r_img_map", 4 * NP, &p))) return rc;
+    uint8_t* img_map = (uint8_t*)p;
+
+    Cam7 cam;
+    for (int q = 0; q < 7; q++) cam.v[q] = cam7[q];
+    GridV gv{g.grid, g.GN, g.GM};
+    const dim3 blk(16, 16);
+    {
+        KTimer t(c, "pix_interp");
+        pix_interp<<<grid2d(W + 1, H + 1), blk, 0, c->stream>>>(gv, g.gridlvl, v.view, W, H, f.hor, f.ver, thphi, gap);
+    }
+    {
+        KTimer t(c, "bh_area");
+        bh_area<<<grid2d(W, H), blk, 0, c->stream>>>(thphi, W, H, bh, area);
+    }
+    {
+        KTimer t(c, "smooth");
+        smooth_h<<<grid2d(W, H), blk, 0, c->stream>>>(area, bh, gap, W, H, area_s);
+        smooth_v<<<grid2d(W, H), blk, 0, c->stream>>>(area_s, bh, gap, W, H, area);
+    }
+    BT_HIP(hipGetLastError());
+    if (f.star) {
+        const int fw = 2 * step + 1;
+        if ((rc = c->buffer("r_taps", sizeof(float) * 3 * fw * fw * NP, &p))) return rc;
+        float* taps = (float*)p;
+        {
+            KTimer t(c, "star_map");
+            if (step == 1)
+                star_map<1><<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view, taps);
+            else if (step == 2)
+                star_map<2><<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view, taps);
+            else
+                star_map<3><<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view, taps);
+        }
+        {
+            KTimer t(c, "star_sum");
+            float* ostar = dbg ? dbg->star : nullptr;
+            if (step == 1)
+                star_sum<1><<<grid2d(W, H), blk, 0, c->stream>>>(taps, W, H, img_star, ostar);
+            else if (step == 2)
+                star_sum<2><<<grid2d(W, H), blk, 0, c->stream>>>(taps, W, H, img_star, ostar);
+            else
+                star_sum<3><<<grid2d(W, H), blk, 0, c->stream>>>(taps, W, H, img_star, ostar);
+        }
+        BT_HIP(hipGetLastError());
+    }
+    if (f.map) {
+        KTimer t(c, "env_map");
+        env_map<<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view, img_map);
+        BT_HIP(hipGetLastError());
+    }
+    {
+        KTimer t(c, "composite");
+        composite<<<(unsigned)((NP + 255) / 256), 256, 0, c->stream>>>(img_star, img_map, (int64_t)NP, f.star, f.map,
+                                                                     d_out);
+    }
+    BT_HIP(hipGetLastError());
+    if (dbg) {
+        if (dbg->thphi) BT_HIP(hipMemcpyAsync(dbg->thphi, thphi, sizeof(float) * 2 * NC, hipMemcpyDeviceToDevice, c->stream));
+        if (dbg->gap) BT_HIP(hipMemcpyAsync(dbg->gap, gap, sizeof(int) * NC, hipMemcpyDeviceToDevice, c->stream));
+        if (dbg->bh) BT_HIP(hipMemcpyAsync(dbg->bh, bh, NP, hipMemcpyDeviceToDevice, c->stream));
+        if (dbg->area) BT_HIP(hipMemcpyAsync(dbg->area, area, sizeof(float) * NP, hipMemcpyDeviceToDevice, c->stream));
+    }
+    return BT_OK;
+}
+
+}  // namespace bt

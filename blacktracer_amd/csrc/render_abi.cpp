@@ -1,21 +1,445 @@
-// render_abi.cpp -- C ABI of the render pass (makeImage).
-#include "bt_internal.h"
-using namespace bt;
-extern "C" {
-void bt_render_flags_default(bt_render_flags* f) {
-    if (!f) return;
-    f->star = 1; f->map = 1; f->lr = 1; f->hor = 0.f; f->ver = 0.f; f->offset = 0.f;
-}
+// render_abi.cpp -- C ABI of the render pass: the makeImage drop-in
+// (MakeImageKernel.cu:1206-1236) and the device-resident scene / view /
+// frame objects used for serving.  Also the host-side inputs the reference
+// computes on the CPU: Viewer::makeCameraView (Viewer.h:109-133) and
+// StarProcessor::makeTree (StarProcessor.h:172-212).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bt_grid.h"
+#include "bt_render.h"
+
+namespace bt {
+
+static const double kPI = 3.141592653589793238;   // Const.h:2-4
+static const double kPI1_2 = kPI / 2.;
+static const double kPI2 = kPI * 2.;
+
+// Viewer.h:20-33 + 109-133 (perspective) and 82-88 (sphere)
+int viewer_matrix(double viewangle, double offsetVer, double offsetHor, int pixw, int pixh, int sphere,
+                  float* view) {
+    if (pixw <= 0 || pixh <= 0 || !view) return fail(BT_ERR_INVALID, "viewer: bad size or null output");
+    const size_t n = (size_t)(pixh + 1) * (pixw + 1);
+    std::memset(view, 0, sizeof(float) * 2 * n);
+    if (sphere) {
+        for (int i = 0; i < pixh + 1; i++)
+            for (int j = 0; j < pixw + 1; j++) {
+                size_t o = (size_t)i * (pixw + 1) + j;
+                view[2 * o] = (float)(1.f * i * kPI / pixh);
+                view[2 * o + 1] = (float)(1.f * j * kPI2 / pixw);
+            }
+        return BT_OK;
+    }
+    double viewAngleWide = viewangle;
+    double viewAngleUp = 1. * pixh * viewAngleWide / (1. * pixw);
+    if (std::fabs(offsetVer) > (kPI - viewAngleUp) / 2.)
+        return fail(BT_ERR_INVALID, "viewer: offsetUp too big");   // Viewer.h:110-113
+    float yleft = (float)std::tan(viewAngleWide / 2.f);
+    float step = (2.f * yleft) / pixw;
+    float zup = yleft * pixh / pixw;
+    for (int i = 0; i < pixh + 1; i++) {
+        float zval = zup - i * step;
+        float halfH = i * 1.f / (pixh / 2.f);
+        float theta = halfH == 1.f ? (float)(kPI1_2 - offsetVer)
+                                   : (float)(std::atan(1. / zval) + (halfH > 1.f) * kPI - offsetVer);
+        for (int j = 0; j < pixw + 1; j++) {
+            float yval = yleft - j * step;
+            float halfV = j * 1.f / (pixw / 2.f);
+            float phi = halfV == 1.f ? (float)(kPI - offsetHor)
+                                     : (float)(std::atan(1. / yval) + (halfV > 1.f) * kPI + kPI1_2 - offsetHor);
+            size_t o = (size_t)(pixw + 1) * i + pixw - j;
+            view[2 * o] = theta;
+            view[2 * o + 1] = phi;
+        }
+    }
+    return BT_OK;
 }
 
-extern "C" int bt_make_image(bt_context* ctx, const float* stars, const int32_t* star_tree,
-                             int star_size, const float* cam_param, const float* magnitude,
-                             int tree_level, int M, int N, int step, const uint8_t* sky_bgr,
-                             int sky_rows, int sky_cols, int G, float grid_start, float grid_step,
-                             const float* hit, const float* viewthing, float view_angle, int GM,
-                             int GN, int gridlvl, const int32_t* offset_tables,
-                             const float* hash_tables, const int32_t* hash_pos_tag,
-                             const int32_t* table_sizes, int otsize, int htsize,
-                             const bt_render_flags* flags, uint8_t* out_bgra) {
-    return fail(BT_ERR_STATE, "render pass not built yet");
+namespace {
+struct TreeBuilder {
+    const float* s4;
+    int level_max;
+    int32_t* tree;
+    float* pos;
+    float* mag;
+    // StarProcessor.h:172-212, stars as an index list
+    void build(std::vector<int>& idx, int level, float thphi[2], float size[2], int writePos) {
+        const int n = (int)idx.size();
+        int searchPos = 0;
+        if (writePos != 0 && ((writePos + 1) & writePos) != 0) searchPos = tree[writePos - 1];
+        tree[writePos] = searchPos + n;
+        if (level == level_max) {
+            for (int i = 0; i < n; i++) {
+                const float* s = s4 + 4 * (size_t)idx[i];
+                pos[(i + searchPos) * 2] = s[0];
+                pos[(i + searchPos) * 2 + 1] = s[1];
+                mag[(i + searchPos) * 2] = s[2];
+                mag[(i + searchPos) * 2 + 1] = s[3];
+            }
+            return;
+        }
+        level++;
+        int nn = level % 2;
+        size[nn] = size[nn] * .5f;
+        std::vector<int> lu, rd;
+        lu.reserve(n);
+        rd.reserve(n);
+        float check = thphi[nn] + size[nn];
+        for (int i : idx) (s4[4 * (size_t)i + nn] < check ? lu : rd).push_back(i);
+        std::vector<int>().swap(idx);
+        float sizeNew[2] = {size[0], size[1]};
+        float thphiNew[2] = {thphi[0], thphi[1]};
+        build(lu, level, thphiNew, sizeNew, writePos * 2 + 1);
+        float thphiNew2[2] = {thphi[0], thphi[1]};
+        float sizeNew2[2] = {size[0], size[1]};
+        thphiNew2[nn] += size[nn];
+        build(rd, level, thphiNew2, sizeNew2, writePos * 2 + 2);
+    }
+};
+}  // namespace
+
+int star_tree(const float* stars4, int S, int tree_level, int32_t* tree, float* pos, float* mag) {
+    if (S < 0 || tree_level < 1 || tree_level > 20 || (S > 0 && !stars4) || !tree || (S > 0 && (!pos || !mag)))
+        return fail(BT_ERR_INVALID, "star_tree: bad arguments");
+    TreeBuilder b{stars4, tree_level, tree, pos, mag};
+    std::vector<int> idx(S);
+    for (int i = 0; i < S; i++) idx[i] = i;
+    float thphi[2] = {0, 0};
+    float size[2] = {(float)kPI, (float)kPI2};
+    b.build(idx, 0, thphi, size, 0);
+    return BT_OK;
 }
+
+static void cam7_of(const PoseK& P, float* cam7) {
+    // Camera::getParamArray, Camera.h:67-78
+    cam7[0] = (float)P.speed;
+    cam7[1] = (float)P.alpha;
+    cam7[2] = (float)P.w;
+    cam7[3] = (float)P.wbar;
+    cam7[4] = (float)P.br;
+    cam7[5] = (float)P.btheta;
+    cam7[6] = (float)P.bphi;
+}
+
+}  // namespace bt
+
+using namespace bt;
+
+struct bt_scene {
+    Context* ctx = nullptr;
+    int32_t* tree = nullptr;
+    float* pos = nullptr;
+    float* mag = nullptr;
+    float* table = nullptr;
+    int tree_level = 0, S = 0, rows = 0, cols = 0;
+    size_t bytes = 0;
+    SceneK k() const { return SceneK{tree, tree_level, pos, mag, S, table, rows, cols}; }
+};
+
+struct bt_view {
+    Context* ctx = nullptr;
+    float* view = nullptr;
+    int W = 0, H = 0;
+    size_t bytes = 0;
+};
+
+static void scene_release(bt_scene* s) {
+    if (s->tree) hipFree(s->tree);
+    if (s->pos) hipFree(s->pos);
+    if (s->mag) hipFree(s->mag);
+    if (s->table) hipFree(s->table);
+    s->tree = nullptr;
+    s->pos = s->mag = s->table = nullptr;
+}
+
+// Uploads an already built star tree and builds the summed sky table on the device.
+static int scene_upload(Context& c, bt_scene* s, const int32_t* tree, int tree_level, const float* pos,
+                        const float* mag, int S, const uint8_t* sky, int rows, int cols) {
+    if (rows <= 0 || cols <= 0 || !sky) return fail(BT_ERR_INVALID, "scene: sky image required");
+    const size_t tsz = (size_t)((1 << (tree_level + 1)) - 1);
+    s->tree_level = tree_level;
+    s->S = S;
+    s->rows = rows;
+    s->cols = cols;
+    BT_HIP(hipMalloc((void**)&s->tree, sizeof(int32_t) * tsz));
+    BT_HIP(hipMalloc((void**)&s->pos, sizeof(float) * 2 * (S > 0 ? S : 1)));
+    BT_HIP(hipMalloc((void**)&s->mag, sizeof(float) * 2 * (S > 0 ? S : 1)));
+    BT_HIP(hipMalloc((void**)&s->table, sizeof(float) * 4 * (size_t)rows * cols));
+    s->bytes = sizeof(int32_t) * tsz + sizeof(float) * 4 * (size_t)S + sizeof(float) * 4 * (size_t)rows * cols;
+    BT_HIP(hipMemcpyAsync(s->tree, tree, sizeof(int32_t) * tsz, hipMemcpyHostToDevice, c.stream));
+    if (S > 0) {
+        BT_HIP(hipMemcpyAsync(s->pos, pos, sizeof(float) * 2 * S, hipMemcpyHostToDevice, c.stream));
+        BT_HIP(hipMemcpyAsync(s->mag, mag, sizeof(float) * 2 * S, hipMemcpyHostToDevice, c.stream));
+    }
+    void* dsky;
+    int rc;
+    if ((rc = c.buffer("scene_sky_upload", (size_t)3 * rows * cols, &dsky))) return rc;
+    BT_HIP(hipMemcpyAsync(dsky, sky, (size_t)3 * rows * cols, hipMemcpyHostToDevice, c.stream));
+    if ((rc = render_sum_table(&c, (const uint8_t*)dsky, rows, cols, s->table))) return rc;
+    BT_HIP(hipStreamSynchronize(c.stream));
+    return BT_OK;
+}
+
+extern "C" {
+
+void bt_render_flags_default(bt_render_flags* f) {
+    if (!f) return;
+    f->star = 1;
+    f->map = 1;
+    f->lr = 1;
+    f->hor = 0.f;
+    f->ver = 0.f;
+    f->offset = 0.f;
+}
+
+int bt_viewer_matrix(double view_angle, double off_up, double off_right, int width, int height, int sphere,
+                     float* out) {
+    return viewer_matrix(view_angle, off_up, off_right, width, height, sphere, out);
+}
+
+int bt_star_tree(const float* stars4, int star_count, int tree_level, int32_t* tree, float* star_pos,
+                 float* star_mag) {
+    return star_tree(stars4, star_count, tree_level, tree, star_pos, star_mag);
+}
+
+int bt_scene_create(bt_context* h, const float* stars4, int star_count, int tree_level, const uint8_t* sky_bgr,
+                    int sky_rows, int sky_cols, bt_scene** out) {
+    if (!h || !out) return fail(BT_ERR_INVALID, "bt_scene_create: null argument");
+    *out = nullptr;
+    if (tree_level < 1 || tree_level > 20 || star_count < 0)
+        return fail(BT_ERR_INVALID, "bt_scene_create: tree_level in [1, 20], star_count >= 0");
+    Context& c = h->impl;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    BT_HIP(hipSetDevice(c.device));
+    std::vector<int32_t> tree((size_t)((1 << (tree_level + 1)) - 1));
+    std::vector<float> pos(2 * (size_t)star_count), mag(2 * (size_t)star_count);
+    int rc = star_tree(stars4, star_count, tree_level, tree.data(), pos.data(), mag.data());
+    if (rc) return rc;
+    bt_scene* s = new bt_scene();
+    s->ctx = &c;
+    rc = scene_upload(c, s, tree.data(), tree_level, pos.data(), mag.data(), star_count, sky_bgr, sky_rows, sky_cols);
+    if (rc) {
+        hipStreamSynchronize(c.stream);
+        scene_release(s);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return BT_OK;
+}
+
+int bt_scene_export_table(const bt_scene* s, float* table) {
+    if (!s || !table) return fail(BT_ERR_INVALID, "null argument");
+    Context& c = *s->ctx;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    BT_HIP(hipSetDevice(c.device));
+    BT_HIP(hipMemcpyAsync(table, s->table, sizeof(float) * 4 * (size_t)s->rows * s->cols, hipMemcpyDeviceToHost,
+                          c.stream));
+    BT_HIP(hipStreamSynchronize(c.stream));
+    return BT_OK;
+}
+
+void bt_scene_free(bt_scene* s) {
+    if (!s) return;
+    if (s->ctx) {
+        std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+        hipSetDevice(s->ctx->device);
+        hipStreamSynchronize(s->ctx->stream);
+        scene_release(s);
+    }
+    delete s;
+}
+
+int bt_view_create(bt_context* h, double view_angle, double off_up, double off_right, int width, int height,
+                   int sphere, bt_view** out) {
+    if (!h || !out) return fail(BT_ERR_INVALID, "bt_view_create: null argument");
+    *out = nullptr;
+    if (width <= 0 || height <= 0 || width > 16384 || height > 16384)
+        return fail(BT_ERR_INVALID, "bt_view_create: size out of range");
+    Context& c = h->impl;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    BT_HIP(hipSetDevice(c.device));
+    const size_t n = (size_t)(width + 1) * (height + 1);
+    std::vector<float> host(2 * n);
+    int rc = viewer_matrix(view_angle, off_up, off_right, width, height, sphere, host.data());
+    if (rc) return rc;
+    bt_view* v = new bt_view();
+    v->ctx = &c;
+    v->W = width;
+    v->H = height;
+    v->bytes = sizeof(float) * 2 * n;
+    if (hipMalloc((void**)&v->view, v->bytes) != hipSuccess) {
+        delete v;
+        return fail(BT_ERR_NOMEM, "bt_view_create: hipMalloc failed");
+    }
+    hipError_t e = hipMemcpy(v->view, host.data(), v->bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(v->view);
+        delete v;
+        return fail(BT_ERR_HIP, std::string("bt_view_create: ") + hipGetErrorString(e));
+    }
+    *out = v;
+    return BT_OK;
+}
+
+void bt_view_free(bt_view* v) {
+    if (!v) return;
+    if (v->ctx) {
+        std::lock_guard<std::recursive_mutex> lk(v->ctx->mu);
+        hipSetDevice(v->ctx->device);
+        hipStreamSynchronize(v->ctx->stream);
+        if (v->view) hipFree(v->view);
+    }
+    delete v;
+}
+
+// Converts (and caches per grid) the FP64 device grid into the FP32 dense
+// grid the render kernels read.
+static int grid_f32_of(Context& c, const bt_grid* g, GridF* out) {
+    void* p;
+    int rc;
+    int sym = (float)g->M / (float)g->N > 3 ? 1 : 0;
+    int GN1 = sym ? 2 * g->N - 1 : g->N;
+    if ((rc = c.buffer("r_gridf", sizeof(float) * 2 * (size_t)GN1 * g->M, &p))) return rc;
+    int gn1 = 0;
+    if ((rc = render_grid_from_cel(&c, g->k.cel, g->N, g->M, (float*)p, &gn1))) return rc;
+    *out = GridF{(const float*)p, gn1, g->M, g->max_level};
+    return BT_OK;
+}
+
+int bt_render_frame_stages(bt_context* h, const bt_grid* g, const bt_scene* s, const bt_view* v, const float* cam7,
+                           const bt_render_flags* flags, int step, uint8_t* out_bgra, void* d_out_bgra,
+                           float* thphi, int32_t* gap, uint8_t* bh, float* area, float* star) {
+    if (!h || !g || !s || !v) return fail(BT_ERR_INVALID, "bt_render_frame: null handle");
+    if (!out_bgra && !d_out_bgra) return fail(BT_ERR_INVALID, "bt_render_frame: no output buffer");
+    Context& c = h->impl;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    BT_HIP(hipSetDevice(c.device));
+    bt_render_flags f;
+    bt_render_flags_default(&f);
+    if (flags) f = *flags;
+    float cam[7];
+    if (cam7)
+        std::memcpy(cam, cam7, sizeof(cam));
+    else
+        cam7_of(g->pose, cam);
+    GridF gf;
+    int rc;
+    if ((rc = grid_f32_of(c, g, &gf))) return rc;
+    const size_t NP = (size_t)v->W * v->H, NC = (size_t)(v->W + 1) * (v->H + 1);
+    void* dout = d_out_bgra;
+    if (!dout && (rc = c.buffer("r_out", 4 * NP, &dout))) return rc;
+    FrameDebug dbg{nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool want = thphi || gap || bh || area || star;
+    if (want) {
+        void* p;
+        size_t off = 0;
+        size_t tot = sizeof(float) * 2 * NC + sizeof(int) * NC + NP + sizeof(float) * NP + sizeof(float) * 3 * NP + 64;
+        if ((rc = c.buffer("r_dbg", tot, &p))) return rc;
+        char* base = (char*)p;
+        dbg.thphi = (float*)(base + off);
+        off += sizeof(float) * 2 * NC;
+        dbg.gap = (int*)(base + off);
+        off += sizeof(int) * NC;
+        dbg.area = (float*)(base + off);
+        off += sizeof(float) * NP;
+        dbg.star = (float*)(base + off);
+        off += sizeof(float) * 3 * NP;
+        dbg.bh = (uint8_t*)(base + off);
+    }
+    if ((rc = render_frame(&c, gf, s->k(), ViewK{v->view, v->W, v->H}, cam, f, step, (uint8_t*)dout,
+                           want ? &dbg : nullptr)))
+        return rc;
+    if (out_bgra) BT_HIP(hipMemcpyAsync(out_bgra, dout, 4 * NP, hipMemcpyDeviceToHost, c.stream));
+    if (thphi) BT_HIP(hipMemcpyAsync(thphi, dbg.thphi, sizeof(float) * 2 * NC, hipMemcpyDeviceToHost, c.stream));
+    if (gap) BT_HIP(hipMemcpyAsync(gap, dbg.gap, sizeof(int) * NC, hipMemcpyDeviceToHost, c.stream));
+    if (bh) BT_HIP(hipMemcpyAsync(bh, dbg.bh, NP, hipMemcpyDeviceToHost, c.stream));
+    if (area) BT_HIP(hipMemcpyAsync(area, dbg.area, sizeof(float) * NP, hipMemcpyDeviceToHost, c.stream));
+    if (star) BT_HIP(hipMemcpyAsync(star, dbg.star, sizeof(float) * 3 * NP, hipMemcpyDeviceToHost, c.stream));
+    if (out_bgra || want) BT_HIP(hipStreamSynchronize(c.stream));
+    return BT_OK;
+}
+
+int bt_render_frame(bt_context* h, const bt_grid* g, const bt_scene* s, const bt_view* v, const float* cam7,
+                    const bt_render_flags* flags, int step, uint8_t* out_bgra, void* d_out_bgra) {
+    return bt_render_frame_stages(h, g, s, v, cam7, flags, step, out_bgra, d_out_bgra, nullptr, nullptr, nullptr,
+                                  nullptr, nullptr);
+}
+
+int bt_make_image(bt_context* h, const float* stars, const int32_t* star_tree_in, int star_size,
+                  const float* cam_param, const float* magnitude, int tree_level, int M, int N, int step,
+                  const uint8_t* sky_bgr, int sky_rows, int sky_cols, int G, float grid_start, float grid_step,
+                  const float* hit, const float* viewthing, float view_angle, int GM, int GN, int gridlvl,
+                  const int32_t* offset_tables, const float* hash_tables, const int32_t* hash_pos_tag,
+                  const int32_t* table_sizes, int otsize, int htsize, const bt_render_flags* flags,
+                  uint8_t* out_bgra) {
+    (void)grid_start;
+    (void)grid_step;
+    (void)hit;
+    (void)view_angle;
+    if (!h) return fail(BT_ERR_INVALID, "bt_make_image: context is null");
+    if (G != 1) return fail(BT_ERR_INVALID, "bt_make_image: only G == 1 (one grid) is supported");
+    if (M <= 0 || N <= 0 || GM <= 0 || GN <= 0 || tree_level < 1 || tree_level > 20 || star_size < 0)
+        return fail(BT_ERR_INVALID, "bt_make_image: bad sizes");
+    if (!cam_param || !viewthing || !offset_tables || !hash_tables || !hash_pos_tag || !table_sizes ||
+        !out_bgra || !star_tree_in || (star_size > 0 && (!stars || !magnitude)))
+        return fail(BT_ERR_INVALID, "bt_make_image: null input");
+    const int hw = table_sizes[0], ow = table_sizes[1];
+    if (hw <= 0 || ow <= 0 || (size_t)hw * hw > (size_t)htsize || (size_t)ow * ow > (size_t)otsize)
+        return fail(BT_ERR_INVALID, "bt_make_image: table sizes inconsistent with htsize/otsize");
+    Context& c = h->impl;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    BT_HIP(hipSetDevice(c.device));
+    int rc;
+    // scene: the caller's star tree + summed table of the sky
+    bt_scene scene;
+    scene.ctx = &c;
+    if ((rc = scene_upload(c, &scene, star_tree_in, tree_level, stars, magnitude, star_size, sky_bgr, sky_rows,
+                           sky_cols))) {
+        hipStreamSynchronize(c.stream);
+        scene_release(&scene);
+        return rc;
+    }
+    // PSH tables -> dense grid (makeGrid)
+    void *dht, *dot, *dtag, *dgrid, *dview, *dout;
+    int sym = (float)GM / (float)GN > 3 ? 1 : 0;
+    int GN1 = sym ? 2 * GN - 1 : GN;
+    const size_t NP = (size_t)M * N, NC = (size_t)(M + 1) * (N + 1);
+    if ((rc = c.buffer("mi_ht", sizeof(float) * 2 * (size_t)htsize, &dht)) ||
+        (rc = c.buffer("mi_ot", sizeof(int32_t) * 2 * (size_t)otsize, &dot)) ||
+        (rc = c.buffer("mi_tag", sizeof(int32_t) * 2 * (size_t)htsize, &dtag)) ||
+        (rc = c.buffer("mi_grid", sizeof(float) * 2 * (size_t)GN1 * GM, &dgrid)) ||
+        (rc = c.buffer("mi_view", sizeof(float) * 2 * NC, &dview)) || (rc = c.buffer("mi_out", 4 * NP, &dout))) {
+        hipStreamSynchronize(c.stream);
+        scene_release(&scene);
+        return rc;
+    }
+    BT_HIP(hipMemcpyAsync(dht, hash_tables, sizeof(float) * 2 * (size_t)hw * hw, hipMemcpyHostToDevice, c.stream));
+    BT_HIP(hipMemcpyAsync(dot, offset_tables, sizeof(int32_t) * 2 * (size_t)ow * ow, hipMemcpyHostToDevice, c.stream));
+    BT_HIP(hipMemcpyAsync(dtag, hash_pos_tag, sizeof(int32_t) * 2 * (size_t)hw * hw, hipMemcpyHostToDevice, c.stream));
+    BT_HIP(hipMemcpyAsync(dview, viewthing, sizeof(float) * 2 * NC, hipMemcpyHostToDevice, c.stream));
+    int gn1 = 0;
+    if ((rc = render_grid_from_psh(&c, (const float*)dht, (const int32_t*)dot, (const int32_t*)dtag, hw, ow, GN, GM,
+                                   (float*)dgrid, &gn1)))
+        goto done;
+    {
+        bt_render_flags f;
+        bt_render_flags_default(&f);
+        if (flags) f = *flags;
+        rc = render_frame(&c, GridF{(const float*)dgrid, gn1, GM, gridlvl}, scene.k(), ViewK{(const float*)dview, M, N},
+                          cam_param, f, step, (uint8_t*)dout, nullptr);
+        if (rc) goto done;
+    }
+    {
+        hipError_t e = hipMemcpyAsync(out_bgra, dout, 4 * NP, hipMemcpyDeviceToHost, c.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+        if (e != hipSuccess) rc = fail(BT_ERR_HIP, std::string("bt_make_image: ") + hipGetErrorString(e));
+    }
+done:
+    hipStreamSynchronize(c.stream);
+    scene_release(&scene);
+    return rc;
+}
+
+}  // extern "C"

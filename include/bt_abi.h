@@ -166,6 +166,46 @@ int bt_make_image(bt_context* ctx, const float* stars, const int32_t* star_tree,
                   const int32_t* table_sizes, int otsize, int htsize,
                   const bt_render_flags* flags, uint8_t* out_bgra);
 
+/* ---- render inputs computed on the host (as the reference does) ---- */
+
+/* Viewer(viewangle, offUp, offRight, w, h, sphere).viewMatrix
+ * (Viewer.h:20-133): (height+1)*(width+1) float2 pixel-corner directions. */
+int bt_viewer_matrix(double view_angle, double off_up, double off_right, int width, int height,
+                     int sphere, float* out);
+/* StarProcessor::makeTree (StarProcessor.h:172-212): stars4 is
+ * star_count x (theta, phi, magnitude, B-V); tree has (1 << (L+1)) - 1
+ * ints; star_pos / star_mag 2*star_count floats in leaf order. */
+int bt_star_tree(const float* stars4, int star_count, int tree_level, int32_t* tree,
+                 float* star_pos, float* star_mag);
+
+/* ---- device-resident render objects (no counterpart in the reference,
+ * which re-uploads everything per makeImage call; used by the serving
+ * loop so that a frame touches only HBM-resident data) ---- */
+typedef struct bt_scene bt_scene;   /* star tree + summed-area sky table */
+typedef struct bt_view bt_view;     /* pixel-corner directions of one resolution */
+
+int  bt_scene_create(bt_context* ctx, const float* stars4, int star_count, int tree_level,
+                     const uint8_t* sky_bgr, int sky_rows, int sky_cols, bt_scene** out);
+/* Host copy of the summed-area table, sky_rows*sky_cols*4 floats. */
+int  bt_scene_export_table(const bt_scene* s, float* table);
+void bt_scene_free(bt_scene* s);
+int  bt_view_create(bt_context* ctx, double view_angle, double off_up, double off_right,
+                    int width, int height, int sphere, bt_view** out);
+void bt_view_free(bt_view* v);
+/* One frame from a device-resident grid.  cam7 null: the grid's own pose
+ * (getParamArray).  Output to a host buffer (width*height*4, BGRA) and/or a
+ * device buffer; with only a device buffer the call is asynchronous. */
+int  bt_render_frame(bt_context* ctx, const bt_grid* g, const bt_scene* s, const bt_view* v,
+                     const float* cam7, const bt_render_flags* flags, int step,
+                     uint8_t* out_bgra, void* d_out_bgra);
+/* Same, also copying the frame's intermediates to host buffers (any may be
+ * null): thphi (h+1)(w+1)*2, gap (h+1)(w+1), bh w*h, smoothed area w*h,
+ * summed star light w*h*3.  For parity tests. */
+int  bt_render_frame_stages(bt_context* ctx, const bt_grid* g, const bt_scene* s,
+                            const bt_view* v, const float* cam7, const bt_render_flags* flags,
+                            int step, uint8_t* out_bgra, void* d_out_bgra, float* thphi,
+                            int32_t* gap, uint8_t* bh, float* area, float* star);
+
 #ifdef __cplusplus
 }
 #endif

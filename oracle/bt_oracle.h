@@ -120,6 +120,41 @@ void   bto_grid_free(bto_grid* g);
 void   bto_hermite2(double v, const double* x0, const double* x1, const double* x2,
                     const double* x3, double* out);
 
+/* ---------------- Path B (bt_oracle_render.c) ---------------- */
+
+/* Viewer(viewangle, offUp, offRight, w, h, false) (Viewer.h:20-33,109-133):
+ * (h+1)*(w+1) float2 camera-sky directions of the pixel corners. */
+int    btor_camera_view(double viewangle, double off_up, double off_right, int w, int h,
+                        float* view);
+void   btor_sphere_view(int w, int h, float* view);
+/* StarProcessor::makeTree (StarProcessor.h:172-212); stars4 = S x
+ * (theta, phi, magnitude, B-V); tree (1 << (L+1)) - 1 ints; pos/mag 2*S. */
+int    btor_make_tree(const float* stars4, int S, int tree_level, int32_t* tree,
+                      float* pos, float* mag);
+/* Summed-area sky table (MakeImageKernel.cu:1312-1331): rows*cols*4 floats. */
+void   btor_sum_table(const uint8_t* sky_bgr, int rows, int cols, float* table);
+/* makeGrid from a perfect spatial hash (MakeImageKernel.cu:369-384). */
+void   btor_make_grid(const float* hash_table, const int32_t* offset_table,
+                      const int32_t* hash_pos_tag, int hw, int ow, int GN, int GM, float* grid);
+/* One G == 1 frame (MakeImageKernel.cu:1455-1529): out_bgra w*h*4.  The
+ * o_* intermediates are optional (thphi (h+1)(w+1)*2, gap (h+1)(w+1),
+ * bh w*h, area w*h (smoothed), star w*h*3 (summed star light)). */
+int    btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* view, int w,
+                  int h, const float* cam7, const int32_t* tree, int tree_level,
+                  const float* star_pos, const float* star_mag, int S, const float* sum_table,
+                  int sky_rows, int sky_cols, int star, int map, int lr, float hor, float ver,
+                  float offset, int step, uint8_t* out_bgra, float* o_thphi, int32_t* o_gap,
+                  uint8_t* o_bh, float* o_area, float* o_star);
+/* Helpers exposed for pinning tests. */
+void   btor_rgb_to_hsp(float r, float g, float b, float* hsp);
+void   btor_hsp_to_rgb(float h, float s, float p, float* rgb);
+float  btor_calc_area4(const float* t, const float* p);
+float  btor_calc_area3(const float* t, const float* p);
+float  btor_redshift(float theta, float phi, const float* cam7);
+int    btor_star_in_polygon(const float* t, const float* p, float st, float sp, int sgn);
+int    btor_search_tree(const int32_t* tree, const float* mn, const float* mx, int tree_level,
+                        int* out, int cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,2 +1,1292 @@
-/* bt_oracle_render.c -- CPU oracle, Path B (render).  Filled in below. */
+/*
+ * bt_oracle_render.c -- CPU oracle, Path B (render pass, FP32).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP render kernels.
+ *
+ * Restates, operation for operation in float, the reference's G == 1 frame
+ * pipeline of MakeImageKernel.cu (makeGrid, pixInterpolation,
+ * findBlackPixels, findArea, smoothAreaH/V, distortStarMap, sumStarLight,
+ * makePix, distortEnvironmentMap, addStarsAndBackground) and kernel.cuh's
+ * device helpers, plus the host-side inputs Viewer::makeCameraView,
+ * StarProcessor::makeTree and the summed-area sky table
+ * (MakeImageKernel.cu:1312-1331).
+ *
+ * Transcendentals.  The reference used CUDA 10's device libm (sinf, cosf,
+ * atanf, powf, expf, log10f, exp10f), which is not available here.  Every
+ * such call is evaluated in double and rounded to float, i.e. correctly
+ * rounded except in ~2^-28 of cases; the HIP kernels do the same, so the
+ * oracle and the device agree bit-for-bit.  Against the reference itself
+ * these values are "parity unpinned" (1/255 per channel is the only bound).
+ *
+ * Divergences from the reference, all where the reference has undefined
+ * behaviour (out-of-bounds reads or a data race), in favour of the evident
+ * intent:
+ *   - grid reads outside rows [0, GN) in findPoint read "absent" (-2,-2);
+ *   - the per-pixel star-leaf list (16 slots, MakeImageKernel.cu:1299) is
+ *     unbounded instead of overflowing into the neighbour's slots;
+ *   - sumStarLight is bounds-checked (no writes past the row when W % 24 != 0);
+ *   - summed-table rows -1 (single-column branch) read as zero and rows
+ *     beyond the last are clamped;
+ *   - the horizontal area-smoothing window wraps with a non-negative modulo;
+ *   - the blackbody index is clamped to the table's last entry (390); the
+ *     reference clamps to 1170 and reads past its 391-entry table.
+ * Pixel 0 of the sky pass stays black, as in the reference (ind > 0 test).
+ * Trails and diffraction are not part of this path (both are identities on
+ * a first frame with no diffraction sprite).
+ *
+ * Pinning: stage outputs are checked by tests against closed forms
+ * (Hermite identity on grid points, spherical-triangle areas summing to
+ * 4 pi, summed-table identity, HSP round trip, star-tree leaf membership by
+ * brute force, point-in-quad by brute force).
+ */
 #include "bt_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* kernel.cuh:23-25: float constants of the render code */
+#define FPI2 6.283185307179586476f
+#define FPI 3.141592653589793238f
+#define SQRT2PI 2.506628274631f
+/* Const.h doubles, used by the host-side Viewer */
+static const double DPI = 3.141592653589793238;
+static const double DPI1_2 = 3.141592653589793238 / 2.;
+static const double DPI2 = 3.141592653589793238 * 2.;
+
+static const int kTempRGB[1173] = {
+#include "blackbody_rgb.inc"
+};
+
+/* correctly rounded float transcendentals (see header) */
+static float f_sin(float x) { return (float)sin((double)x); }
+static float f_cos(float x) { return (float)cos((double)x); }
+static float f_atan(float x) { return (float)atan((double)x); }
+static float f_pow(float x, float y) { return (float)pow((double)x, (double)y); }
+static float f_exp(float x) { return (float)exp((double)x); }
+static float f_log10(float x) { return (float)log10((double)x); }
+static float f_exp10(float x) { return (float)pow(10.0, (double)x); }
+
+typedef struct { float x, y; } f2;
+
+/* ------------------------------------------------------------ Viewer */
+
+/* Viewer.h:20-33 + 109-133 */
+int btor_camera_view(double viewangle, double offsetVer, double offsetHor, int pixw, int pixh,
+                     float* view) {
+    double viewAngleWide = viewangle;
+    double viewAngleUp = 1. * pixh * viewAngleWide / (1. * pixw);
+    memset(view, 0, sizeof(float) * 2 * (size_t)(pixh + 1) * (pixw + 1));
+    if (fabs(offsetVer) > (DPI - viewAngleUp) / 2.) return -1;
+    float yleft = (float)tan(viewAngleWide / 2.f);
+    float step = (2.f * yleft) / pixw;
+    float zup = yleft * pixh / pixw;
+    for (int i = 0; i < pixh + 1; i++) {
+        float zval = zup - i * step;
+        float halfH = i * 1.f / (pixh / 2.f);
+        float theta = halfH == 1.f ? (float)(DPI1_2 - offsetVer)
+                                   : (float)(atan(1. / zval) + (halfH > 1.f) * DPI - offsetVer);
+        for (int j = 0; j < pixw + 1; j++) {
+            float yval = yleft - j * step;
+            float halfV = j * 1.f / (pixw / 2.f);
+            float phi = halfV == 1.f ? (float)(DPI - offsetHor)
+                                     : (float)(atan(1. / yval) + (halfV > 1.f) * DPI + DPI1_2 - offsetHor);
+            size_t o = (size_t)(pixw + 1) * i + pixw - j;
+            view[2 * o] = theta;
+            view[2 * o + 1] = phi;
+        }
+    }
+    return 0;
+}
+
+/* Viewer.h:82-88 */
+void btor_sphere_view(int pixw, int pixh, float* view) {
+    for (int i = 0; i < pixh + 1; i++)
+        for (int j = 0; j < pixw + 1; j++) {
+            size_t o = (size_t)i * (pixw + 1) + j;
+            view[2 * o] = (float)(1.f * i * DPI / pixh);
+            view[2 * o + 1] = (float)(1.f * j * DPI2 / pixw);
+        }
+}
+
+/* ------------------------------------------------------------ star tree */
+
+typedef struct {
+    int level_max;
+    int32_t* tree;
+    float* pos;
+    float* mag;
+} tree_ctx;
+
+/* StarProcessor.h:172-212; stars are index lists into s4 (theta, phi, mag, bv) */
+static void make_tree(tree_ctx* c, const float* s4, int* idx, int n, int level, float thphi[2],
+                      float size[2], int writePos) {
+    int searchPos = 0;
+    if (writePos != 0 && ((writePos + 1) & writePos) != 0) searchPos = c->tree[writePos - 1];
+    c->tree[writePos] = searchPos + n;
+    if (level == c->level_max) {
+        for (int i = 0; i < n; i++) {
+            const float* s = s4 + 4 * (size_t)idx[i];
+            c->pos[(i + searchPos) * 2] = s[0];
+            c->pos[(i + searchPos) * 2 + 1] = s[1];
+            c->mag[(i + searchPos) * 2] = s[2];
+            c->mag[(i + searchPos) * 2 + 1] = s[3];
+        }
+        return;
+    }
+    level++;
+    int nn = level % 2;
+    size[nn] = size[nn] * .5f;
+    int* lu = (int*)malloc(sizeof(int) * (n > 0 ? n : 1));
+    int* rd = (int*)malloc(sizeof(int) * (n > 0 ? n : 1));
+    int nl = 0, nr = 0;
+    float check = thphi[nn] + size[nn];
+    for (int i = 0; i < n; i++) {
+        if (s4[4 * (size_t)idx[i] + nn] < check)
+            lu[nl++] = idx[i];
+        else
+            rd[nr++] = idx[i];
+    }
+    float sizeNew[2] = {size[0], size[1]};
+    float thphiNew[2] = {thphi[0], thphi[1]};
+    make_tree(c, s4, lu, nl, level, thphiNew, sizeNew, writePos * 2 + 1);
+    float thphiNew2[2] = {thphi[0], thphi[1]};
+    float sizeNew2[2] = {size[0], size[1]};
+    thphiNew2[nn] += size[nn];
+    make_tree(c, s4, rd, nr, level, thphiNew2, sizeNew2, writePos * 2 + 2);
+    free(lu);
+    free(rd);
+}
+
+/* StarProcessor.h:190-198: tree of (1 << (L+1)) - 1 ints, pos/mag 2*S floats */
+int btor_make_tree(const float* stars4, int S, int tree_level, int32_t* tree, float* pos, float* mag) {
+    tree_ctx c = {tree_level, tree, pos, mag};
+    int* idx = (int*)malloc(sizeof(int) * (S > 0 ? S : 1));
+    for (int i = 0; i < S; i++) idx[i] = i;
+    float thphi[2] = {0, 0};
+    float size[2] = {(float)DPI, (float)DPI2};
+    make_tree(&c, stars4, idx, S, 0, thphi, size, 0);
+    free(idx);
+    return 0;
+}
+
+/* ------------------------------------------------------------ math helpers */
+
+/* kernel.cuh:586-610 */
+static float calc_area4(const float t[4], const float p[4]) {
+    float x[4], y[4], z[4];
+    for (int q = 0; q < 4; q++) {
+        float sint = f_sin(t[q]);
+        x[q] = sint * f_cos(p[q]);
+        y[q] = sint * f_sin(p[q]);
+        z[q] = f_cos(t[q]);
+    }
+    float dotpr1 = 1.f;
+    dotpr1 += x[0] * x[2] + y[0] * y[2] + z[0] * z[2];
+    float dotpr2 = dotpr1;
+    dotpr1 += x[2] * x[1] + y[2] * y[1] + z[2] * z[1];
+    dotpr1 += x[0] * x[1] + y[0] * y[1] + z[0] * z[1];
+    dotpr2 += x[0] * x[3] + y[0] * y[3] + z[0] * z[3];
+    dotpr2 += x[2] * x[3] + y[2] * y[3] + z[2] * z[3];
+    float triprod1 = fabsf(x[0] * (y[1] * z[2] - y[2] * z[1]) - y[0] * (x[1] * z[2] - x[2] * z[1]) +
+                           z[0] * (x[1] * y[2] - x[2] * y[1]));
+    float triprod2 = fabsf(x[0] * (y[2] * z[3] - y[3] * z[2]) - y[0] * (x[2] * z[3] - x[3] * z[2]) +
+                           z[0] * (x[2] * y[3] - x[3] * y[2]));
+    return 2.f * (f_atan(triprod1 / dotpr1) + f_atan(triprod2 / dotpr2));
+}
+
+/* kernel.cuh:612-649 */
+static float calc_area3(const float t[3], const float p[3]) {
+    float xi[3], yi[3], zi[3];
+    for (int q = 0; q < 3; q++) {
+        float sint = f_sin(t[q]);
+        xi[q] = sint * f_cos(p[q]);
+        yi[q] = sint * f_sin(p[q]);
+        zi[q] = f_cos(t[q]);
+    }
+    float dot01 = xi[0] * xi[1] + yi[0] * yi[1] + zi[0] * zi[1];
+    float dot02 = xi[0] * xi[2] + yi[0] * yi[2] + zi[0] * zi[2];
+    float dot12 = xi[2] * xi[1] + yi[2] * yi[1] + zi[2] * zi[1];
+    float x[3] = {xi[0], xi[1], xi[2]};
+    float y[3] = {yi[0], yi[1], yi[2]};
+    float z[3] = {zi[0], zi[1], zi[2]};
+    if (dot01 < dot02 && dot01 < dot12) {
+        x[0] = xi[2]; x[1] = xi[0]; x[2] = xi[1];
+        y[0] = yi[2]; y[1] = yi[0]; y[2] = yi[1];
+        z[0] = zi[2]; z[1] = zi[0]; z[2] = zi[1];
+    } else if (dot02 < dot12 && dot02 < dot01) {
+        x[0] = xi[1]; x[1] = xi[2]; x[2] = xi[0];
+        y[0] = yi[1]; y[1] = yi[2]; y[2] = yi[0];
+        z[0] = zi[1]; z[1] = zi[2]; z[2] = zi[0];
+    }
+    float dotpr1 = 1.f;
+    dotpr1 += x[0] * x[2] + y[0] * y[2] + z[0] * z[2];
+    dotpr1 += x[2] * x[1] + y[2] * y[1] + z[2] * z[1];
+    dotpr1 += x[0] * x[1] + y[0] * y[1] + z[0] * z[1];
+    float triprod1 = fabsf(x[0] * (y[1] * z[2] - y[2] * z[1]) - y[0] * (x[1] * z[2] - x[2] * z[1]) +
+                           z[0] * (x[1] * y[2] - x[2] * y[1]));
+    return 2.f * (f_atan(triprod1 / dotpr1));
+}
+
+/* kernel.cuh:274-295 */
+static int pi_check(float* p, float factor) {
+    float factor1 = FPI2 * (1.f - factor);
+    int check = 0;
+    for (int q = 0; q < 4; q++)
+        if (p[q] > factor1) {
+            check = 1;
+            break;
+        }
+    if (!check) return 0;
+    check = 0;
+    float factor2 = FPI2 * factor;
+    for (int q = 0; q < 4; q++)
+        if (p[q] < factor2) {
+            p[q] += FPI2;
+            check = 1;
+        }
+    return check;
+}
+
+/* kernel.cuh:303-322 */
+static int pi_check_tot(f2* tp, float factor, int size) {
+    float factor1 = FPI2 * (1.f - factor);
+    int check = 0;
+    for (int q = 0; q < size; q++)
+        if (tp[q].y > factor1) {
+            check = 1;
+            break;
+        }
+    if (!check) return 0;
+    check = 0;
+    float factor2 = FPI2 * factor;
+    for (int q = 0; q < size; q++)
+        if (tp[q].y < factor2) {
+            tp[q].y += FPI2;
+            check = 1;
+        }
+    return check;
+}
+
+/* kernel.cuh:694-703 */
+static void wrap_to_pi_f(float* thetaW, float* phiW) {
+    *thetaW = fmodf(*thetaW, FPI2);
+    while (*thetaW < 0.f) *thetaW += FPI2;
+    if (*thetaW > FPI) {
+        *thetaW -= 2.f * (*thetaW - FPI);
+        *phiW += FPI;
+    }
+    while (*phiW < 0.f) *phiW += FPI2;
+    *phiW = fmodf(*phiW, FPI2);
+}
+
+/* kernel.cuh:790-815 with tension = bias = 0 */
+static f2 hermite_f(float v, f2 x0, f2 x1, f2 x2, f2 x3) {
+    float const v2 = v * v;
+    float const v3 = v * v2;
+    float const aa = (1.f + 0.f) * (1.f - 0.f) / 2.f;
+    float const bb = (1.f - 0.f) * (1.f - 0.f) / 2.f;
+    float const m0T = aa * (x1.x - x0.x) + bb * (x2.x - x1.x);
+    float const m0P = aa * (x1.y - x0.y) + bb * (x2.y - x1.y);
+    float const m1T = aa * (x2.x - x1.x) + bb * (x3.x - x2.x);
+    float const m1P = aa * (x2.y - x1.y) + bb * (x3.y - x2.y);
+    float const u0 = 2.f * v3 - 3.f * v2 + 1.f;
+    float const u1 = v3 - 2.f * v2 + v;
+    float const u2 = v3 - v2;
+    float const u3 = -2.f * v3 + 3.f * v2;
+    f2 r = {u0 * x1.x + u1 * m0T + u2 * m1T + u3 * x2.x, u0 * x1.y + u1 * m0P + u2 * m1P + u3 * x2.y};
+    return r;
+}
+
+/* kernel.cuh:754-772 (double intersection of two lines) */
+static f2 intersection(float ax, float ay, float bx, float by, float cx, float cy, float dx, float dy) {
+    double a1 = by - ay;
+    double b1 = ax - bx;
+    double c1 = a1 * (ax) + b1 * (ay);
+    double a2 = dy - cy;
+    double b2 = cx - dx;
+    double c2 = a2 * (cx) + b2 * (cy);
+    double determinant = a1 * b2 - a2 * b1;
+    f2 r = {-1, -1};
+    if (determinant == 0) return r;
+    double x = (b2 * c1 - b1 * c2) / determinant;
+    double y = (a1 * c2 - a2 * c1) / determinant;
+    r.x = (float)x;
+    r.y = (float)y;
+    return r;
+}
+
+/* ------------------------------------------------------------ grid interpolation */
+
+typedef struct {
+    const float* grid; /* GN x GM float2 */
+    int GN, GM;
+} gridv;
+
+static f2 grid_at(const gridv* g, int i, int j) {
+    f2 r = {-2.f, -2.f};
+    if (i < 0 || i >= g->GN || j < 0 || j >= g->GM) return r;
+    size_t o = (size_t)i * g->GM + j;
+    r.x = g->grid[2 * o];
+    r.y = g->grid[2 * o + 1];
+    return r;
+}
+
+/* kernel.cuh:774-788 */
+static f2 interpolate_linear(float percDown, float percRight, const f2* c) {
+    float phi[4] = {c[0].y, c[1].y, c[2].y, c[3].y};
+    float theta[4] = {c[0].x, c[1].x, c[2].x, c[3].x};
+    pi_check(phi, 0.2f);
+    float leftT = theta[0] + percDown * (theta[2] - theta[0]);
+    float leftP = phi[0] + percDown * (phi[2] - phi[0]);
+    float rightT = theta[1] + percDown * (theta[3] - theta[1]);
+    float rightP = phi[1] + percDown * (phi[3] - phi[1]);
+    float upT = theta[0] + percRight * (theta[1] - theta[0]);
+    float upP = phi[0] + percRight * (phi[1] - phi[0]);
+    float downT = theta[2] + percRight * (theta[3] - theta[2]);
+    float downP = phi[2] + percRight * (phi[3] - phi[2]);
+    return intersection(upT, upP, downT, downP, leftT, leftP, rightT, rightP);
+}
+
+static f2 interpolate_hermite(const gridv* g, int i, int j, int gap, float percDown, float percRight,
+                              f2* cornersCel, int count);
+
+/* kernel.cuh:817-889 */
+static f2 find_point(const gridv* g, int i, int j, int offver, int offhor, int gap, int count) {
+    const int GM = g->GM, GN = g->GN;
+    f2 gridpt = grid_at(g, i, j);
+    f2 m1 = {-1, -1};
+    if (gridpt.x == -2 && gridpt.y == -2) {
+        int j2 = (j + offhor * gap + GM) % GM;
+        int i2 = i + offver * gap;
+        f2 ij2 = grid_at(g, i2, j2);
+        if (ij2.x == -1 && ij2.y == -1) return m1;
+        else if (ij2.x != -2 && ij2.y != -2) {
+            int j0 = (j - offhor * gap + GM) % GM;
+            int i0 = (i - offver * gap);
+            f2 ij0 = grid_at(g, i0, j0);
+            if (ij0.x < 0) return m1;
+            int jprev = (j - 3 * offhor * gap + GM) % GM;
+            int jnext = (j + 3 * offhor * gap + GM) % GM;
+            int iprev = i - offver * 3 * gap;
+            int inext = i + offver * 3 * gap;
+            if (offver != 0) {
+                if (i2 == 0) {
+                    jnext = (j0 + GM / 2) % GM;
+                    inext = i0;
+                } else if (i0 == GN - 1) {
+                    jprev = (j0 + GM / 2) % GM;
+                    iprev = i2;
+                } else if (i2 == GN - 1) {
+                    inext = i0;
+                    jnext = (j0 + GM / 2) % GM;
+                }
+            }
+            f2 ijprev = grid_at(g, iprev, jprev);
+            f2 ijnext = grid_at(g, inext, jnext);
+            if (ijprev.x > -2 && ijnext.x > -2) {
+                f2 pt[4] = {ijprev, ij0, ij2, ijnext};
+                if (pt[0].x != -1 && pt[3].x != -1) {
+                    pi_check_tot(pt, 0.2f, 4);
+                    return hermite_f(0.5f, pt[0], pt[1], pt[2], pt[3]);
+                }
+            }
+            f2 pt[2] = {ij2, ij0};
+            pi_check_tot(pt, 0.2f, 2);
+            f2 r = {.5f * (pt[0].x + pt[1].x), .5f * (pt[0].y + pt[1].y)};
+            return r;
+        } else {
+            if (count > 0) return m1;
+            int j0 = (j + gap) % GM;
+            int j1 = (j - gap + GM) % GM;
+            if (i - gap < 0) return m1;
+            f2 cc[12];
+            cc[0] = grid_at(g, i + gap, j0);
+            cc[1] = grid_at(g, i - gap, j0);
+            cc[2] = grid_at(g, i - gap, j1);
+            cc[3] = grid_at(g, i + gap, j1);
+            for (int q = 0; q < 4; q++)
+                if (cc[q].x == -1 || cc[q].x == -2) return m1;
+            return interpolate_hermite(g, i - gap, j1, 2 * gap, .5f, .5f, cc, 1);
+        }
+    }
+    return gridpt;
+}
+
+/* kernel.cuh:891-938 */
+static f2 interpolate_hermite(const gridv* g, int i, int j, int gap, float percDown, float percRight,
+                              f2* c, int count) {
+    const int GM = g->GM, GN = g->GN;
+    int k = i + gap;
+    int l = (j + gap) % GM;
+    int imin1 = i - gap;
+    int kplus1 = k + gap;
+    int jmin1 = (j - gap + GM) % GM;
+    int lplus1 = (l + gap) % GM;
+    int jx = j, jy = j, lx = l, ly = l;
+    if (i == 0) {
+        jx = (j + GM / 2) % GM;
+        lx = (jx + gap) % GM;
+        imin1 = k;
+    } else if (k == GN - 1) {
+        jy = (j + GM / 2) % GM;
+        ly = (jy + gap) % GM;
+        kplus1 = i;
+    }
+    c[4] = find_point(g, i, jmin1, 0, -1, gap, count);
+    c[5] = find_point(g, i, lplus1, 0, 1, gap, count);
+    c[6] = find_point(g, k, jmin1, 0, -1, gap, count);
+    c[7] = find_point(g, k, lplus1, 0, 1, gap, count);
+    c[8] = find_point(g, imin1, jx, -1, 0, gap, count);
+    c[9] = find_point(g, imin1, lx, -1, 0, gap, count);
+    c[10] = find_point(g, kplus1, jy, 1, 0, gap, count);
+    c[11] = find_point(g, kplus1, ly, 1, 0, gap, count);
+    for (int q = 4; q < 12; q++)
+        if (c[q].x == -1) return interpolate_linear(percDown, percRight, c);
+    pi_check_tot(c, 0.2f, 12);
+    f2 up = hermite_f(percRight, c[4], c[0], c[1], c[5]);
+    f2 down = hermite_f(percRight, c[6], c[2], c[3], c[7]);
+    f2 upup = {c[8].x + (c[9].x - c[8].x) * percRight, c[8].y + (c[9].y - c[8].y) * percRight};
+    f2 downdown = {c[10].x + (c[11].x - c[10].x) * percRight, c[10].y + (c[11].y - c[10].y) * percRight};
+    return hermite_f(percDown, upup, up, down, downdown);
+}
+
+/* kernel.cuh:940-968 */
+static f2 interpolate_spline(const gridv* g, int i, int j, int gap, float thetaCam, float phiCam,
+                             f2* c, const float* cornersCam) {
+    float thetaUp = cornersCam[0];
+    float thetaDown = cornersCam[2];
+    float phiLeft = cornersCam[1];
+    float phiRight = cornersCam[3];
+    if (thetaUp == thetaCam) {
+        if (phiLeft == phiCam) return c[0];
+        if (phiRight == phiCam) return c[1];
+        if (i == 0.f) return c[0];
+    } else if (thetaDown == thetaCam) {
+        if (phiLeft == phiCam) return c[2];
+        if (phiRight == phiCam) return c[3];
+    }
+    for (int q = 0; q < 4; q++)
+        if (c[q].x == -1 && c[q].y == -1) {
+            f2 m1 = {-1.f, -1.f};
+            return m1;
+        }
+    float percDown = (thetaCam - thetaUp) / (thetaDown - thetaUp);
+    float percRight = (phiCam - phiLeft) / (phiRight - phiLeft);
+    return interpolate_hermite(g, i, j, gap, percDown, percRight, c, 0);
+}
+
+/* kernel.cuh:736-752 */
+static void find_block(const gridv* g, float theta, float phi, int* i, int* j, int* gap, int level) {
+    const int GM = g->GM;
+    for (int s = 0; s < level + 1; s++) {
+        int ngap = *gap / 2;
+        int k = *i + ngap;
+        int l = *j + ngap;
+        if (*gap <= 1 || grid_at(g, k, l).x == -2.f) return;
+        float thHalf = FPI2 * k / (1.f * GM);
+        float phHalf = FPI2 * l / (1.f * GM);
+        if (thHalf <= theta) *i = k;
+        if (phHalf <= phi) *j = l;
+        *gap = ngap;
+    }
+}
+
+/* MakeImageKernel.cu:328-351 */
+static f2 interpolate_pix(const gridv* g, float theta, float phi, int gridlvl, int* gapsave) {
+    const int GM = g->GM;
+    int half = (phi < FPI) ? 0 : 1;
+    int a = 0;
+    int b = half * GM / 2;
+    int gap = GM / 2;
+    find_block(g, theta, phi, &a, &b, &gap, gridlvl);
+    *gapsave = gap;
+    int k = a + gap;
+    int l = b + gap;
+    float factor = FPI2 / (1.f * GM);
+    float cornersCam[4] = {factor * a, factor * b, factor * k, factor * l};
+    l = l % GM;
+    f2 nul = {-1, -1};
+    f2 c[12] = {grid_at(g, a, b), grid_at(g, a, l), grid_at(g, k, b), grid_at(g, k, l),
+                nul, nul, nul, nul, nul, nul, nul, nul};
+    f2 r = interpolate_spline(g, a, b, gap, theta, phi, c, cornersCam);
+    if (!(r.x == -1 && r.y == -1)) wrap_to_pi_f(&r.x, &r.y);
+    return r;
+}
+
+/* ------------------------------------------------------------ colour helpers */
+
+#define Pr .299f
+#define Pg .587f
+#define Pb .114f
+
+/* kernel.cuh:388-427 */
+static void rgb_to_hsp(float R, float G, float B, float* H, float* S, float* P) {
+    *P = sqrtf(R * R * Pr + G * G * Pg + B * B * Pb);
+    if (R == G && R == B) {
+        *H = 0.f;
+        *S = 0.f;
+        return;
+    }
+    if (R >= G && R >= B) {
+        if (B >= G) {
+            *H = 6.f / 6.f - 1.f / 6.f * (B - G) / (R - G);
+            *S = 1.f - G / R;
+        } else {
+            *H = 0.f / 6.f + 1.f / 6.f * (G - B) / (R - B);
+            *S = 1.f - B / R;
+        }
+    } else if (G >= R && G >= B) {
+        if (R >= B) {
+            *H = 2.f / 6.f - 1.f / 6.f * (R - B) / (G - B);
+            *S = 1.f - B / G;
+        } else {
+            *H = 2.f / 6.f + 1.f / 6.f * (B - R) / (G - R);
+            *S = 1.f - R / G;
+        }
+    } else {
+        if (G >= R) {
+            *H = 4.f / 6.f - 1.f / 6.f * (G - R) / (B - R);
+            *S = 1.f - R / B;
+        } else {
+            *H = 4.f / 6.f + 1.f / 6.f * (R - G) / (B - G);
+            *S = 1.f - G / B;
+        }
+    }
+}
+
+/* kernel.cuh:440-524 */
+static void hsp_to_rgb(float H, float S, float P, float* R, float* G, float* B) {
+    float part, minOverMax = 1.f - S;
+    if (minOverMax > 0.f) {
+        if (H < 1.f / 6.f) {
+            H = 6.f * H;
+            part = 1.f + H * (1.f / minOverMax - 1.f);
+            *B = P / sqrtf(Pr / minOverMax / minOverMax + Pg * part * part + Pb);
+            *R = *B / minOverMax;
+            *G = *B + H * (*R - *B);
+        } else if (H < 2.f / 6.f) {
+            H = 6.f * (-H + 2.f / 6.f);
+            part = 1.f + H * (1.f / minOverMax - 1.f);
+            *B = P / sqrtf(Pg / minOverMax / minOverMax + Pr * part * part + Pb);
+            *G = *B / minOverMax;
+            *R = *B + H * (*G - *B);
+        } else if (H < 3.f / 6.f) {
+            H = 6.f * (H - 2.f / 6.f);
+            part = 1.f + H * (1.f / minOverMax - 1.f);
+            *R = P / sqrtf(Pg / minOverMax / minOverMax + Pb * part * part + Pr);
+            *G = *R / minOverMax;
+            *B = *R + H * (*G - *R);
+        } else if (H < 4.f / 6.f) {
+            H = 6.f * (-H + 4.f / 6.f);
+            part = 1.f + H * (1.f / minOverMax - 1.f);
+            *R = P / sqrtf(Pb / minOverMax / minOverMax + Pg * part * part + Pr);
+            *B = *R / minOverMax;
+            *G = *R + H * (*B - *R);
+        } else if (H < 5.f / 6.f) {
+            H = 6.f * (H - 4.f / 6.f);
+            part = 1.f + H * (1.f / minOverMax - 1.f);
+            *G = P / sqrtf(Pb / minOverMax / minOverMax + Pr * part * part + Pg);
+            *B = *G / minOverMax;
+            *R = *G + H * (*B - *G);
+        } else {
+            H = 6.f * (-H + 6.f / 6.f);
+            part = 1.f + H * (1.f / minOverMax - 1.f);
+            *G = P / sqrtf(Pr / minOverMax / minOverMax + Pb * part * part + Pg);
+            *R = *G / minOverMax;
+            *B = *G + H * (*R - *G);
+        }
+    } else {
+        if (H < 1.f / 6.f) {
+            H = 6.f * (H);
+            *R = sqrtf(P * P / (Pr + Pg * H * H));
+            *G = *R * H;
+            *B = 0.f;
+        } else if (H < 2.f / 6.f) {
+            H = 6.f * (-H + 2.f / 6.f);
+            *G = sqrtf(P * P / (Pg + Pr * H * H));
+            *R = *G * H;
+            *B = 0.f;
+        } else if (H < .5f) {
+            H = 6.f * (H - 2.f / 6.f);
+            *G = sqrtf(P * P / (Pg + Pb * H * H));
+            *B = *G * H;
+            *R = 0.f;
+        } else if (H < 4.f / 6.f) {
+            H = 6.f * (-H + 4.f / 6.f);
+            *B = sqrtf(P * P / (Pb + Pg * H * H));
+            *G = *B * H;
+            *R = 0.f;
+        } else if (H < 5.f / 6.f) {
+            H = 6.f * (H - 4.f / 6.f);
+            *B = sqrtf(P * P / (Pb + Pr * H * H));
+            *R = *B * H;
+            *G = 0.f;
+        } else {
+            H = 6.f * (-H + 1.f);
+            *R = sqrtf(P * P / (Pr + Pb * H * H));
+            *B = *R * H;
+            *G = 0.f;
+        }
+    }
+}
+
+void btor_rgb_to_hsp(float r, float g, float b, float* hsp) { rgb_to_hsp(r, g, b, &hsp[0], &hsp[1], &hsp[2]); }
+void btor_hsp_to_rgb(float h, float s, float p, float* rgb) { hsp_to_rgb(h, s, p, &rgb[0], &rgb[1], &rgb[2]); }
+float btor_calc_area4(const float* t, const float* p) { return calc_area4(t, p); }
+float btor_calc_area3(const float* t, const float* p) { return calc_area3(t, p); }
+
+/* kernel.cuh:348-365; cam = getParamArray() {speed, alpha, w, wbar, br, btheta, bphi} */
+static float redshift(float theta, float phi, const float* cam) {
+    float xCam = f_sin(theta) * f_cos(phi);
+    float zCam = f_cos(theta);
+    float yCam = f_sin(theta) * f_sin(phi);
+    float part = (1.f - cam[0] * yCam);
+    float betaPart = sqrtf(1.f - cam[0] * cam[0]) / part;
+    float xFido = -sqrtf(1.f - cam[0] * cam[0]) * xCam / part;
+    float zFido = -sqrtf(1.f - cam[0] * cam[0]) * zCam / part;
+    float yFido = (-yCam + cam[0]) / part;
+    float k = sqrtf(1.f - cam[5] * cam[5]);
+    float phiFido = -xFido * cam[4] / k + cam[6] * yFido + cam[6] * cam[5] / k * zFido;
+    float eF = 1.f / (cam[1] + cam[2] * cam[3] * phiFido);
+    float b = eF * cam[3] * phiFido;
+    return 1.f / (betaPart * (1.f - b * cam[2]) / cam[1]);
+}
+float btor_redshift(float theta, float phi, const float* cam) { return redshift(theta, phi, cam); }
+
+/* ------------------------------------------------------------ frame stages */
+
+typedef struct {
+    int W, H;
+    const float* view;
+    float* thphi;
+} frame_ctx;
+
+/* kernel.cuh:652-672; returns 0 (ind = -1) when a corner is negative */
+static int retrieve_corners(const float* thphi, int W, int ind, float* t, float* p, int* picheck,
+                            float offset) {
+    const int M1 = W + 1;
+    t[0] = thphi[2 * (ind + M1)];
+    t[1] = thphi[2 * ind];
+    t[2] = thphi[2 * (ind + 1)];
+    t[3] = thphi[2 * (ind + M1 + 1)];
+    p[0] = thphi[2 * (ind + M1) + 1];
+    p[1] = thphi[2 * ind + 1];
+    p[2] = thphi[2 * (ind + 1) + 1];
+    p[3] = thphi[2 * (ind + M1 + 1) + 1];
+    for (int q = 0; q < 4; q++) {
+        if (p[q] < 0 || t[q] < 0) return 0;
+        p[q] = fmodf(p[q] + offset, FPI2);
+    }
+    *picheck = pi_check(p, .2f);
+    return 1;
+}
+
+/* kernel.cuh:674-692 */
+static void lensing_redshift(const float* t, const float* p, int W, int ind, const float* cam,
+                             const float* view, float* frac, float* redshft, float solidAngle) {
+    const int M1 = W + 1;
+    if (solidAngle == 0.f) {
+        float th1[3] = {t[0], t[1], t[2]}, ph1[3] = {p[0], p[1], p[2]};
+        float th2[3] = {t[0], t[2], t[3]}, ph2[3] = {p[0], p[2], p[3]};
+        solidAngle = calc_area3(th1, ph1) + calc_area3(th2, ph2);
+    }
+    float ver4[4] = {view[2 * ind], view[2 * (ind + 1)], view[2 * (ind + M1 + 1)], view[2 * (ind + M1)]};
+    float hor4[4] = {view[2 * ind + 1], view[2 * (ind + 1) + 1], view[2 * (ind + M1 + 1) + 1],
+                     view[2 * (ind + M1) + 1]};
+    float pixArea = calc_area4(ver4, hor4);
+    *frac = pixArea / solidAngle;
+    float thetaCam = (ver4[0] + ver4[1] + ver4[2] + ver4[3]) * .25f;
+    float phiCam = (hor4[0] + hor4[1] + hor4[2] + hor4[3]) * .25f;
+    *redshft = redshift(thetaCam, phiCam, cam);
+}
+
+/* kernel.cuh:245-266 */
+static int cross_ok(float t_a, float t_b, float p_a, float p_b, float st, float sp, int sgn) {
+    float c1t = (float)sgn * (t_a - t_b);
+    float c1p = (float)sgn * (p_a - p_b);
+    float c2t = sgn ? st - t_b : st - t_a;
+    float c2p = sgn ? sp - p_b : sp - p_a;
+    return (c1t * c2p - c2t * c1p) > 0;
+}
+static int star_in_polygon(const float* t, const float* p, float st, float sp, int sgn) {
+    return cross_ok(t[0], t[1], p[0], p[1], st, sp, sgn) && cross_ok(t[1], t[2], p[1], p[2], st, sp, sgn) &&
+           cross_ok(t[2], t[3], p[2], p[3], st, sp, sgn) && cross_ok(t[3], t[0], p[3], p[0], st, sp, sgn);
+}
+int btor_star_in_polygon(const float* t, const float* p, float st, float sp, int sgn) {
+    return star_in_polygon(t, p, st, sp, sgn);
+}
+
+/* kernel.cuh:112-200: sub-pixel position by 10-step bisection */
+static void interpolate_star(float t0, float t1, float t2, float t3, float p0, float p1, float p2, float p3,
+                             float* start, float* starp, int sgn) {
+    float error = 0.00001f;
+    float midT = (t0 + t1 + t2 + t3) * .25f;
+    float midP = (p0 + p1 + p2 + p3) * .25f;
+    float starInPixY = 0.5f;
+    float starInPixX = 0.5f;
+    float perc = 0.5f;
+    for (int q = 0; q < 10; q++) {
+        if ((fabsf(*start - midT) < error) && (fabsf(*starp - midP) < error)) break;
+        float half01T = (t0 + t1) * .5f;
+        float half23T = (t2 + t3) * .5f;
+        float half12T = (t2 + t1) * .5f;
+        float half03T = (t0 + t3) * .5f;
+        float half01P = (p0 + p1) * .5f;
+        float half23P = (p2 + p3) * .5f;
+        float half12P = (p2 + p1) * .5f;
+        float half03P = (p0 + p3) * .5f;
+        float line01to23T = half23T - half01T;
+        float line03to12T = half12T - half03T;
+        float line01to23P = half23P - half01P;
+        float line03to12P = half12P - half03P;
+        float line01toStarT = *start - half01T;
+        float line03toStarT = *start - half03T;
+        float line01toStarP = *starp - half01P;
+        float line03toStarP = *starp - half03P;
+        int a = (((line03to12T * line03toStarP) - (line03toStarT * line03to12P)) > 0.f) ? 1 : -1;
+        int b = (((line01to23T * line01toStarP) - (line01toStarT * line01to23P)) > 0.f) ? 1 : -1;
+        perc *= 0.5f;
+        if (sgn * a > 0) {
+            if (sgn * b > 0) {
+                t2 = half12T; t0 = half01T; t3 = midT;
+                p2 = half12P; p0 = half01P; p3 = midP;
+                starInPixX -= perc; starInPixY -= perc;
+            } else {
+                t2 = midT; t1 = half01T; t3 = half03T;
+                p2 = midP; p1 = half01P; p3 = half03P;
+                starInPixX -= perc; starInPixY += perc;
+            }
+        } else {
+            if (sgn * b > 0) {
+                t1 = half12T; t3 = half23T; t0 = midT;
+                p1 = half12P; p3 = half23P; p0 = midP;
+                starInPixX += perc; starInPixY -= perc;
+            } else {
+                t0 = half03T; t1 = midT; t2 = half23T;
+                p0 = half03P; p1 = midP; p2 = half23P;
+                starInPixX += perc; starInPixY += perc;
+            }
+        }
+        midT = (t0 + t1 + t2 + t3) * .25f;
+        midP = (p0 + p1 + p2 + p3) * .25f;
+    }
+    *start = starInPixY;
+    *starp = starInPixX;
+}
+
+/* kernel.cuh:335-338 */
+static float gaussian(float dist) {
+    float sigma = 1.f / 2.5f;
+    return f_exp(-.5f * dist / (sigma * sigma)) * 1.f / (sigma * SQRT2PI);
+}
+
+/* MakeImageKernel.cu:27-96, searching the implicit star tree for leaves
+ * (or nearly covered inner nodes) that overlap the pixel's bounding box.
+ * Nodes are appended to out (unbounded); returns their count. */
+static int search_tree(const int32_t* tree, const float* mn, const float* mx, int treeLevel, int* out,
+                       int cap, int picheck) {
+    float nodeStart[2] = {0.f, 0.f + picheck * FPI};
+    float nodeSize[2] = {FPI, FPI2};
+    int node = 0;
+    unsigned bitMask = (unsigned)powf(2, treeLevel);
+    int level = 0;
+    int lvl = 0;
+    int pos = 0;
+    while (bitMask != 0) {
+        bitMask &= ~(1UL << (treeLevel - level));
+        for (lvl = level + 1; lvl <= treeLevel; lvl++) {
+            int star_n = tree[node];
+            if (node != 0 && ((node + 1) & node) != 0) star_n -= tree[node - 1];
+            int tp = lvl & 1;
+            float x_overlap = fmaxf(0.f, fminf(mx[0], nodeStart[0] + nodeSize[0]) - fmaxf(mn[0], nodeStart[0]));
+            float y_overlap = fmaxf(0.f, fminf(mx[1], nodeStart[1] + nodeSize[1]) - fmaxf(mn[1], nodeStart[1]));
+            float overlapArea = x_overlap * y_overlap;
+            int size = overlapArea / (nodeSize[0] * nodeSize[1]) > 0.8f;
+            nodeSize[tp] = nodeSize[tp] * .5f;
+            if (star_n == 0) {
+                node = node * 2 + 1;
+                break;
+            }
+            float check = nodeStart[tp] + nodeSize[tp];
+            int lu = mn[tp] < check;
+            int rd = mx[tp] >= check;
+            if (lvl == 1 && picheck) {
+                int tmp = lu;
+                lu = rd;
+                rd = tmp;
+            }
+            if (lvl == treeLevel || (rd && lu && size)) {
+                if (rd) {
+                    if (pos < cap) out[pos] = node * 2 + 2;
+                    pos++;
+                }
+                if (lu) {
+                    if (pos < cap) out[pos] = node * 2 + 1;
+                    pos++;
+                }
+                node = node * 2 + 1;
+                break;
+            } else {
+                node = node * 2 + 1;
+                if (rd) bitMask |= 1UL << (treeLevel - lvl);
+                if (!lu) break;
+                else if (lvl == 1 && picheck) nodeStart[1] += nodeSize[1];
+            }
+        }
+        level = treeLevel - __builtin_ffs((int)bitMask) + 1;
+        if (level >= 0) {
+            int diff = lvl - level;
+            for (int i = 0; i < diff; i++) {
+                int tp = (lvl - i) & 1;
+                if (!(node & 1)) nodeStart[tp] -= nodeSize[tp];
+                nodeSize[tp] = nodeSize[tp] * 2.f;
+                node = (node - 1) / 2;
+            }
+            node++;
+            int tp = level & 1;
+            if (picheck && level == 1) nodeStart[tp] -= nodeSize[tp];
+            else nodeStart[tp] += nodeSize[tp];
+        }
+    }
+    return pos;
+}
+int btor_search_tree(const int32_t* tree, const float* mn, const float* mx, int treeLevel, int* out, int cap) {
+    return search_tree(tree, mn, mx, treeLevel, out, cap, 0);
+}
+
+/* ------------------------------------------------------------ summed table */
+
+/* MakeImageKernel.cu:1312-1331: table[(p * cols + q) * 4 + c] (BGR + area) */
+void btor_sum_table(const uint8_t* sky_bgr, int rows, int cols, float* table) {
+    float pxsz = FPI2 / (1.f * cols);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int q = 0; q < cols; q++) {
+        float prev[4] = {0.f, 0.f, 0.f, 0.f};
+        float phi[4] = {pxsz * q, pxsz * q, pxsz * (q + 1), pxsz * (q + 1)};
+        for (int p = 0; p < rows; p++) {
+            float theta[4] = {pxsz * (p + 1), pxsz * p, pxsz * p, pxsz * (p + 1)};
+            float area = calc_area4(theta, phi);
+            const uint8_t* px = sky_bgr + 3 * ((size_t)p * cols + q);
+            prev[0] += f_pow((float)px[0], 2.2f) * area;
+            prev[1] += f_pow((float)px[1], 2.2f) * area;
+            prev[2] += f_pow((float)px[2], 2.2f) * area;
+            prev[3] += area;
+            float* o = table + 4 * ((size_t)p * cols + q);
+            o[0] = prev[0];
+            o[1] = prev[1];
+            o[2] = prev[2];
+            o[3] = prev[3];
+        }
+    }
+}
+
+/* ------------------------------------------------------------ dense grid from the PSH */
+
+/* MakeImageKernel.cu:369-384 with hashLookup (kernel.cuh:705-733), G == 1 */
+void btor_make_grid(const float* hash_table, const int32_t* offset_table, const int32_t* hash_pos_tag,
+                    int hw, int ow, int GN, int GM, float* grid /* GN1 * GM * 2 */) {
+    int sym = (float)GM / (float)GN > 3 ? 1 : 0;
+    int GN1 = sym ? 2 * GN - 1 : GN;
+    for (int i = 0; i < GN; i++)
+        for (int j = 0; j < GM; j++) {
+            int ix = (i + ow) % ow, iy = (j + ow) % ow;
+            int ax = (i + hw) % hw + offset_table[2 * (ix * ow + iy)];
+            int ay = (j + hw) % hw + offset_table[2 * (ix * ow + iy) + 1];
+            int hx = (ax + hw) % hw, hy = (ay + hw) % hw;
+            int s = hx * hw + hy;
+            float vx = -2.f, vy = -2.f;
+            if (hash_pos_tag[2 * s] == i && hash_pos_tag[2 * s + 1] == j) {
+                vx = hash_table[2 * s];
+                vy = hash_table[2 * s + 1];
+            }
+            grid[2 * ((size_t)i * GM + j)] = vx;
+            grid[2 * ((size_t)i * GM + j) + 1] = vy;
+            if (sym == 1) {
+                if (vx != -1 && vx != -2) vx = FPI - vx;
+                int x = GN1 - i - 1;
+                grid[2 * ((size_t)x * GM + j)] = vx;
+                grid[2 * ((size_t)x * GM + j) + 1] = vy;
+            }
+        }
+}
+
+/* ------------------------------------------------------------ the frame */
+
+int btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* view, int W, int H,
+               const float* cam, const int32_t* tree, int tree_level, const float* star_pos,
+               const float* star_mag, int S, const float* sum_table, int sky_rows, int sky_cols,
+               int star, int map, int lr, float hor, float ver, float offset, int step,
+               uint8_t* out_bgra, float* o_thphi, int32_t* o_gap, uint8_t* o_bh, float* o_area,
+               float* o_star) {
+    const int M = W, N = H, M1 = W + 1, N1 = H + 1;
+    const size_t NP = (size_t)N * M, NC = (size_t)N1 * M1;
+    gridv g = {grid, GN, GM};
+    float* thphi = o_thphi ? o_thphi : (float*)malloc(sizeof(float) * 2 * NC);
+    int32_t* gap = o_gap ? o_gap : (int32_t*)malloc(sizeof(int32_t) * NC);
+    uint8_t* bh = o_bh ? o_bh : (uint8_t*)malloc(NP);
+    float* area = o_area ? o_area : (float*)malloc(sizeof(float) * NP);
+    float* areaSmooth = (float*)malloc(sizeof(float) * NP);
+    (void)S;
+
+    /* pixInterpolation (MakeImageKernel.cu:458-515), G == 1 */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 8)
+#endif
+    for (int i = 0; i < N1; i++)
+        for (int j = 0; j < M1; j++) {
+            size_t c = (size_t)i * M1 + j;
+            float theta = view[2 * c] + ver;
+            float phi = fmodf(view[2 * c + 1] + hor + FPI2, FPI2);
+            int gs = 0;
+            f2 r = interpolate_pix(&g, theta, phi, gridlvl, &gs);
+            thphi[2 * c] = r.x;
+            thphi[2 * c + 1] = r.y;
+            gap[c] = gs;
+        }
+
+    /* findBlackPixels + findArea (MakeImageKernel.cu:539-568) */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < M; j++) {
+            float t[4], p[4];
+            int pc = 0;
+            int ok = retrieve_corners(thphi, W, i * M1 + j, t, p, &pc, 0.0f);
+            bh[(size_t)i * M + j] = ok ? 0 : 1;
+            float a = 0.f;
+            if (ok) {
+                float th1[3] = {t[0], t[1], t[2]}, ph1[3] = {p[0], p[1], p[2]};
+                float th2[3] = {t[0], t[2], t[3]}, ph2[3] = {p[0], p[2], p[3]};
+                a = calc_area3(th1, ph1) + calc_area3(th2, ph2);
+            }
+            area[(size_t)i * M + j] = a;
+        }
+
+    /* smoothAreaH / smoothAreaV (MakeImageKernel.cu:570-616) */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < M; j++) {
+            size_t ij = (size_t)i * M + j;
+            areaSmooth[ij] = 0.f;
+            if (bh[ij] == 1) continue;
+            int fs = gap[i * M1 + j];
+            if (gap[i * M1 + M1 + j] > fs) fs = gap[i * M1 + M1 + j];
+            if (gap[i * M1 + j + 1] > fs) fs = gap[i * M1 + j + 1];
+            if (gap[i * M1 + M1 + j + 1] > fs) fs = gap[i * M1 + M1 + j + 1];
+            fs = fs / 2;
+            float sum = 0.f;
+            int count = 0;
+            for (int h = -fs; h <= fs; h++) {
+                int jj = ((j + h) % M + M) % M;
+                if (bh[(size_t)i * M + jj] == 0) {
+                    sum += area[(size_t)i * M + jj];
+                    count++;
+                }
+            }
+            areaSmooth[ij] = sum / (1.f * count);
+        }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < M; j++) {
+            size_t ij = (size_t)i * M + j;
+            if (bh[ij] == 1) continue;
+            int fs = gap[i * M1 + j];
+            if (gap[i * M1 + M1 + j] > fs) fs = gap[i * M1 + M1 + j];
+            if (gap[i * M1 + j + 1] > fs) fs = gap[i * M1 + j + 1];
+            if (gap[i * M1 + M1 + j + 1] > fs) fs = gap[i * M1 + M1 + j + 1];
+            fs = fs / 2;
+            float sum = 0.f;
+            int minv = i - fs > 0 ? i - fs : 0;
+            int maxv = i + fs < N - 1 ? i + fs : N - 1;
+            int count = 0;
+            for (int h = minv; h <= maxv; h++)
+                if (bh[(size_t)h * M + j] == 0) {
+                    sum += areaSmooth[(size_t)h * M + j];
+                    count++;
+                }
+            area[ij] = sum / (1.f * count);
+        }
+
+    uint8_t* img_star = NULL;
+    uint8_t* img_map = NULL;
+
+    if (star) {
+        const int fw = 2 * step + 1;
+        float* temp = (float*)calloc(NP * fw * fw * 3, sizeof(float));
+        float maxDistSq = (step + .5f) * (step + .5f);
+        /* distortStarMap (MakeImageKernel.cu:206-326) */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+        for (int i = 0; i < N; i++) {
+            int* nodes = (int*)malloc(sizeof(int) * (1 << (tree_level + 1)));
+            for (int j = 0; j < M; j++) {
+                size_t ij = (size_t)i * M + j;
+                if (bh[ij] != 0) continue;
+                float t[4], p[4];
+                int picheck = 0;
+                int ind = i * M1 + j;
+                retrieve_corners(thphi, W, ind, t, p, &picheck, offset);
+                float mxv[2] = {fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3])),
+                                fmaxf(fmaxf(p[0], p[1]), fmaxf(p[2], p[3]))};
+                float mnv[2] = {fminf(fminf(t[0], t[1]), fminf(t[2], t[3])),
+                                fminf(fminf(p[0], p[1]), fminf(p[2], p[3]))};
+                int pos = search_tree(tree, mnv, mxv, tree_level, nodes, 1 << (tree_level + 1), 0);
+                if (pos == 0) continue;
+                float orient = (t[1] - t[0]) * (p[1] + p[0]) + (t[2] - t[1]) * (p[2] + p[1]) +
+                               (t[3] - t[2]) * (p[3] + p[2]) + (t[0] - t[3]) * (p[0] + p[3]);
+                int sgn = orient < 0 ? -1 : 1;
+                float redshft = 1.f, frac = 1.f;
+                if (lr) lensing_redshift(t, p, W, ind, cam, view, &frac, &redshft, area[ij]);
+                float red = 4.f * f_log10(redshft);
+                float* sl = temp + 3 * (size_t)fw * fw * ij;
+                for (int s = 0; s < pos; s++) {
+                    int node = nodes[s];
+                    int startN = 0;
+                    if (node != 0 && ((node + 1) & node) != 0) startN = tree[node - 1];
+                    for (int q = startN; q < tree[node]; q++) {
+                        float st = star_pos[2 * q];
+                        float sp = star_pos[2 * q + 1];
+                        int in = star_in_polygon(t, p, st, sp, sgn);
+                        if (picheck && !in && sp < FPI2 * .2f) {
+                            sp += FPI2;
+                            in = star_in_polygon(t, p, st, sp, sgn);
+                        }
+                        if (!in) continue;
+                        interpolate_star(t[0], t[1], t[2], t[3], p[0], p[1], p[2], p[3], &st, &sp, sgn);
+                        float part = star_mag[2 * q] + red;
+                        float bv = star_mag[2 * q + 1];
+                        float temp_k = 46.f / redshft * ((1.f / ((0.92f * bv) + 1.7f)) +
+                                                         (1.f / ((0.92f * bv) + 0.62f))) - 10.f;
+                        /* the reference clamps to 1170 (= 3 * 390, an element index)
+                         * and reads past its 391-entry table; clamp to entry 390 */
+                        int index = (int)floorf(temp_k);
+                        if (index > 390) index = 390;
+                        if (index < 0) index = 0;
+                        float r0 = (float)kTempRGB[3 * index], g0 = (float)kTempRGB[3 * index + 1],
+                              b0 = (float)kTempRGB[3 * index + 2];
+                        float hh, ss, pp;
+                        rgb_to_hsp(r0 / 255.f, g0 / 255.f, b0 / 255.f, &hh, &ss, &pp);
+                        for (int u = 0; u <= 2 * step; u++)
+                            for (int v = 0; v <= 2 * step; v++) {
+                                float dist = (-step + u + .5f - st) * (-step + u + .5f - st) +
+                                             (-step + v + .5f - sp) * (-step + v + .5f - sp);
+                                if (dist > maxDistSq) continue;
+                                float appMag = part - 2.5f * f_log10(frac * gaussian(dist));
+                                float brightness = 100 * f_exp10(-.4f * appMag);
+                                float rr, gg, bb;
+                                hsp_to_rgb(hh, ss, pp * brightness, &rr, &gg, &bb);
+                                rr *= 255.f;
+                                gg *= 255.f;
+                                bb *= 255.f;
+                                float* o = sl + 3 * (fw * u + v);
+                                o[0] += rr * rr;
+                                o[1] += gg * gg;
+                                o[2] += bb * bb;
+                            }
+                    }
+                }
+            }
+            free(nodes);
+        }
+        /* sumStarLight (MakeImageKernel.cu:753-773) and makePix (:802-835) */
+        img_star = (uint8_t*)malloc(NP * 4);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < M; j++) {
+                float br = 0.f, bg = 0.f, bb = 0.f;
+                int start = step - i > 0 ? step - i : 0;
+                int stop = 2 * step < step + N - i - 1 ? 2 * step : step + N - i - 1;
+                float factor = 20.f;
+                for (int u = start; u <= stop; u++)
+                    for (int v = 0; v <= 2 * step; v++) {
+                        size_t src = (size_t)(i + u - step) * M + ((j + v - step + M) % M);
+                        const float* s = temp + 3 * ((size_t)fw * fw * src + fw * fw - (fw * u + v + 1));
+                        br += factor * s[0];
+                        bg += factor * s[1];
+                        bb += factor * s[2];
+                    }
+                if (o_star) {
+                    size_t ij = (size_t)i * M + j;
+                    o_star[3 * ij] = br;
+                    o_star[3 * ij + 1] = bg;
+                    o_star[3 * ij + 2] = bb;
+                }
+                float rgb[3] = {sqrtf(br), sqrtf(bg), sqrtf(bb)};
+                float mxc = fmaxf(fmaxf(rgb[0], rgb[1]), rgb[2]);
+                if (mxc > 255.f) {
+                    float hh, ss, pp;
+                    rgb_to_hsp(rgb[0] / 255.f, rgb[1] / 255.f, rgb[2] / 255.f, &hh, &ss, &pp);
+                    pp = 1.0f;
+                    hsp_to_rgb(hh, ss, pp, &rgb[0], &rgb[1], &rgb[2]);
+                    rgb[0] *= 255;
+                    rgb[1] *= 255;
+                    rgb[2] *= 255;
+                }
+                uint8_t* o = img_star + 4 * ((size_t)i * M + j);
+                int cb = (int)(rgb[2] + 0.f), cg = (int)(rgb[1] + 0.f), cr = (int)(rgb[0] + 0.f);
+                o[0] = (uint8_t)(cb < 255 ? cb : 255);
+                o[1] = (uint8_t)(cg < 255 ? cg : 255);
+                o[2] = (uint8_t)(cr < 255 ? cr : 255);
+                o[3] = 255;
+            }
+        free(temp);
+    }
+
+    if (map) {
+        img_map = (uint8_t*)malloc(NP * 4);
+        const int rows = sky_rows, cols = sky_cols;
+        /* distortEnvironmentMap (MakeImageKernel.cu:631-751) */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < M; j++) {
+                size_t ij = (size_t)i * M + j;
+                float color[4] = {0.f, 0.f, 0.f, 0.f};
+                if (bh[ij] == 0) {
+                    float t[4], p[4];
+                    int ind = i * M1 + j;
+                    int picheck = 0;
+                    retrieve_corners(thphi, W, ind, t, p, &picheck, offset);
+                    if (ind > 0) {
+                        float pixSize = FPI / (float)rows;
+                        float phMax = fmaxf(fmaxf(p[0], p[1]), fmaxf(p[2], p[3]));
+                        float phMin = fminf(fminf(p[0], p[1]), fminf(p[2], p[3]));
+                        int pixMax = (int)(phMax / pixSize);
+                        int pixMin = (int)(phMin / pixSize);
+                        int pixNum = pixMax - pixMin + 1;
+                        if (pixNum > 1 && pixMax * pixSize > phMax) pixNum -= 1;
+                        float thMax = fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3]));
+                        float thMin = fminf(fminf(t[0], t[1]), fminf(t[2], t[3]));
+                        if (pixNum == 1) {
+                            int rmax = (int)(thMax / pixSize);
+                            int rmin = (int)(thMin / pixSize) - 1;
+                            if (rmax > rows - 1) rmax = rows - 1;
+                            if (rmax < 0) rmax = 0;
+                            const float* mxc = sum_table + 4 * ((size_t)rmax * cols + pixMin);
+                            float mnc[4] = {0.f, 0.f, 0.f, 0.f};
+                            if (rmin >= 0) {
+                                if (rmin > rows - 1) rmin = rows - 1;
+                                const float* s = sum_table + 4 * ((size_t)rmin * cols + pixMin);
+                                mnc[0] = s[0]; mnc[1] = s[1]; mnc[2] = s[2]; mnc[3] = s[3];
+                            }
+                            for (int c = 0; c < 4; c++) color[c] += mxc[c] - mnc[c];
+                        } else {
+                            float max1 = -100, min1 = cols + 1, max2 = -100, min2 = cols + 1;
+                            if (pixNum < cols) {
+                                for (int s = 0; s <= pixNum; s++) {
+                                    float cp = (pixMin + s) * pixSize;
+                                    for (int q = 0; q < 4; q++) {
+                                        float ap = p[q], bp = p[(q + 1) % 4];
+                                        float at = t[q], bt = t[(q + 1) % 4];
+                                        if ((cp - ap) * (cp - bp) <= 0) {
+                                            float ct = at + (bt - at) / (bp - ap) * (cp - ap);
+                                            min2 = ct < min2 ? ct : min2;
+                                            max2 = ct > max2 ? ct : max2;
+                                        }
+                                    }
+                                    if (s > 0) {
+                                        float max_ = fmaxf(max1, max2);
+                                        float min_ = fminf(min1, min2);
+                                        int rmax = (int)(max_ / pixSize);
+                                        if (rmax > rows - 1) rmax = rows - 1;
+                                        if (rmax < 0) rmax = 0;
+                                        int col = (pixMin + s) % cols;
+                                        const float* mxc = sum_table + 4 * ((size_t)rmax * cols + col);
+                                        int index2 = ((int)(min_ / pixSize) - 1) * cols + col;
+                                        float mnc[4] = {0.f, 0.f, 0.f, 0.f};
+                                        if (index2 > 0) {
+                                            if (index2 >= rows * cols) index2 = (rows - 1) * cols + col;
+                                            const float* s2 = sum_table + 4 * (size_t)index2;
+                                            mnc[0] = s2[0]; mnc[1] = s2[1]; mnc[2] = s2[2]; mnc[3] = s2[3];
+                                        }
+                                        for (int c = 0; c < 4; c++) color[c] += mxc[c] - mnc[c];
+                                    }
+                                    max1 = max2;
+                                    min1 = min2;
+                                }
+                            } else {
+                                int thMaxPix = (int)(thMax / pixSize);
+                                int thMinPix = (int)(thMin / pixSize);
+                                if (thMaxPix > rows - 1) thMaxPix = rows - 1;
+                                if (thMinPix > rows - 1) thMinPix = rows - 1;
+                                if (thMaxPix < 0) thMaxPix = 0;
+                                if (thMinPix < 0) thMinPix = 0;
+                                thMaxPix *= cols;
+                                thMinPix *= cols;
+                                for (int q = 0; q < pixNum; q++) {
+                                    const float* mxc = sum_table + 4 * (size_t)(thMaxPix + (pixMin + q) % cols);
+                                    const float* mnc = sum_table + 4 * (size_t)(thMinPix + (pixMin + q) % cols);
+                                    for (int c = 0; c < 4; c++) color[c] += mxc[c] - mnc[c];
+                                }
+                            }
+                        }
+                        color[0] = fminf(255.f, f_pow(color[0] / color[3], 1.f / 2.2f));
+                        color[1] = fminf(255.f, f_pow(color[1] / color[3], 1.f / 2.2f));
+                        color[2] = fminf(255.f, f_pow(color[2] / color[3], 1.f / 2.2f));
+                        float Hh, Ss, Pp;
+                        rgb_to_hsp(color[2] / 255.f, color[1] / 255.f, color[0] / 255.f, &Hh, &Ss, &Pp);
+                        if (lr) {
+                            float redshft, frac;
+                            lensing_redshift(t, p, W, ind, cam, view, &frac, &redshft, area[ij]);
+                            Pp *= frac;
+                            Pp = f_pow(Pp, redshft);
+                        }
+                        hsp_to_rgb(Hh, Ss, fminf(1.f, Pp), &color[2], &color[1], &color[0]);
+                    }
+                }
+                uint8_t* o = img_map + 4 * ij;
+                int cx = (int)(color[0] * 255), cy = (int)(color[1] * 255), cz = (int)(color[2] * 255);
+                o[0] = (uint8_t)(cx < 255 ? cx : 255);
+                o[1] = (uint8_t)(cy < 255 ? cy : 255);
+                o[2] = (uint8_t)(cz < 255 ? cz : 255);
+                o[3] = 255;
+            }
+    }
+
+    /* addStarsAndBackground (MakeImageKernel.cu:837-853) */
+    for (size_t q = 0; q < NP; q++) {
+        uint8_t* o = out_bgra + 4 * q;
+        if (star && map) {
+            const uint8_t* s = img_star + 4 * q;
+            const uint8_t* b = img_map + 4 * q;
+            for (int c = 0; c < 3; c++) {
+                float sv = (float)s[c] * s[c];
+                float bv = (float)b[c] * b[c];
+                float p = 1.f;
+                int v = (int)sqrtf(p * sv + (2.f - p) * bv);
+                o[c] = (uint8_t)(v < 255 ? v : 255);
+            }
+            o[3] = 255;
+        } else if (map) {
+            memcpy(o, img_map + 4 * q, 4);
+        } else if (star) {
+            memcpy(o, img_star + 4 * q, 4);
+        } else {
+            o[0] = o[1] = o[2] = 0;
+            o[3] = 255;
+        }
+    }
+    free(img_star);
+    free(img_map);
+    free(areaSmooth);
+    if (!o_thphi) free(thphi);
+    if (!o_gap) free(gap);
+    if (!o_bh) free(bh);
+    if (!o_area) free(area);
+    return 0;
+}

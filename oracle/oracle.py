@@ -82,6 +82,25 @@ def lib():
         L.bto_grid_fix_tvertices.argtypes = [C.POINTER(Grid)]
         L.bto_grid_free.argtypes = [C.POINTER(Grid)]
         L.bto_hermite2.argtypes = [C.c_double, DP, DP, DP, DP, DP]
+        # Path B
+        L.btor_camera_view.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, FP]
+        L.btor_sphere_view.argtypes = [C.c_int, C.c_int, FP]
+        L.btor_make_tree.argtypes = [FP, C.c_int, C.c_int, IP, FP, FP]
+        L.btor_sum_table.argtypes = [U8P, C.c_int, C.c_int, FP]
+        L.btor_make_grid.argtypes = [FP, IP, IP, C.c_int, C.c_int, C.c_int, C.c_int, FP]
+        L.btor_frame.argtypes = [FP, C.c_int, C.c_int, C.c_int, FP, C.c_int, C.c_int, FP, IP,
+                                 C.c_int, FP, FP, C.c_int, FP, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, U8P, FP, IP,
+                                 U8P, FP, FP]
+        L.btor_rgb_to_hsp.argtypes = [C.c_float] * 3 + [FP]
+        L.btor_hsp_to_rgb.argtypes = [C.c_float] * 3 + [FP]
+        for n in ("btor_calc_area4", "btor_calc_area3"):
+            getattr(L, n).argtypes = [FP, FP]
+            getattr(L, n).restype = C.c_float
+        L.btor_redshift.argtypes = [C.c_float, C.c_float, FP]
+        L.btor_redshift.restype = C.c_float
+        L.btor_star_in_polygon.argtypes = [FP, FP, C.c_float, C.c_float, C.c_int]
+        L.btor_search_tree.argtypes = [IP, FP, FP, C.c_int, IP, C.c_int]
         _LIB = L
     return _LIB
 
@@ -187,3 +206,77 @@ class OracleGrid:
             lib().bto_grid_free(C.byref(self.g))
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- Path B
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def camera_view(view_angle, off_up, off_right, w, h):
+    """Viewer(viewangle, offUp, offRight, w, h, false).viewMatrix as (h+1, w+1, 2)."""
+    v = np.zeros((h + 1, w + 1, 2), dtype=np.float32)
+    lib().btor_camera_view(view_angle, off_up, off_right, w, h, _ptr(v, FP))
+    return v
+
+
+def sphere_view(w, h):
+    v = np.zeros((h + 1, w + 1, 2), dtype=np.float32)
+    lib().btor_sphere_view(w, h, _ptr(v, FP))
+    return v
+
+
+def make_tree(stars4, tree_level):
+    """StarProcessor::makeTree: returns (tree, pos (S,2), mag (S,2))."""
+    s4 = f32(stars4)
+    S = s4.shape[0]
+    tree = np.zeros((1 << (tree_level + 1)) - 1, dtype=np.int32)
+    pos = np.zeros((S, 2), dtype=np.float32)
+    mag = np.zeros((S, 2), dtype=np.float32)
+    lib().btor_make_tree(_ptr(s4, FP), S, tree_level, _ptr(tree, IP), _ptr(pos, FP), _ptr(mag, FP))
+    return tree, pos, mag
+
+
+def sum_table(sky_bgr):
+    sky = np.ascontiguousarray(sky_bgr, dtype=np.uint8)
+    rows, cols = sky.shape[:2]
+    t = np.zeros((rows, cols, 4), dtype=np.float32)
+    lib().btor_sum_table(_ptr(sky, U8P), rows, cols, _ptr(t, FP))
+    return t
+
+
+def make_grid(hash_table, offset_table, hash_pos_tag, hw, ow, GN, GM):
+    sym = GM / GN > 3
+    GN1 = 2 * GN - 1 if sym else GN
+    grid = np.zeros((GN1, GM, 2), dtype=np.float32)
+    lib().btor_make_grid(_ptr(f32(hash_table), FP), _ptr(np.ascontiguousarray(offset_table, np.int32), IP),
+                         _ptr(np.ascontiguousarray(hash_pos_tag, np.int32), IP), hw, ow, GN, GM,
+                         _ptr(grid, FP))
+    return grid
+
+
+def frame(grid, gridlvl, view, cam7, tree, tree_level, star_pos, star_mag, table, star=True,
+          map=True, lr=True, hor=0.0, ver=0.0, offset=0.0, step=1, stages=False):
+    """One G == 1 frame; returns the BGRA image (h, w, 4) [and intermediates]."""
+    grid = f32(grid)
+    view = f32(view)
+    h, w = view.shape[0] - 1, view.shape[1] - 1
+    GN, GM = grid.shape[:2]
+    out = np.zeros((h, w, 4), dtype=np.uint8)
+    thphi = np.zeros((h + 1, w + 1, 2), dtype=np.float32)
+    gap = np.zeros((h + 1, w + 1), dtype=np.int32)
+    bh = np.zeros((h, w), dtype=np.uint8)
+    area = np.zeros((h, w), dtype=np.float32)
+    starl = np.zeros((h, w, 3), dtype=np.float32)
+    pos, mag = f32(star_pos), f32(star_mag)
+    table = f32(table)
+    lib().btor_frame(_ptr(grid, FP), GN, GM, gridlvl, _ptr(view, FP), w, h, _ptr(f32(cam7), FP),
+                     _ptr(np.ascontiguousarray(tree, np.int32), IP), tree_level, _ptr(pos, FP),
+                     _ptr(mag, FP), pos.shape[0], _ptr(table, FP), table.shape[0], table.shape[1],
+                     int(star), int(map), int(lr), hor, ver, offset, step, _ptr(out, U8P),
+                     _ptr(thphi, FP), _ptr(gap, IP), _ptr(bh, U8P), _ptr(area, FP), _ptr(starl, FP))
+    if stages:
+        return out, dict(thphi=thphi, gap=gap, bh=bh, area=area, star=starl)
+    return out
