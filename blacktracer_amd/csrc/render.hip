@@ -19,6 +19,8 @@
 // Pixel (i, j) is processed by thread x = j, y = i so that a wavefront
 // (16 x 4 pixels of a 16 x 16 block) reads consecutive row addresses.
 // Compiled with -ffp-contract=off: results equal the CPU oracle bit for bit.
+#include <cmath>
+
 #include "bt_render.h"
 #include "render_dev.h"
 
@@ -416,7 +418,7 @@ __global__ __launch_bounds__(256) void star_map(const float* thphi, const uint8_
                 float part = mg.x + red;
                 float temp = 46.f / redshft * ((1.f / ((0.92f * mg.y) + 1.7f)) + (1.f / ((0.92f * mg.y) + 0.62f))) - 10.f;
                 // clamp to the table's last entry (the reference's 1170 reads past it)
-                int index = max(0, min((int)floorf(temp), 390));
+                int index = max(0, min(cvt_i32(floorf(temp)), 390));
                 float hh, ss, pp;
                 rgb_to_hsp((float)kTempRGB[3 * index] / 255.f, (float)kTempRGB[3 * index + 1] / 255.f,
                            (float)kTempRGB[3 * index + 2] / 255.f, hh, ss, pp);
@@ -540,9 +542,9 @@ __global__ __launch_bounds__(256) void star_sum(const float* taps, int W, int H,
         b *= 255;
     }
     uchar4 o;
-    o.x = (uint8_t)min(255, (int)(b + 0.f));
-    o.y = (uint8_t)min(255, (int)(g + 0.f));
-    o.z = (uint8_t)min(255, (int)(r + 0.f));
+    o.x = (uint8_t)min(255, cvt_i32(b + 0.f));
+    o.y = (uint8_t)min(255, cvt_i32(g + 0.f));
+    o.z = (uint8_t)min(255, cvt_i32(r + 0.f));
     o.w = 255;
     reinterpret_cast<uchar4*>(img)[ij] = o;
 }
@@ -567,18 +569,19 @@ __global__ __launch_bounds__(256) void env_map(const float* thphi, const uint8_t
             float pixSize = BT_FPI / (float)rows;
             float phMax = fmaxf(fmaxf(p[0], p[1]), fmaxf(p[2], p[3]));
             float phMin = fminf(fminf(p[0], p[1]), fminf(p[2], p[3]));
-            int pixMax = (int)(phMax / pixSize);
-            int pixMin = (int)(phMin / pixSize);
+            int pixMax = cvt_i32(phMax / pixSize);
+            int pixMin = cvt_i32(phMin / pixSize);
             int pixNum = pixMax - pixMin + 1;
             if (pixNum > 1 && pixMax * pixSize > phMax) pixNum -= 1;
             float thMax = fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3]));
             float thMin = fminf(fminf(t[0], t[1]), fminf(t[2], t[3]));
             if (pixNum == 1) {
-                int rmax = min(max((int)(thMax / pixSize), 0), rows - 1);
-                int rmin = (int)(thMin / pixSize) - 1;
-                float4 mxc = table[(size_t)rmax * cols + pixMin];
+                int rmax = min(max(cvt_i32(thMax / pixSize), 0), rows - 1);
+                int rmin = cvt_i32(thMin / pixSize) - 1;
+                int col = min(max(pixMin, 0), cols - 1);
+                float4 mxc = table[(size_t)rmax * cols + col];
                 float4 mnc = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (rmin >= 0) mnc = table[(size_t)min(rmin, rows - 1) * cols + pixMin];
+                if (rmin >= 0) mnc = table[(size_t)min(rmin, rows - 1) * cols + col];
                 cx += mxc.x - mnc.x;
                 cy += mxc.y - mnc.y;
                 cz += mxc.z - mnc.z;
@@ -600,10 +603,10 @@ __global__ __launch_bounds__(256) void env_map(const float* thphi, const uint8_t
                         if (sidx > 0) {
                             float max_ = fmaxf(max1, max2);
                             float min_ = fminf(min1, min2);
-                            int rmax = min(max((int)(max_ / pixSize), 0), rows - 1);
+                            int rmax = min(max(cvt_i32(max_ / pixSize), 0), rows - 1);
                             int col = (pixMin + sidx) % cols;
                             float4 mxc = table[(size_t)rmax * cols + col];
-                            int index2 = ((int)(min_ / pixSize) - 1) * cols + col;
+                            int index2 = (cvt_i32(min_ / pixSize) - 1) * cols + col;
                             float4 mnc = make_float4(0.f, 0.f, 0.f, 0.f);
                             if (index2 > 0) {
                                 if (index2 >= rows * cols) index2 = (rows - 1) * cols + col;
@@ -618,8 +621,8 @@ __global__ __launch_bounds__(256) void env_map(const float* thphi, const uint8_t
                         min1 = min2;
                     }
                 } else {
-                    int thMaxPix = min(max((int)(thMax / pixSize), 0), rows - 1) * cols;
-                    int thMinPix = min(max((int)(thMin / pixSize), 0), rows - 1) * cols;
+                    int thMaxPix = min(max(cvt_i32(thMax / pixSize), 0), rows - 1) * cols;
+                    int thMinPix = min(max(cvt_i32(thMin / pixSize), 0), rows - 1) * cols;
                     for (int q = 0; q < pixNum; q++) {
                         float4 mxc = table[thMaxPix + (pixMin + q) % cols];
                         float4 mnc = table[thMinPix + (pixMin + q) % cols];
@@ -645,9 +648,9 @@ __global__ __launch_bounds__(256) void env_map(const float* thphi, const uint8_t
         }
     }
     uchar4 o;
-    o.x = (uint8_t)min(255, (int)(cx * 255));
-    o.y = (uint8_t)min(255, (int)(cy * 255));
-    o.z = (uint8_t)min(255, (int)(cz * 255));
+    o.x = (uint8_t)min(255, cvt_i32(cx * 255));
+    o.y = (uint8_t)min(255, cvt_i32(cy * 255));
+    o.z = (uint8_t)min(255, cvt_i32(cz * 255));
     o.w = 255;
     reinterpret_cast<uchar4*>(img)[ij] = o;
 }
@@ -664,9 +667,9 @@ __global__ __launch_bounds__(256) void composite(const uint8_t* img_star, const 
         float p = 1.f;
         float sx = (float)s.x * s.x, sy = (float)s.y * s.y, sz = (float)s.z * s.z;
         float bx = (float)b.x * b.x, by = (float)b.y * b.y, bz = (float)b.z * b.z;
-        o.x = (uint8_t)min((int)sqrtf(p * sx + (2.f - p) * bx), 255);
-        o.y = (uint8_t)min((int)sqrtf(p * sy + (2.f - p) * by), 255);
-        o.z = (uint8_t)min((int)sqrtf(p * sz + (2.f - p) * bz), 255);
+        o.x = (uint8_t)min(cvt_i32(sqrtf(p * sx + (2.f - p) * bx)), 255);
+        o.y = (uint8_t)min(cvt_i32(sqrtf(p * sy + (2.f - p) * by)), 255);
+        o.z = (uint8_t)min(cvt_i32(sqrtf(p * sz + (2.f - p) * bz)), 255);
         o.w = 255;
     } else if (map) {
         o = reinterpret_cast<const uchar4*>(img_map)[q];
@@ -778,6 +781,12 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
     const size_t NP = (size_t)W * H, NC = (size_t)(W + 1) * (H + 1);
     if (step != 1 && step != 2 && step != 3)
         return fail(BT_ERR_INVALID, "render: star splat step must be 1, 2 or 3");
+    // the reference's frame offset is 2*pi*q/(...) >= 0 (MakeImageKernel.cu:1413); the sky
+    // lookups assume phi + offset >= 0
+    if (!(f.offset >= 0.f) || !std::isfinite(f.offset) || !std::isfinite(f.hor) || !std::isfinite(f.ver))
+        return fail(BT_ERR_INVALID, "render: offset must be finite and >= 0; hor/ver finite");
+    if (W <= 0 || H <= 0 || g.GN < 2 || g.GM < 2 || s.rows <= 0 || s.cols <= 0)
+        return fail(BT_ERR_INVALID, "render: empty frame, grid or sky");
     void* p;
     int rc;
     if ((rc = c->buffer("r_thphi", sizeof(float) * 2 * NC, &p))) return rc;

@@ -22,6 +22,15 @@ struct F2 {
     float x, y;
 };
 
+// float -> int as CUDA's cvt.rzi.s32.f32: truncate, saturate, NaN -> 0
+// (LLVM's fptosi is poison for NaN and out-of-range inputs).
+__device__ __forceinline__ int cvt_i32(float x) {
+    if (!(x == x)) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x <= -2147483648.0f) return -2147483647 - 1;
+    return (int)x;
+}
+
 __device__ __forceinline__ float f_sin(float x) { return (float)sin((double)x); }
 __device__ __forceinline__ float f_cos(float x) { return (float)cos((double)x); }
 __device__ __forceinline__ float f_atan(float x) { return (float)atan((double)x); }

@@ -72,6 +72,15 @@ static float f_exp10(float x) { return (float)pow(10.0, (double)x); }
 
 typedef struct { float x, y; } f2;
 
+/* float -> int as CUDA's cvt.rzi.s32.f32 does it: truncate, saturate,
+ * NaN -> 0 (a plain C cast of NaN or an out-of-range value is undefined). */
+static int cvt_i32(float x) {
+    if (!(x == x)) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x <= -2147483648.0f) return -2147483647 - 1;
+    return (int)x;
+}
+
 /* ------------------------------------------------------------ Viewer */
 
 /* Viewer.h:20-33 + 109-133 */
@@ -1071,7 +1080,7 @@ int btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* view
                                                          (1.f / ((0.92f * bv) + 0.62f))) - 10.f;
                         /* the reference clamps to 1170 (= 3 * 390, an element index)
                          * and reads past its 391-entry table; clamp to entry 390 */
-                        int index = (int)floorf(temp_k);
+                        int index = cvt_i32(floorf(temp_k));
                         if (index > 390) index = 390;
                         if (index < 0) index = 0;
                         float r0 = (float)kTempRGB[3 * index], g0 = (float)kTempRGB[3 * index + 1],
@@ -1137,7 +1146,7 @@ int btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* view
                     rgb[2] *= 255;
                 }
                 uint8_t* o = img_star + 4 * ((size_t)i * M + j);
-                int cb = (int)(rgb[2] + 0.f), cg = (int)(rgb[1] + 0.f), cr = (int)(rgb[0] + 0.f);
+                int cb = cvt_i32(rgb[2] + 0.f), cg = cvt_i32(rgb[1] + 0.f), cr = cvt_i32(rgb[0] + 0.f);
                 o[0] = (uint8_t)(cb < 255 ? cb : 255);
                 o[1] = (uint8_t)(cg < 255 ? cg : 255);
                 o[2] = (uint8_t)(cr < 255 ? cr : 255);
@@ -1166,22 +1175,23 @@ int btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* view
                         float pixSize = FPI / (float)rows;
                         float phMax = fmaxf(fmaxf(p[0], p[1]), fmaxf(p[2], p[3]));
                         float phMin = fminf(fminf(p[0], p[1]), fminf(p[2], p[3]));
-                        int pixMax = (int)(phMax / pixSize);
-                        int pixMin = (int)(phMin / pixSize);
+                        int pixMax = cvt_i32(phMax / pixSize);
+                        int pixMin = cvt_i32(phMin / pixSize);
                         int pixNum = pixMax - pixMin + 1;
                         if (pixNum > 1 && pixMax * pixSize > phMax) pixNum -= 1;
                         float thMax = fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3]));
                         float thMin = fminf(fminf(t[0], t[1]), fminf(t[2], t[3]));
                         if (pixNum == 1) {
-                            int rmax = (int)(thMax / pixSize);
-                            int rmin = (int)(thMin / pixSize) - 1;
+                            int rmax = cvt_i32(thMax / pixSize);
+                            int rmin = cvt_i32(thMin / pixSize) - 1;
                             if (rmax > rows - 1) rmax = rows - 1;
                             if (rmax < 0) rmax = 0;
-                            const float* mxc = sum_table + 4 * ((size_t)rmax * cols + pixMin);
+                            int col = pixMin < 0 ? 0 : (pixMin > cols - 1 ? cols - 1 : pixMin);
+                            const float* mxc = sum_table + 4 * ((size_t)rmax * cols + col);
                             float mnc[4] = {0.f, 0.f, 0.f, 0.f};
                             if (rmin >= 0) {
                                 if (rmin > rows - 1) rmin = rows - 1;
-                                const float* s = sum_table + 4 * ((size_t)rmin * cols + pixMin);
+                                const float* s = sum_table + 4 * ((size_t)rmin * cols + col);
                                 mnc[0] = s[0]; mnc[1] = s[1]; mnc[2] = s[2]; mnc[3] = s[3];
                             }
                             for (int c = 0; c < 4; c++) color[c] += mxc[c] - mnc[c];
@@ -1202,12 +1212,12 @@ int btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* view
                                     if (s > 0) {
                                         float max_ = fmaxf(max1, max2);
                                         float min_ = fminf(min1, min2);
-                                        int rmax = (int)(max_ / pixSize);
+                                        int rmax = cvt_i32(max_ / pixSize);
                                         if (rmax > rows - 1) rmax = rows - 1;
                                         if (rmax < 0) rmax = 0;
                                         int col = (pixMin + s) % cols;
                                         const float* mxc = sum_table + 4 * ((size_t)rmax * cols + col);
-                                        int index2 = ((int)(min_ / pixSize) - 1) * cols + col;
+                                        int index2 = (cvt_i32(min_ / pixSize) - 1) * cols + col;
                                         float mnc[4] = {0.f, 0.f, 0.f, 0.f};
                                         if (index2 > 0) {
                                             if (index2 >= rows * cols) index2 = (rows - 1) * cols + col;
@@ -1220,8 +1230,8 @@ int btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* view
                                     min1 = min2;
                                 }
                             } else {
-                                int thMaxPix = (int)(thMax / pixSize);
-                                int thMinPix = (int)(thMin / pixSize);
+                                int thMaxPix = cvt_i32(thMax / pixSize);
+                                int thMinPix = cvt_i32(thMin / pixSize);
                                 if (thMaxPix > rows - 1) thMaxPix = rows - 1;
                                 if (thMinPix > rows - 1) thMinPix = rows - 1;
                                 if (thMaxPix < 0) thMaxPix = 0;
@@ -1250,7 +1260,7 @@ int btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* view
                     }
                 }
                 uint8_t* o = img_map + 4 * ij;
-                int cx = (int)(color[0] * 255), cy = (int)(color[1] * 255), cz = (int)(color[2] * 255);
+                int cx = cvt_i32(color[0] * 255), cy = cvt_i32(color[1] * 255), cz = cvt_i32(color[2] * 255);
                 o[0] = (uint8_t)(cx < 255 ? cx : 255);
                 o[1] = (uint8_t)(cy < 255 ? cy : 255);
                 o[2] = (uint8_t)(cz < 255 ? cz : 255);
@@ -1268,7 +1278,7 @@ int btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* view
                 float sv = (float)s[c] * s[c];
                 float bv = (float)b[c] * b[c];
                 float p = 1.f;
-                int v = (int)sqrtf(p * sv + (2.f - p) * bv);
+                int v = cvt_i32(sqrtf(p * sv + (2.f - p) * bv));
                 o[c] = (uint8_t)(v < 255 ? v : 255);
             }
             o[3] = 255;

@@ -70,7 +70,7 @@ def test_frame_matches_oracle(ctx, world, flags):
     nb = st["bh"] == 0
     assert np.array_equal(st["area"][nb], ost["area"][nb])
     if f.star:
-        assert np.allclose(st["star"], ost["star"], rtol=1e-5, atol=1e-3)
+        assert np.allclose(st["star"], ost["star"], rtol=1e-5, atol=1e-3, equal_nan=True)
     compare_frames(img, oimg)
     assert img[..., 3].min() == 255
 
