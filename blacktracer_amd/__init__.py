@@ -50,10 +50,45 @@ class Context:
     def reset_timers(self):
         check(_lib.load().bt_timer_reset(self.handle))
 
+    def alloc(self, nbytes):
+        """A device buffer owned by the caller (DeviceBuffer)."""
+        return DeviceBuffer(self, nbytes)
+
     def close(self):
         if getattr(self, "handle", None):
             _lib.load().bt_context_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """Device memory from bt_device_alloc, freed with the object."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(_lib.load().bt_device_alloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    def to_host(self, out):
+        check(_lib.load().bt_memcpy(self.ctx.handle, out.ctypes.data_as(C.c_void_p), self.ptr,
+                                    min(self.nbytes, out.nbytes), 0))
+        return out
+
+    def from_host(self, arr):
+        check(_lib.load().bt_memcpy(self.ctx.handle, self.ptr, arr.ctypes.data_as(C.c_void_p),
+                                    min(self.nbytes, arr.nbytes), 1))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            _lib.load().bt_device_free(self.ctx.handle, self.ptr)
+            self.ptr = None
 
     def __del__(self):
         try:
@@ -326,6 +361,15 @@ class View:
             self.close()
         except Exception:
             pass
+
+
+def render_frame_device(grid, scene, view, out_buffer, flags=None, cam7=None, step=1, ctx=None):
+    """One frame into a device buffer (asynchronous; no host copy)."""
+    ctx = ctx or grid.ctx
+    flags = flags or RenderFlags()
+    c7 = None if cam7 is None else _f32(cam7).ctypes.data_as(_lib.FP)
+    check(_lib.load().bt_render_frame(ctx.handle, grid.handle, scene.handle, view.handle, c7,
+                                      C.byref(flags), step, None, out_buffer.ptr))
 
 
 def render_frame(grid, scene, view, flags=None, cam7=None, step=1, stages=False, ctx=None):

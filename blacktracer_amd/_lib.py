@@ -68,6 +68,9 @@ SIGNATURES = {
     "bt_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "bt_timer_get": (C.c_int, [C.c_void_p, C.c_char_p, I64P, DP]),
     "bt_timer_reset": (C.c_int, [C.c_void_p]),
+    "bt_device_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "bt_device_free": (None, [C.c_void_p, C.c_void_p]),
+    "bt_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
     "bt_camera_init": (C.c_int, [C.POINTER(BtPose), DP, FP]),
     "bt_integ_params_default": (None, [C.POINTER(BtIntegParams)]),
     "bt_trace_rays": (C.c_int, [C.c_void_p, C.POINTER(BtPose), C.POINTER(BtIntegParams), C.c_int64,

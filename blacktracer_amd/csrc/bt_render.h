@@ -20,7 +20,21 @@ struct SceneK {
     int S;
     const float* table;      // rows * cols float4
     int rows, cols;
+    // binned copy of the stars: kBinT x kBinP cells over (theta, phi), stars
+    // sorted by cell (then by tree order); cell c holds [bin_start[c], bin_start[c+1])
+    const int32_t* bin_start;
+    const float* bin_pos;
+    const float* bin_mag;
 };
+
+constexpr int kBinT = 256;
+constexpr int kBinP = 512;
+constexpr float kBinDT = 3.141592653589793238f / kBinT;
+constexpr float kBinDP = 6.283185307179586476f / kBinP;
+
+// Host: the binned star arrays for SceneK (bin_start kBinT*kBinP+1 ints,
+// pos/mag 2*S floats each), from the tree-ordered arrays.
+void star_bins(const float* pos, const float* mag, int S, int32_t* bin_start, float* bin_pos, float* bin_mag);
 
 // Pixel-corner camera-sky directions ((H+1) x (W+1) float2).
 struct ViewK {

@@ -387,3 +387,36 @@ int bt_trace_rays(bt_context* h, const bt_pose* pose, const bt_integ_params* prm
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int bt_device_alloc(bt_context* h, size_t bytes, void** out) {
+    if (!h || !out || bytes == 0) return fail(BT_ERR_INVALID, "bt_device_alloc: bad argument");
+    std::lock_guard<std::recursive_mutex> lk(h->impl.mu);
+    BT_HIP(hipSetDevice(h->impl.device));
+    if (hipMalloc(out, bytes) != hipSuccess) {
+        *out = nullptr;
+        return fail(BT_ERR_NOMEM, "bt_device_alloc: hipMalloc failed");
+    }
+    return BT_OK;
+}
+
+void bt_device_free(bt_context* h, void* ptr) {
+    if (!h || !ptr) return;
+    std::lock_guard<std::recursive_mutex> lk(h->impl.mu);
+    hipSetDevice(h->impl.device);
+    hipStreamSynchronize(h->impl.stream);
+    hipFree(ptr);
+}
+
+int bt_memcpy(bt_context* h, void* dst, const void* src, size_t bytes, int to_device) {
+    if (!h || (bytes && (!dst || !src))) return fail(BT_ERR_INVALID, "bt_memcpy: bad argument");
+    std::lock_guard<std::recursive_mutex> lk(h->impl.mu);
+    BT_HIP(hipSetDevice(h->impl.device));
+    BT_HIP(hipMemcpyAsync(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost,
+                          h->impl.stream));
+    BT_HIP(hipStreamSynchronize(h->impl.stream));
+    return BT_OK;
+}
+
+}  // extern "C"

@@ -359,20 +359,206 @@ __device__ void lensing_redshift(const float* t, const float* p, int W, int ind,
     redshft = redshift(thetaCam, phiCam, cam.v);
 }
 
-// Per-pixel star accumulation: the 3x3 taps live in registers.
-template <int STEP>
-struct Taps {
-    static constexpr int FW = 2 * STEP + 1;
-    float v[FW * FW][3];
+// ---- star pass ---------------------------------------------------------
+// distortStarMap (MakeImageKernel.cu:206-326).
+//
+// Candidates.  The reference tests every star of the star-tree nodes that
+// overlap a pixel's quad bounding box (searchTree, MakeImageKernel.cu:27-96):
+// ~390 stars per level-8 leaf for a quad of ~1e-3 rad.  Any superset of the
+// stars inside the quad yields the same inside-star set, so pixels read
+// candidates from a 256 x 512 (theta, phi) binning of the stars instead:
+// one contiguous range per bin row, ~1 star per bin.  The exception is a
+// quad that crosses phi = 2 pi (picheck): there the reference retests
+// candidates with phi < 0.4 pi shifted by 2 pi, so which stars are
+// candidates matters, and those pixels walk the reference's tree.
+//
+// Work split.  Near the shadow a pixel can project onto a large part of the
+// sky with tens of thousands of candidates; one lane would serialise them and
+// stall its wavefront.  star_map_light takes pixels with at most kHeavyStars
+// candidates, one lane each, and queues the others for star_map_heavy, where
+// a 256-lane workgroup splits one pixel's candidates and reduces the taps in
+// a fixed order.
+//
+// Per tap the reference evaluates 100 * 10^(-0.4 (m - 2.5 log10 x)) with
+// x = frac * gaussian(d) and converts HSP -> RGB at brightness P * that.
+// For finite x > 0 this equals 100 * x * 10^(-0.4 m), and HSP -> RGB is
+// linear in P, so the colour is converted once per star and each tap is a
+// few multiplies; non-finite or non-positive x (which yield 0 or NaN in the
+// reference) take the reference's own expression.  The star light agrees
+// with the reference arithmetic to a few ulps up to summation order; frames
+// to within 1/255.
+constexpr int kHeavyStars = 512;
+
+struct StarPix {
+    float t[4], p[4];
+    float mn[2], mx[2];
+    int sgn;
+    bool picheck;
+    float frac, redshft, red;
 };
 
-// distortStarMap (MakeImageKernel.cu:206-326) for step = STEP, with the
-// star-tree search (MakeImageKernel.cu:27-96) visiting nodes in the
-// reference's list order.
+__device__ bool star_pixel(const float* thphi, const uint8_t* bh, int W, int i, int j, float offset, int lr,
+                           const float* area, const float* view, const Cam7& cam, StarPix& P) {
+    const size_t ij = (size_t)i * W + j;
+    if (bh[ij] != 0) return false;
+    const int ind = i * (W + 1) + j;
+    P.picheck = false;
+    corners(thphi, W, ind, P.t, P.p, P.picheck, offset);
+    const float* t = P.t;
+    const float* p = P.p;
+    P.mx[0] = fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3]));
+    P.mx[1] = fmaxf(fmaxf(p[0], p[1]), fmaxf(p[2], p[3]));
+    P.mn[0] = fminf(fminf(t[0], t[1]), fminf(t[2], t[3]));
+    P.mn[1] = fminf(fminf(p[0], p[1]), fminf(p[2], p[3]));
+    float orient = (t[1] - t[0]) * (p[1] + p[0]) + (t[2] - t[1]) * (p[2] + p[1]) + (t[3] - t[2]) * (p[3] + p[2]) +
+                   (t[0] - t[3]) * (p[0] + p[3]);
+    P.sgn = orient < 0 ? -1 : 1;
+    P.redshft = 1.f;
+    P.frac = 1.f;
+    if (lr) lensing_redshift(t, p, W, ind, cam, view, P.frac, P.redshft, area[ij]);
+    P.red = 4.f * f_log10(P.redshft);
+    return true;
+}
+
+__device__ __forceinline__ void node_range(const int32_t* tree, int node, int& b, int& e) {
+    b = (node != 0 && ((node + 1) & node) != 0) ? tree[node - 1] : 0;
+    e = tree[node];
+}
+
+// searchTree (MakeImageKernel.cu:27-96) with picheck = 0: visit(node) is
+// called for every listed node in the reference's list order.
+template <class F>
+__device__ void search_tree(const int32_t* tree, int treeLevel, const float* mn, const float* mx, F&& visit) {
+    float nodeStart[2] = {0.f, 0.f};
+    float nodeSize[2] = {BT_FPI, BT_FPI2};
+    int node = 0;
+    unsigned bitMask = 1u << treeLevel;
+    int level = 0;
+    int lvl = 0;
+    while (bitMask != 0) {
+        bitMask &= ~(1u << (treeLevel - level));
+        for (lvl = level + 1; lvl <= treeLevel; lvl++) {
+            int star_n = tree[node];
+            if (node != 0 && ((node + 1) & node) != 0) star_n -= tree[node - 1];
+            int tp = lvl & 1;
+            float x_overlap = fmaxf(0.f, fminf(mx[0], nodeStart[0] + nodeSize[0]) - fmaxf(mn[0], nodeStart[0]));
+            float y_overlap = fmaxf(0.f, fminf(mx[1], nodeStart[1] + nodeSize[1]) - fmaxf(mn[1], nodeStart[1]));
+            float overlapArea = x_overlap * y_overlap;
+            bool size = overlapArea / (nodeSize[0] * nodeSize[1]) > 0.8f;
+            nodeSize[tp] = nodeSize[tp] * .5f;
+            if (star_n == 0) {
+                node = node * 2 + 1;
+                break;
+            }
+            float check = nodeStart[tp] + nodeSize[tp];
+            bool lu = mn[tp] < check;
+            bool rd = mx[tp] >= check;
+            if (lvl == treeLevel || (rd && lu && size)) {
+                if (rd) visit(node * 2 + 2);
+                if (lu) visit(node * 2 + 1);
+                node = node * 2 + 1;
+                break;
+            } else {
+                node = node * 2 + 1;
+                if (rd) bitMask |= 1u << (treeLevel - lvl);
+                if (!lu) break;
+            }
+        }
+        level = treeLevel - (__ffs((int)bitMask)) + 1;
+        if (level >= 0) {
+            int diff = lvl - level;
+            for (int q = 0; q < diff; q++) {
+                int tp = (lvl - q) & 1;
+                if (!(node & 1)) nodeStart[tp] -= nodeSize[tp];
+                nodeSize[tp] = nodeSize[tp] * 2.f;
+                node = (node - 1) / 2;
+            }
+            node++;
+            int tp = level & 1;
+            nodeStart[tp] += nodeSize[tp];
+        }
+    }
+}
+
+__device__ __forceinline__ int bin_of(float v, float d, int n) { return min(max(cvt_i32(v / d), 0), n - 1); }
+
+// f(begin, end) for every candidate range of a pixel; ranges index
+// ranges_pos/ranges_mag (tree order for picheck pixels, bin order otherwise).
+template <class F>
+__device__ void for_ranges(const StarPix& P, const SceneK& s, F&& f) {
+    if (P.picheck) {
+        search_tree(s.tree, s.tree_level, P.mn, P.mx, [&](int node) {
+            int b, e;
+            node_range(s.tree, node, b, e);
+            f(b, e);
+        });
+    } else {
+        const int r0 = bin_of(P.mn[0], kBinDT, kBinT), r1 = bin_of(P.mx[0], kBinDT, kBinT);
+        const int p0 = bin_of(P.mn[1], kBinDP, kBinP), p1 = bin_of(P.mx[1], kBinDP, kBinP);
+        for (int r = r0; r <= r1; r++) f(s.bin_start[r * kBinP + p0], s.bin_start[r * kBinP + p1 + 1]);
+    }
+}
+
+// One candidate star of one pixel (the body of MakeImageKernel.cu:285-322).
 template <int STEP>
-__global__ __launch_bounds__(256) void star_map(const float* thphi, const uint8_t* bh, SceneK s, Cam7 cam,
-                                                int W, int H, float offset, int lr, const float* area,
-                                                const float* view, float* taps) {
+__device__ __forceinline__ void star_accumulate(const StarPix& P, const float* pos, const float* mag, int q,
+                                                float (&acc)[(2 * STEP + 1) * (2 * STEP + 1)][3]) {
+    constexpr int FW = 2 * STEP + 1;
+    const float2 sp2 = reinterpret_cast<const float2*>(pos)[q];
+    float st = sp2.x, sp = sp2.y;
+    bool in = star_in_polygon(P.t, P.p, st, sp, P.sgn);
+    if (P.picheck && !in && sp < BT_FPI2 * .2f) {
+        sp += BT_FPI2;
+        in = star_in_polygon(P.t, P.p, st, sp, P.sgn);
+    }
+    if (!in) return;
+    interpolate_star(P.t[0], P.t[1], P.t[2], P.t[3], P.p[0], P.p[1], P.p[2], P.p[3], st, sp, P.sgn);
+    const float2 mg = reinterpret_cast<const float2*>(mag)[q];
+    const float part = mg.x + P.red;
+    float temp = 46.f / P.redshft * ((1.f / ((0.92f * mg.y) + 1.7f)) + (1.f / ((0.92f * mg.y) + 0.62f))) - 10.f;
+    // clamp to the table's last entry (the reference's 1170 reads past it)
+    int index = max(0, min(cvt_i32(floorf(temp)), 390));
+    float hh, ss, pp;
+    rgb_to_hsp((float)kTempRGB[3 * index] / 255.f, (float)kTempRGB[3 * index + 1] / 255.f,
+               (float)kTempRGB[3 * index + 2] / 255.f, hh, ss, pp);
+    float r1, g1, b1;
+    hsp_to_rgb(hh, ss, pp, r1, g1, b1);
+    const float E = f_exp10(-.4f * part);
+    const float maxDistSq = (STEP + .5f) * (STEP + .5f);
+#pragma unroll
+    for (int u = 0; u < FW; u++)
+#pragma unroll
+        for (int v = 0; v < FW; v++) {
+            float dt = -STEP + u + .5f - st;
+            float dp = -STEP + v + .5f - sp;
+            float dist = dt * dt + dp * dp;
+            if (dist > maxDistSq) continue;
+            float x = P.frac * gaussian(dist);
+            float rr, gg, bb;
+            if (x > 0.f && x <= 3.4e38f && fabsf(E) <= 3.4e38f) {
+                float bright = 100.f * x * E;
+                rr = r1 * bright;
+                gg = g1 * bright;
+                bb = b1 * bright;
+            } else {
+                float appMag = part - 2.5f * f_log10(x);
+                float brightness = 100 * f_exp10(-.4f * appMag);
+                hsp_to_rgb(hh, ss, pp * brightness, rr, gg, bb);
+            }
+            rr *= 255.f;
+            gg *= 255.f;
+            bb *= 255.f;
+            acc[FW * u + v][0] += rr * rr;
+            acc[FW * u + v][1] += gg * gg;
+            acc[FW * u + v][2] += bb * bb;
+        }
+}
+
+template <int STEP>
+__global__ __launch_bounds__(256) void star_map_light(const float* thphi, const uint8_t* bh, SceneK s, Cam7 cam,
+                                                      int W, int H, float offset, int lr, const float* area,
+                                                      const float* view, float* taps, uint32_t* heavy,
+                                                      unsigned* nheavy) {
     constexpr int FW = 2 * STEP + 1;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y * blockDim.y + threadIdx.y;
@@ -382,117 +568,18 @@ __global__ __launch_bounds__(256) void star_map(const float* thphi, const uint8_
     float acc[FW * FW][3];
 #pragma unroll
     for (int u = 0; u < FW * FW; u++) acc[u][0] = acc[u][1] = acc[u][2] = 0.f;
-
-    if (bh[ij] == 0) {
-        float t[4], p[4];
-        bool picheck = false;
-        const int ind = i * (W + 1) + j;
-        corners(thphi, W, ind, t, p, picheck, offset);
-        const float mx[2] = {fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3])), fmaxf(fmaxf(p[0], p[1]), fmaxf(p[2], p[3]))};
-        const float mn[2] = {fminf(fminf(t[0], t[1]), fminf(t[2], t[3])), fminf(fminf(p[0], p[1]), fminf(p[2], p[3]))};
-        float orient = (t[1] - t[0]) * (p[1] + p[0]) + (t[2] - t[1]) * (p[2] + p[1]) + (t[3] - t[2]) * (p[3] + p[2]) +
-                       (t[0] - t[3]) * (p[0] + p[3]);
-        int sgn = orient < 0 ? -1 : 1;
-        float redshft = 1.f, frac = 1.f;
-        if (lr) lensing_redshift(t, p, W, ind, cam, view, frac, redshft, area[ij]);
-        float red = 4.f * f_log10(redshft);
-        const float maxDistSq = (STEP + .5f) * (STEP + .5f);
-        const int32_t* tree = s.tree;
-        const int treeLevel = s.tree_level;
-
-        auto visit = [&](int node) {
-            int startN = 0;
-            if (node != 0 && ((node + 1) & node) != 0) startN = tree[node - 1];
-            const int endN = tree[node];
-            for (int q = startN; q < endN; q++) {
-                const float2 sp2 = reinterpret_cast<const float2*>(s.star_pos)[q];
-                float st = sp2.x, sp = sp2.y;
-                bool in = star_in_polygon(t, p, st, sp, sgn);
-                if (picheck && !in && sp < BT_FPI2 * .2f) {
-                    sp += BT_FPI2;
-                    in = star_in_polygon(t, p, st, sp, sgn);
-                }
-                if (!in) continue;
-                interpolate_star(t[0], t[1], t[2], t[3], p[0], p[1], p[2], p[3], st, sp, sgn);
-                const float2 mg = reinterpret_cast<const float2*>(s.star_mag)[q];
-                float part = mg.x + red;
-                float temp = 46.f / redshft * ((1.f / ((0.92f * mg.y) + 1.7f)) + (1.f / ((0.92f * mg.y) + 0.62f))) - 10.f;
-                // clamp to the table's last entry (the reference's 1170 reads past it)
-                int index = max(0, min(cvt_i32(floorf(temp)), 390));
-                float hh, ss, pp;
-                rgb_to_hsp((float)kTempRGB[3 * index] / 255.f, (float)kTempRGB[3 * index + 1] / 255.f,
-                           (float)kTempRGB[3 * index + 2] / 255.f, hh, ss, pp);
-#pragma unroll
-                for (int u = 0; u < FW; u++)
-#pragma unroll
-                    for (int v = 0; v < FW; v++) {
-                        float dt = -STEP + u + .5f - st;
-                        float dp = -STEP + v + .5f - sp;
-                        float dist = dt * dt + dp * dp;
-                        if (dist > maxDistSq) continue;
-                        float appMag = part - 2.5f * f_log10(frac * gaussian(dist));
-                        float brightness = 100 * f_exp10(-.4f * appMag);
-                        float rr, gg, bb;
-                        hsp_to_rgb(hh, ss, pp * brightness, rr, gg, bb);
-                        rr *= 255.f;
-                        gg *= 255.f;
-                        bb *= 255.f;
-                        acc[FW * u + v][0] += rr * rr;
-                        acc[FW * u + v][1] += gg * gg;
-                        acc[FW * u + v][2] += bb * bb;
-                    }
-            }
-        };
-
-        // searchTree (MakeImageKernel.cu:27-96) with picheck = 0
-        float nodeStart[2] = {0.f, 0.f};
-        float nodeSize[2] = {BT_FPI, BT_FPI2};
-        int node = 0;
-        unsigned bitMask = 1u << treeLevel;
-        int level = 0;
-        int lvl = 0;
-        while (bitMask != 0) {
-            bitMask &= ~(1u << (treeLevel - level));
-            for (lvl = level + 1; lvl <= treeLevel; lvl++) {
-                int star_n = tree[node];
-                if (node != 0 && ((node + 1) & node) != 0) star_n -= tree[node - 1];
-                int tp = lvl & 1;
-                float x_overlap = fmaxf(0.f, fminf(mx[0], nodeStart[0] + nodeSize[0]) - fmaxf(mn[0], nodeStart[0]));
-                float y_overlap = fmaxf(0.f, fminf(mx[1], nodeStart[1] + nodeSize[1]) - fmaxf(mn[1], nodeStart[1]));
-                float overlapArea = x_overlap * y_overlap;
-                bool size = overlapArea / (nodeSize[0] * nodeSize[1]) > 0.8f;
-                nodeSize[tp] = nodeSize[tp] * .5f;
-                if (star_n == 0) {
-                    node = node * 2 + 1;
-                    break;
-                }
-                float check = nodeStart[tp] + nodeSize[tp];
-                bool lu = mn[tp] < check;
-                bool rd = mx[tp] >= check;
-                if (lvl == treeLevel || (rd && lu && size)) {
-                    if (rd) visit(node * 2 + 2);
-                    if (lu) visit(node * 2 + 1);
-                    node = node * 2 + 1;
-                    break;
-                } else {
-                    node = node * 2 + 1;
-                    if (rd) bitMask |= 1u << (treeLevel - lvl);
-                    if (!lu) break;
-                }
-            }
-            level = treeLevel - (__ffs((int)bitMask)) + 1;
-            if (level >= 0) {
-                int diff = lvl - level;
-                for (int q = 0; q < diff; q++) {
-                    int tp = (lvl - q) & 1;
-                    if (!(node & 1)) nodeStart[tp] -= nodeSize[tp];
-                    nodeSize[tp] = nodeSize[tp] * 2.f;
-                    node = (node - 1) / 2;
-                }
-                node++;
-                int tp = level & 1;
-                nodeStart[tp] += nodeSize[tp];
-            }
+    StarPix P;
+    if (star_pixel(thphi, bh, W, i, j, offset, lr, area, view, cam, P)) {
+        int count = 0;
+        for_ranges(P, s, [&](int b, int e) { count += e - b; });
+        if (count > kHeavyStars) {
+            heavy[atomicAdd(nheavy, 1u)] = (uint32_t)ij;
+        } else if (count > 0) {
+            const float* pos = P.picheck ? s.star_pos : s.bin_pos;
+            const float* mag = P.picheck ? s.star_mag : s.bin_mag;
+            for_ranges(P, s, [&](int b, int e) {
+                for (int q = b; q < e; q++) star_accumulate<STEP>(P, pos, mag, q, acc);
+            });
         }
     }
     // tap-major layout: taps[(tap * 3 + c) * NP + ij]
@@ -501,6 +588,74 @@ __global__ __launch_bounds__(256) void star_map(const float* thphi, const uint8_
         taps[(size_t)(u * 3 + 0) * NP + ij] = acc[u][0];
         taps[(size_t)(u * 3 + 1) * NP + ij] = acc[u][1];
         taps[(size_t)(u * 3 + 2) * NP + ij] = acc[u][2];
+    }
+}
+
+// One workgroup per queued pixel: lanes stride over the pixel's candidates,
+// partial taps are reduced in a fixed order.
+template <int STEP>
+__global__ __launch_bounds__(256) void star_map_heavy(const float* thphi, const uint8_t* bh, SceneK s, Cam7 cam,
+                                                      int W, int H, float offset, int lr, const float* area,
+                                                      const float* view, float* taps, const uint32_t* heavy,
+                                                      const unsigned* nheavy) {
+    constexpr int FW = 2 * STEP + 1;
+    constexpr int NT = FW * FW * 3;
+    __shared__ int s_b[2048];
+    __shared__ int s_pref[2049];
+    __shared__ int s_n;
+    __shared__ float s_red[4][NT];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const size_t NP = (size_t)W * H;
+    const unsigned nh = *nheavy;
+    for (unsigned hidx = blockIdx.x; hidx < nh; hidx += gridDim.x) {
+        const uint32_t ij = heavy[hidx];
+        const int i = (int)(ij / (uint32_t)W), j = (int)(ij % (uint32_t)W);
+        StarPix P;
+        star_pixel(thphi, bh, W, i, j, offset, lr, area, view, cam, P);
+        if (tid == 0) {
+            int n = 0, acc_n = 0;
+            for_ranges(P, s, [&](int b, int e) {
+                if (n < 2048 && e > b) {
+                    s_b[n] = b;
+                    s_pref[n] = acc_n;
+                    acc_n += e - b;
+                    n++;
+                }
+            });
+            s_pref[n] = acc_n;
+            s_n = n;
+        }
+        __syncthreads();
+        const int n = s_n;
+        const int total = s_pref[n];
+        const float* pos = P.picheck ? s.star_pos : s.bin_pos;
+        const float* mag = P.picheck ? s.star_mag : s.bin_mag;
+        float acc[FW * FW][3];
+#pragma unroll
+        for (int u = 0; u < FW * FW; u++) acc[u][0] = acc[u][1] = acc[u][2] = 0.f;
+        int cur = 0;
+        for (int k = tid; k < total; k += blockDim.x) {
+            while (cur + 1 < n && s_pref[cur + 1] <= k) cur++;
+            star_accumulate<STEP>(P, pos, mag, s_b[cur] + (k - s_pref[cur]), acc);
+        }
+        // xor butterfly within each wave, then the 4 waves in order
+#pragma unroll
+        for (int u = 0; u < FW * FW; u++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float v = acc[u][c];
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                if ((tid & 63) == 0) s_red[wave][3 * u + c] = v;
+            }
+        __syncthreads();
+        if (tid < NT) {
+            int u = tid / 3, c = tid % 3;
+            float v = ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
+            taps[(size_t)(u * 3 + c) * NP + ij] = v;
+        }
+        __syncthreads();
     }
 }
 
@@ -743,6 +898,10 @@ __global__ void grid_to_f32(const double* cel, int GN, int GM, int sym, int GN1,
     }
 }
 
+__global__ void zero_u32_r(unsigned* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
 dim3 grid2d(int W, int H) { return dim3((unsigned)((W + 15) / 16), (unsigned)((H + 15) / 16)); }
 
 }  // namespace
@@ -826,15 +985,38 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         const int fw = 2 * step + 1;
         if ((rc = c->buffer("r_taps", sizeof(float) * 3 * fw * fw * NP, &p))) return rc;
         float* taps = (float*)p;
+        if ((rc = c->buffer("r_heavy", sizeof(uint32_t) * NP + 64, &p))) return rc;
+        uint32_t* heavy = (uint32_t*)p;
+        if ((rc = c->buffer("r_heavy_n", 64, &p))) return rc;
+        unsigned* nheavy = (unsigned*)p;
+        zero_u32_r<<<1, 64, 0, c->stream>>>(nheavy, 1);
+        const unsigned nblk = (unsigned)c->num_cus * 8;
         {
-            KTimer t(c, "star_map");
+            KTimer t(c, "star_light");
             if (step == 1)
-                star_map<1><<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view, taps);
+                star_map_light<1><<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area,
+                                                                      v.view, taps, heavy, nheavy);
             else if (step == 2)
-                star_map<2><<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view, taps);
+                star_map_light<2><<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area,
+                                                                      v.view, taps, heavy, nheavy);
             else
-                star_map<3><<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view, taps);
+                star_map_light<3><<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area,
+                                                                      v.view, taps, heavy, nheavy);
         }
+        {
+            KTimer t(c, "star_heavy");
+            if (step == 1) {
+                star_map_heavy<1><<<nblk, 256, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view,
+                                                               taps, heavy, nheavy);
+            } else if (step == 2) {
+                star_map_heavy<2><<<nblk, 256, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view,
+                                                               taps, heavy, nheavy);
+            } else {
+                star_map_heavy<3><<<nblk, 256, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view,
+                                                               taps, heavy, nheavy);
+            }
+        }
+        BT_HIP(hipGetLastError());
         {
             KTimer t(c, "star_sum");
             float* ostar = dbg ? dbg->star : nullptr;

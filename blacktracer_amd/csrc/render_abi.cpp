@@ -111,6 +111,35 @@ int star_tree(const float* stars4, int S, int tree_level, int32_t* tree, float* 
     return BT_OK;
 }
 
+static int bin_host(float v, float d, int n) {
+    float x = v / d;
+    int b = (x == x) ? (x >= 2147483648.0f ? 2147483647 : (x <= -2147483648.0f ? (-2147483647 - 1) : (int)x)) : 0;
+    return b < 0 ? 0 : (b > n - 1 ? n - 1 : b);
+}
+
+// Counting sort of the stars into kBinT x kBinP (theta, phi) cells; stable,
+// so stars keep their tree order inside a cell.  The cell of a value is
+// computed exactly as the device computes the cell range of a bounding box.
+void star_bins(const float* pos, const float* mag, int S, int32_t* bin_start, float* bin_pos, float* bin_mag) {
+    const int NB = kBinT * kBinP;
+    std::vector<int32_t> cell(S > 0 ? S : 1);
+    for (int c = 0; c <= NB; c++) bin_start[c] = 0;
+    for (int q = 0; q < S; q++) {
+        int c = bin_host(pos[2 * q], kBinDT, kBinT) * kBinP + bin_host(pos[2 * q + 1], kBinDP, kBinP);
+        cell[q] = c;
+        bin_start[c + 1]++;
+    }
+    for (int c = 0; c < NB; c++) bin_start[c + 1] += bin_start[c];
+    std::vector<int32_t> fill(bin_start, bin_start + NB);
+    for (int q = 0; q < S; q++) {
+        int d = fill[cell[q]]++;
+        bin_pos[2 * d] = pos[2 * q];
+        bin_pos[2 * d + 1] = pos[2 * q + 1];
+        bin_mag[2 * d] = mag[2 * q];
+        bin_mag[2 * d + 1] = mag[2 * q + 1];
+    }
+}
+
 static void cam7_of(const PoseK& P, float* cam7) {
     // Camera::getParamArray, Camera.h:67-78
     cam7[0] = (float)P.speed;
@@ -132,9 +161,14 @@ struct bt_scene {
     float* pos = nullptr;
     float* mag = nullptr;
     float* table = nullptr;
+    int32_t* bin_start = nullptr;
+    float* bin_pos = nullptr;
+    float* bin_mag = nullptr;
     int tree_level = 0, S = 0, rows = 0, cols = 0;
     size_t bytes = 0;
-    SceneK k() const { return SceneK{tree, tree_level, pos, mag, S, table, rows, cols}; }
+    SceneK k() const {
+        return SceneK{tree, tree_level, pos, mag, S, table, rows, cols, bin_start, bin_pos, bin_mag};
+    }
 };
 
 struct bt_view {
@@ -149,8 +183,12 @@ static void scene_release(bt_scene* s) {
     if (s->pos) hipFree(s->pos);
     if (s->mag) hipFree(s->mag);
     if (s->table) hipFree(s->table);
+    if (s->bin_start) hipFree(s->bin_start);
+    if (s->bin_pos) hipFree(s->bin_pos);
+    if (s->bin_mag) hipFree(s->bin_mag);
     s->tree = nullptr;
-    s->pos = s->mag = s->table = nullptr;
+    s->bin_start = nullptr;
+    s->pos = s->mag = s->table = s->bin_pos = s->bin_mag = nullptr;
 }
 
 // Uploads an already built star tree and builds the summed sky table on the device.
@@ -166,11 +204,21 @@ static int scene_upload(Context& c, bt_scene* s, const int32_t* tree, int tree_l
     BT_HIP(hipMalloc((void**)&s->pos, sizeof(float) * 2 * (S > 0 ? S : 1)));
     BT_HIP(hipMalloc((void**)&s->mag, sizeof(float) * 2 * (S > 0 ? S : 1)));
     BT_HIP(hipMalloc((void**)&s->table, sizeof(float) * 4 * (size_t)rows * cols));
-    s->bytes = sizeof(int32_t) * tsz + sizeof(float) * 4 * (size_t)S + sizeof(float) * 4 * (size_t)rows * cols;
-    BT_HIP(hipMemcpyAsync(s->tree, tree, sizeof(int32_t) * tsz, hipMemcpyHostToDevice, c.stream));
+    const size_t nb = (size_t)kBinT * kBinP + 1;
+    BT_HIP(hipMalloc((void**)&s->bin_start, sizeof(int32_t) * nb));
+    BT_HIP(hipMalloc((void**)&s->bin_pos, sizeof(float) * 2 * (S > 0 ? S : 1)));
+    BT_HIP(hipMalloc((void**)&s->bin_mag, sizeof(float) * 2 * (S > 0 ? S : 1)));
+    s->bytes = sizeof(int32_t) * (tsz + nb) + sizeof(float) * 8 * (size_t)S + sizeof(float) * 4 * (size_t)rows * cols;
+    std::vector<int32_t> bstart(nb);
+    std::vector<float> bpos(2 * (size_t)(S > 0 ? S : 1)), bmag(2 * (size_t)(S > 0 ? S : 1));
+    star_bins(pos, mag, S, bstart.data(), bpos.data(), bmag.data());
+    BT_HIP(hipMemcpy(s->tree, tree, sizeof(int32_t) * tsz, hipMemcpyHostToDevice));
+    BT_HIP(hipMemcpy(s->bin_start, bstart.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
     if (S > 0) {
-        BT_HIP(hipMemcpyAsync(s->pos, pos, sizeof(float) * 2 * S, hipMemcpyHostToDevice, c.stream));
-        BT_HIP(hipMemcpyAsync(s->mag, mag, sizeof(float) * 2 * S, hipMemcpyHostToDevice, c.stream));
+        BT_HIP(hipMemcpy(s->pos, pos, sizeof(float) * 2 * S, hipMemcpyHostToDevice));
+        BT_HIP(hipMemcpy(s->mag, mag, sizeof(float) * 2 * S, hipMemcpyHostToDevice));
+        BT_HIP(hipMemcpy(s->bin_pos, bpos.data(), sizeof(float) * 2 * S, hipMemcpyHostToDevice));
+        BT_HIP(hipMemcpy(s->bin_mag, bmag.data(), sizeof(float) * 2 * S, hipMemcpyHostToDevice));
     }
     void* dsky;
     int rc;
@@ -207,8 +255,9 @@ int bt_scene_create(bt_context* h, const float* stars4, int star_count, int tree
                     int sky_rows, int sky_cols, bt_scene** out) {
     if (!h || !out) return fail(BT_ERR_INVALID, "bt_scene_create: null argument");
     *out = nullptr;
-    if (tree_level < 1 || tree_level > 20 || star_count < 0)
-        return fail(BT_ERR_INVALID, "bt_scene_create: tree_level in [1, 20], star_count >= 0");
+    // the device star search lists at most 2^(L+1) - 1 < 2048 nodes
+    if (tree_level < 1 || tree_level > 10 || star_count < 0)
+        return fail(BT_ERR_INVALID, "bt_scene_create: tree_level in [1, 10], star_count >= 0");
     Context& c = h->impl;
     std::lock_guard<std::recursive_mutex> lk(c.mu);
     BT_HIP(hipSetDevice(c.device));
@@ -380,7 +429,7 @@ int bt_make_image(bt_context* h, const float* stars, const int32_t* star_tree_in
     (void)view_angle;
     if (!h) return fail(BT_ERR_INVALID, "bt_make_image: context is null");
     if (G != 1) return fail(BT_ERR_INVALID, "bt_make_image: only G == 1 (one grid) is supported");
-    if (M <= 0 || N <= 0 || GM <= 0 || GN <= 0 || tree_level < 1 || tree_level > 20 || star_size < 0)
+    if (M <= 0 || N <= 0 || GM <= 0 || GN <= 0 || tree_level < 1 || tree_level > 10 || star_size < 0)
         return fail(BT_ERR_INVALID, "bt_make_image: bad sizes");
     if (!cam_param || !viewthing || !offset_tables || !hash_tables || !hash_pos_tag || !table_sizes ||
         !out_bgra || !star_tree_in || (star_size > 0 && (!stars || !magnitude)))

@@ -56,6 +56,11 @@ int  bt_context_sync(bt_context* ctx);
 int  bt_timing_enable(bt_context* ctx, int enable);
 int  bt_timer_get(bt_context* ctx, const char* kernel, int64_t* launches, double* total_ms);
 int  bt_timer_reset(bt_context* ctx);
+/* Device buffers owned by the caller (frame outputs of bt_render_frame,
+ * inputs of bt_trace_rays_device); freed with bt_device_free. */
+int  bt_device_alloc(bt_context* ctx, size_t bytes, void** out);
+void bt_device_free(bt_context* ctx, void* ptr);
+int  bt_memcpy(bt_context* ctx, void* dst, const void* src, size_t bytes, int to_device);
 
 /* ---- pose: Camera(theta, phi, r, br, btheta, bphi) + BlackHole(a) ---- */
 typedef struct {

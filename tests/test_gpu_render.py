@@ -1,8 +1,10 @@
 """Parity of the device render pass with the CPU oracle.
 
 Bar: per-corner celestial angles, block gaps, shadow mask and smoothed solid
-angle bit-exact; summed star light within 1e-5 relative; frames within 1/255
-per channel (max |diff| <= 1) with at most 0.1% of pixels differing at all.
+angle bit-exact; summed star light within 1e-4 relative (the device factors
+the per-tap brightness and splits heavy pixels across a workgroup, which
+reassociates the sums); frames within 1/255 per channel (max |diff| <= 1)
+with at most 0.1% of pixels differing at all.
 """
 import math
 
@@ -70,7 +72,7 @@ def test_frame_matches_oracle(ctx, world, flags):
     nb = st["bh"] == 0
     assert np.array_equal(st["area"][nb], ost["area"][nb])
     if f.star:
-        assert np.allclose(st["star"], ost["star"], rtol=1e-5, atol=1e-3, equal_nan=True)
+        assert np.allclose(st["star"], ost["star"], rtol=1e-4, atol=1e-3, equal_nan=True)
     compare_frames(img, oimg)
     assert img[..., 3].min() == 255
 
