@@ -44,7 +44,7 @@ PEAK_HBM_GBS = 8000.0
 
 POSE = dict(a=0.999, theta=math.pi / 2, phi=0.0, r=5.0)
 W, H = 1920, 1080
-RENDER_KERNELS = ("pix_interp", "bh_area", "smooth", "star_map", "star_sum", "env_map", "composite")
+RENDER_KERNELS = ("pix_interp", "bh_area", "smooth", "star_light", "star_heavy", "star_sum", "env_map", "composite")
 
 
 def parse():
