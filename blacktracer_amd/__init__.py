@@ -207,6 +207,10 @@ class Grid:
         check(L.bt_grid_build(self.ctx.handle, C.byref(camera.pose(black_hole.a)),
                               C.byref(self.params), max_level, start_level, 1 if angle else 0,
                               C.byref(gp), C.byref(h)))
+        self._adopt(h)
+
+    def _adopt(self, h):
+        L = _lib.load()
         self.handle = h
         vals = [C.c_int() for _ in range(5)]
         rays = C.c_int64()
@@ -214,6 +218,35 @@ class Grid:
         self.N, self.M, self.MAXLEVEL, self.STARTLVL, self.equafactor = [v.value for v in vals]
         self.rays_traced = rays.value
         self._cel = self._steps = self._levels = None
+
+    @classmethod
+    def build_batch(cls, cameras, black_holes, max_levels, start_levels, angle=True, params=None,
+                    refine_thresh=0.015, fix_tvertices=True, ctx=None):
+        """Several grids built together (bt_grid_build_batch): returns
+        (list of Grid, batch stats dict)."""
+        ctx = ctx or default_context()
+        L = _lib.load()
+        n = len(cameras)
+        gp = BtGridParams()
+        L.bt_grid_params_default(C.byref(gp))
+        gp.refine_thresh = refine_thresh
+        gp.fix_tvertices = 1 if fix_tvertices else 0
+        params = params or IntegParams()
+        poses = (BtPose * n)(*[cam.pose(bh.a) for cam, bh in zip(cameras, black_holes)])
+        mls = (C.c_int * n)(*max_levels)
+        sls = (C.c_int * n)(*start_levels)
+        hs = (C.c_void_p * n)()
+        st = BtTraceStats()
+        check(L.bt_grid_build_batch(ctx.handle, n, poses, C.byref(params), mls, sls, 1 if angle else 0,
+                                    C.byref(gp), hs, C.byref(st)))
+        grids = []
+        for k in range(n):
+            g = cls.__new__(cls)
+            g.ctx = ctx
+            g.params = params
+            g._adopt(C.c_void_p(hs[k]))
+            grids.append(g)
+        return grids, _stats_dict(st)
 
     def stats(self):
         s = BtTraceStats()

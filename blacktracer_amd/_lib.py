@@ -81,6 +81,10 @@ SIGNATURES = {
     "bt_grid_params_default": (None, [C.POINTER(BtGridParams)]),
     "bt_grid_build": (C.c_int, [C.c_void_p, C.POINTER(BtPose), C.POINTER(BtIntegParams), C.c_int,
                                 C.c_int, C.c_int, C.POINTER(BtGridParams), C.POINTER(C.c_void_p)]),
+    "bt_grid_build_batch": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(BtPose), C.POINTER(BtIntegParams),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
+                                      C.POINTER(BtGridParams), C.POINTER(C.c_void_p),
+                                      C.POINTER(BtTraceStats)]),
     "bt_grid_info": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int)] * 5 + [I64P]),
     "bt_grid_stats": (C.c_int, [C.c_void_p, C.POINTER(BtTraceStats)]),
     "bt_grid_export": (C.c_int, [C.c_void_p, DP, I32P, I8P]),

@@ -17,6 +17,11 @@ struct GridK {
 int grid_alloc(Context* c, bt_grid* g);
 int grid_build(Context* c, bt_grid* g, const bt_grid_params* gp);
 int grid_fix_tvertices(Context* c, const GridK& G);
+// Builds all grids (allocated, same integration parameters) together; the
+// rays of each level of every grid share one multi-pose launch.  h_stats
+// (nullable) receives the batch's counters.
+int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_params* gp,
+                     StatsK* h_stats);
 void grid_release(bt_grid* g);
 
 }  // namespace bt

@@ -45,6 +45,7 @@ struct PoseK {
     double cos2S;      // cos(thetaS)*cos(thetaS)                      (Grid.h:547)
     double sin2S;      // sin(thetaS)*sin(thetaS)
     double b1, b2;     // _b0(_r2), _b0(_r1)                           (Metric.h:164-165)
+    double r_ph1, r_ph2;  // prograde / retrograde equatorial photon-orbit radii (Metric.h:146-147)
     double r_cap;      // capture radius (0: off)
     double gr;         // golden ratio (sqrt(5)+1)/2                   (Metric.h:111)
     int    a_zero;     // checkRup returns false when a == 0           (Metric.h:128)
@@ -83,6 +84,47 @@ struct RayIO {
     int32_t* out_step;
 };
 
+// Where the rays of one launch come from.  SingleSrc: one pose, one RayIO
+// (kernel arguments).  MultiSrc: a batch of segments, each with its own
+// pose and RayIO; global ray k belongs to the segment s with
+// start[s] <= k < start[s+1] and is ray k - start[s] of that segment.
+struct SingleSrc {
+    static constexpr bool kMulti = false;
+    PoseK P;
+    RayIO io;
+    __device__ __forceinline__ int seg(int64_t) const { return 0; }
+    __device__ __forceinline__ const PoseK& pose(int) const { return P; }
+    __device__ __forceinline__ const RayIO& rio(int) const { return io; }
+    __device__ __forceinline__ int64_t local(int, int64_t k) const { return k; }
+};
+
+struct MultiSrc {
+    static constexpr bool kMulti = true;
+    const PoseK* poses;
+    const RayIO* ios;
+    const int64_t* start;   // nseg + 1 offsets
+    int nseg;
+    __device__ __forceinline__ int seg(int64_t k) const {
+        int lo = 0, hi = nseg - 1;   // largest s with start[s] <= k
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (start[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    }
+    __device__ __forceinline__ const PoseK& pose(int s) const { return poses[s]; }
+    __device__ __forceinline__ const RayIO& rio(int s) const { return ios[s]; }
+    __device__ __forceinline__ int64_t local(int s, int64_t k) const { return k - start[s]; }
+};
+
+// Host description of a multi-pose batch (one segment per pose).
+struct BatchHost {
+    std::vector<PoseK> poses;
+    std::vector<RayIO> ios;
+    std::vector<int64_t> counts;
+};
+
 // ---------------------------------------------------------------- context
 struct PendingTimer {
     std::string name;
@@ -94,8 +136,16 @@ struct Buffer {
     size_t bytes = 0;
 };
 
+// Launch-shape knobs of the persistent integrators.
+struct Tune {
+    unsigned chunk = 128;   // queue positions a wave reserves per atomic (>= 64)
+    unsigned block = 256;   // threads per workgroup (64, 128 or 256)
+    int waves = 0;          // resident waves per SIMD (0: as many as registers allow)
+};
+
 struct Context {
     int device = 0;
+    Tune tune;
     int num_cus = 256;
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
@@ -125,12 +175,52 @@ struct KTimer {
     }
 };
 
+// Work queue of escaping rays: FP64 constants of motion indexed by ray,
+// and a queue of ray indices with its length/head counters.
+struct QueueK {
+    double* pr;
+    double* pt;
+    double* b;
+    double* q;
+    uint32_t* slots;    // queue of ray indices
+    unsigned* counters; // [0] = queue length, [1] = head
+};
+
+// Per-ray state handed from one integrator kernel to another (indexed by
+// ray): y holds six arrays of `stride` doubles (r, theta, phi, pR, pTheta, z).
+struct ContK {
+    double* y;
+    float* h;
+    int32_t* nstp;
+    int64_t stride;
+};
+
 // ---------------------------------------------------------------- launchers
 // kerr_f64.hip: the plain FP64 path (Metric.h / Grid.h semantics).
 // Traces n rays described by io; d_stats accumulates counters (device).
 int trace_f64(Context* c, const PoseK& pose, const ParamsK& prm, int64_t n, const RayIO& io,
               StatsK* d_stats);
-// kerr_mixed.hip: FP32 bulk with FP64 set-up, fallback and accumulation.
+// Building blocks shared by the FP64 and mixed paths, for one pose
+// (SingleSrc) or a multi-pose batch (MultiSrc); ray indices are global.
+// queue_alloc: per-ray arrays for n rays (context buffers).  trace_setup:
+// zeroes the counters, computes initial conditions and the capture test;
+// escaping rays go to Q, or, when Q64 is given, near-critical rays go to
+// Q64 instead.  trace_f64_queue: traces every ray queued in Q in FP64 from
+// its initial state (at most n_max rays); with cont given (warm-up mode) a
+// ray leaves after its first attempt with errmax > ERRCON and its state
+// goes to cont (cont.nstp = -1 for a ray that ended during the warm-up).
+int queue_alloc(Context* c, int64_t n, QueueK* Q);
+template <class Src>
+int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const QueueK& Q,
+                const QueueK* Q64, StatsK* d_stats);
+template <class Src>
+int trace_f64_queue(Context* c, const Src& S, const ParamsK& prm, int64_t n_max, const QueueK& Q,
+                    StatsK* d_stats, const char* timer, const ContK* cont);
+template <class Src>
+int trace_f64_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, StatsK* d_stats);
+// kerr_mixed.hip: FP32 bulk with FP64 set-up, warm-up, fallback and accumulation.
+template <class Src>
+int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, StatsK* d_stats);
 int trace_mixed(Context* c, const PoseK& pose, const ParamsK& prm, int64_t n, const RayIO& io,
                 StatsK* d_stats);
 
@@ -139,6 +229,11 @@ inline int trace(Context* c, const PoseK& pose, const ParamsK& prm, int64_t n, c
     return prm.precision == BT_PREC_MIXED ? trace_mixed(c, pose, prm, n, io, d_stats)
                                           : trace_f64(c, pose, prm, n, io, d_stats);
 }
+
+// Traces every segment of a multi-pose batch in one set of launches
+// (kerr_mixed.hip): the poses and RayIOs go to device memory, the rays of
+// all segments share one work queue.
+int trace_batch(Context* c, const BatchHost& b, const ParamsK& prm, StatsK* d_stats);
 
 void stats_to_abi(const StatsK& s, bt_trace_stats* out);
 

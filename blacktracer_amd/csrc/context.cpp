@@ -154,6 +154,8 @@ int make_pose(const bt_pose* in, PoseK* P) {
     double r2 = 2. * (1. + std::cos(2. * std::acos(a) / 3.));
     P->b1 = h_b0(a, r2);
     P->b2 = h_b0(a, r1);
+    P->r_ph1 = r1;
+    P->r_ph2 = r2;
     P->gr = (std::sqrt(5.0) + 1.) / 2.;
     P->a_zero = (a == 0) ? 1 : 0;
     P->r_cap = 0;

@@ -439,6 +439,160 @@ int grid_build(Context* c, bt_grid* g, const bt_grid_params* gp) {
     return BT_OK;
 }
 
+// The same build for several grids at once.  The grids advance level by
+// level in lockstep; at each step the rays every grid needs (poles, start
+// lattice, then each refinement level) go into one multi-pose batch, traced
+// by one set of persistent launches with a shared work queue.  Refinement,
+// final levels and T-vertex repair stay per grid (they are cheap and
+// per-grid ordered).  All grids share the integration parameters of gs[0].
+int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_params* gp,
+                     StatsK* h_stats) {
+    const int G = (int)gs.size();
+    if (G == 0) return BT_OK;
+    if (G > 128) return fail(BT_ERR_INVALID, "grid batch: at most 128 grids");
+    const double thresh = gp ? gp->refine_thresh : 0.015;
+    const int fix = gp ? gp->fix_tvertices : 1;
+    struct Per {
+        uint32_t* blocks;
+        uint32_t* next;
+        uint32_t* keys;
+        unsigned nb;
+        int level;
+        bool active;
+        RayIO io;
+    };
+    std::vector<Per> per(G);
+    int rc;
+    void* p;
+    if ((rc = c->buffer("gbatch_ctr", sizeof(unsigned) * 2 * 128, &p))) return rc;
+    unsigned* ctr = (unsigned*)p;
+    if ((rc = c->buffer("gbatch_stats", sizeof(StatsK), &p))) return rc;
+    StatsK* d_stats = (StatsK*)p;
+    BT_HIP(hipMemsetAsync(d_stats, 0, sizeof(StatsK), c->stream));
+    const ParamsK& prm = gs[0]->prm;
+
+    for (int k = 0; k < G; k++) {
+        bt_grid* g = gs[k];
+        GridK& Gk = g->k;
+        const int64_t NM = (int64_t)g->N * g->M;
+        std::string sfx = "_" + std::to_string(k);
+        if ((rc = c->buffer("gbatch_a" + sfx, sizeof(uint32_t) * (NM + 64), &p))) return rc;
+        per[k].blocks = (uint32_t*)p;
+        if ((rc = c->buffer("gbatch_b" + sfx, sizeof(uint32_t) * (NM + 64), &p))) return rc;
+        per[k].next = (uint32_t*)p;
+        if ((rc = c->buffer("gbatch_t" + sfx, sizeof(uint32_t) * (NM + 64), &p))) return rc;
+        per[k].keys = (uint32_t*)p;
+        const int64_t nwords = (NM + 31) / 32;
+        init_grid<<<(unsigned)std::min<int64_t>(blocks_for(NM, 256), 4096), 256, 0, c->stream>>>(Gk, NM, nwords);
+        BT_HIP(hipGetLastError());
+        RayIO& io = per[k].io;
+        std::memset(&io, 0, sizeof(io));
+        io.N = g->N;
+        io.M = g->M;
+        io.eq = g->eq;
+        io.out_th = Gk.cel;
+        io.out_ph = Gk.cel + 1;
+        io.ostride = 2;
+        io.out_step = Gk.steps;
+        io.keys = per[k].keys;
+        g->rays = 0;
+        pole_keys<<<1, 64, 0, c->stream>>>(per[k].keys, (uint32_t)(g->N - 1) * (uint32_t)g->M);
+        BT_HIP(hipGetLastError());
+    }
+    auto run_batch = [&](const std::vector<int64_t>& counts) -> int {
+        BatchHost b;
+        for (int k = 0; k < G; k++) {
+            if (counts[k] <= 0) continue;
+            b.poses.push_back(gs[k]->pose);
+            b.ios.push_back(per[k].io);
+            b.counts.push_back(counts[k]);
+            gs[k]->rays += counts[k];
+        }
+        return trace_batch(c, b, prm, d_stats);
+    };
+    // raytrace(): the pole rays of every grid (Grid.h:295-301)
+    std::vector<int64_t> counts(G);
+    for (int k = 0; k < G; k++) counts[k] = 1 + gs[k]->eq;
+    if ((rc = run_batch(counts))) return rc;
+    // integrateFirst(gap) of every grid
+    for (int k = 0; k < G; k++) {
+        bt_grid* g = gs[k];
+        const int N = g->N, M = g->M, eq = g->eq;
+        const int gap0 = 1 << (g->max_level - g->start_level);
+        const unsigned npole = (unsigned)(M / gap0);
+        pole_rows<<<blocks_for(npole, 256), 256, 0, c->stream>>>(g->k, gap0, per[k].blocks);
+        BT_HIP(hipGetLastError());
+        unsigned nrows = 0;
+        for (int i = gap0; i < N - eq; i += gap0) nrows++;
+        const unsigned ncols = (unsigned)(M / gap0);
+        const int64_t nlat = (int64_t)nrows * ncols;
+        const bool uniform = (g->max_level == g->start_level);
+        start_lattice<<<blocks_for(nlat, 256), 256, 0, c->stream>>>(
+            g->k, gap0, nrows, ncols, per[k].keys, per[k].blocks + npole, uniform ? g->start_level : -1);
+        BT_HIP(hipGetLastError());
+        counts[k] = nlat;
+        per[k].nb = npole;
+        if (!uniform) {
+            unsigned lat_blocks = nrows * ncols;
+            if (!eq) lat_blocks -= ncols;   // row N-1 is traced but is no block
+            per[k].nb += lat_blocks;
+        }
+        per[k].level = g->start_level;
+        per[k].active = true;
+    }
+    if ((rc = run_batch(counts))) return rc;
+    // adaptiveBlockIntegration(startLevel), all grids one level per round
+    std::vector<unsigned> h_ctr(2 * G);
+    while (true) {
+        bool any = false;
+        for (int k = 0; k < G; k++) {
+            Per& q = per[k];
+            if (!q.active) continue;
+            if (q.level >= gs[k]->max_level || q.nb == 0) {
+                // adaptiveBlockIntegration's exits (Grid.h:497-505)
+                if (q.nb > 0 && q.level >= gs[k]->max_level) {
+                    final_level<<<blocks_for(q.nb, 256), 256, 0, c->stream>>>(gs[k]->k, q.blocks, q.nb, q.level);
+                    BT_HIP(hipGetLastError());
+                }
+                q.active = false;
+                continue;
+            }
+            any = true;
+        }
+        if (!any) break;
+        zero_u32<<<1, 256, 0, c->stream>>>(ctr, 2 * G);
+        BT_HIP(hipGetLastError());
+        for (int k = 0; k < G; k++) {
+            Per& q = per[k];
+            if (!q.active) continue;
+            KTimer t(c, "refine_level");
+            refine_level<<<blocks_for(q.nb, 256), 256, 0, c->stream>>>(gs[k]->k, q.blocks, q.nb, q.level, thresh,
+                                                                        q.next, q.keys, ctr + 2 * k);
+            BT_HIP(hipGetLastError());
+        }
+        BT_HIP(hipMemcpyAsync(h_ctr.data(), ctr, sizeof(unsigned) * 2 * G, hipMemcpyDeviceToHost, c->stream));
+        BT_HIP(hipStreamSynchronize(c->stream));
+        for (int k = 0; k < G; k++) counts[k] = per[k].active ? (int64_t)h_ctr[2 * k] : 0;
+        if ((rc = run_batch(counts))) return rc;
+        for (int k = 0; k < G; k++) {
+            Per& q = per[k];
+            if (!q.active) continue;
+            q.level++;
+            std::swap(q.blocks, q.next);
+            q.nb = h_ctr[2 * k + 1];
+        }
+    }
+    if (fix) {
+        for (int k = 0; k < G; k++)
+            if ((rc = grid_fix_tvertices(c, gs[k]->k))) return rc;
+    }
+    StatsK s;
+    BT_HIP(hipMemcpyAsync(&s, d_stats, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    BT_HIP(hipStreamSynchronize(c->stream));
+    if (h_stats) *h_stats = s;
+    return BT_OK;
+}
+
 int grid_alloc(Context* c, bt_grid* g) {
     const int64_t NM = (int64_t)g->N * g->M;
     const int64_t nwords = (NM + 31) / 32;

@@ -16,15 +16,14 @@ void bt_grid_params_default(bt_grid_params* p) {
     p->fix_tvertices = 1;
 }
 
-int bt_grid_build(bt_context* h, const bt_pose* pose, const bt_integ_params* prm, int max_level,
-                  int start_level, int angle, const bt_grid_params* gp, bt_grid** out) {
-    if (!h || !out) return fail(BT_ERR_INVALID, "bt_grid_build: null argument");
+}  // extern "C"
+
+// A grid object with its pose, parameters and dense arrays allocated.
+static int grid_create(Context& c, const bt_pose* pose, const bt_integ_params* prm, int max_level,
+                       int start_level, int angle, bt_grid** out) {
     *out = nullptr;
     if (max_level < 1 || max_level > 12 || start_level < 1 || start_level > max_level)
-        return fail(BT_ERR_INVALID, "bt_grid_build: need 1 <= start_level <= max_level <= 12");
-    Context& c = h->impl;
-    std::lock_guard<std::recursive_mutex> lk(c.mu);
-    BT_HIP(hipSetDevice(c.device));
+        return fail(BT_ERR_INVALID, "grid build: need 1 <= start_level <= max_level <= 12");
     bt_grid* g = new bt_grid();
     g->ctx = &c;
     int rc;
@@ -45,13 +44,63 @@ int bt_grid_build(bt_context* h, const bt_pose* pose, const bt_integ_params* prm
         delete g;
         return rc;
     }
+    *out = g;
+    return BT_OK;
+}
+
+static void grid_destroy(bt_grid* g) {
+    if (!g) return;
+    grid_release(g);
+    delete g;
+}
+
+extern "C" {
+
+int bt_grid_build(bt_context* h, const bt_pose* pose, const bt_integ_params* prm, int max_level,
+                  int start_level, int angle, const bt_grid_params* gp, bt_grid** out) {
+    if (!h || !out) return fail(BT_ERR_INVALID, "bt_grid_build: null argument");
+    *out = nullptr;
+    Context& c = h->impl;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    BT_HIP(hipSetDevice(c.device));
+    bt_grid* g;
+    int rc;
+    if ((rc = grid_create(c, pose, prm, max_level, start_level, angle, &g))) return rc;
     if ((rc = grid_build(&c, g, gp))) {
         hipStreamSynchronize(c.stream);
-        grid_release(g);
-        delete g;
+        grid_destroy(g);
         return rc;
     }
     *out = g;
+    return BT_OK;
+}
+
+int bt_grid_build_batch(bt_context* h, int n, const bt_pose* poses, const bt_integ_params* prm,
+                        const int* max_levels, const int* start_levels, int angle,
+                        const bt_grid_params* gp, bt_grid** out, bt_trace_stats* stats) {
+    if (!h || !poses || !max_levels || !start_levels || !out)
+        return fail(BT_ERR_INVALID, "bt_grid_build_batch: null argument");
+    if (n < 1 || n > 128) return fail(BT_ERR_INVALID, "bt_grid_build_batch: need 1 <= n <= 128");
+    for (int k = 0; k < n; k++) out[k] = nullptr;
+    Context& c = h->impl;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    BT_HIP(hipSetDevice(c.device));
+    std::vector<bt_grid*> gs(n, nullptr);
+    int rc = BT_OK;
+    for (int k = 0; k < n && !rc; k++)
+        rc = grid_create(c, &poses[k], prm, max_levels[k], start_levels[k], angle, &gs[k]);
+    StatsK s{};
+    if (!rc) rc = grid_build_batch(&c, gs, gp, &s);
+    if (rc) {
+        hipStreamSynchronize(c.stream);
+        for (bt_grid* g : gs) grid_destroy(g);
+        return rc;
+    }
+    if (stats) stats_to_abi(s, stats);
+    for (int k = 0; k < n; k++) {
+        gs[k]->stats.rays = gs[k]->rays;
+        out[k] = gs[k];
+    }
     return BT_OK;
 }
 

@@ -80,8 +80,9 @@ __device__ __forceinline__ void initial_conditions(const PoseK& P, double theta,
 }
 
 // Metric.h:145-167 (Newton loop bounded by newton_cap).
+// *r0 (nullable) receives the b-matched photon-orbit radius of the Newton solve.
 __device__ __forceinline__ bool check_bq(const PoseK& P, double bV, double qV, int cap,
-                                         unsigned& iters) {
+                                         unsigned& iters, double* r0 = nullptr) {
     const double error = 0.0000001;
     double r0V = 2.0;
     double bcheck = 100;
@@ -96,7 +97,41 @@ __device__ __forceinline__ bool check_bq(const PoseK& P, double bV, double qV, i
     }
     iters += it;
     double qb = q0(P.asq, r0V);
+    if (r0) *r0 = r0V;
     return (P.b1 >= bV) || (P.b2 <= bV) || (qV >= qb);
+}
+
+// dq0/dr0 of q0 above: with N = -r^3 (r^3 - 6r^2 + 9r - 4a^2),
+// q0 = N / (a^2 (r-1)^2) and q0' = (N' (r-1) - 2N) / (a^2 (r-1)^3).
+__device__ __forceinline__ double q0diff(double asq, double r) {
+    double N = -sq3(r) * (sq3(r) - 6. * sq(r) + 9. * r - 4. * asq);
+    double Nd = -(6. * sq(sq(r)) * r - 30. * sq(sq(r)) + 36. * sq3(r) - 12. * asq * sq(r));
+    return (Nd * (r - 1.) - 2. * N) / (asq * sq3(r - 1.));
+}
+
+// Euclidean distance in the (b, q) plane from a ray's constants of motion
+// to the critical curve of spherical photon orbits {(b0(r0), q0(r0)) :
+// r0 in [r_ph1, r_ph2]} (Bardeen's curve, the shadow boundary; q0 = 0 at
+// both ends).  Clamped Gauss-Newton projection from the b-matched r0; for
+// a = 0 the curve is b^2 + q = 27.
+__device__ __forceinline__ double critical_distance(const PoseK& P, double b, double q, double r0) {
+    if (P.a_zero) {
+        double t = fmin(fmax(b, -sqrt(27.)), sqrt(27.));
+        for (int it = 0; it < 8; it++) {
+            double dq = -2. * t;
+            double num = (b - t) + (q - (27. - t * t)) * dq;
+            t = fmin(fmax(t + num / (1. + dq * dq), -sqrt(27.)), sqrt(27.));
+        }
+        return hypot(b - t, q - (27. - t * t));
+    }
+    double lo = P.r_ph1, hi = P.r_ph2;
+    double r = fmin(fmax(r0, lo), hi);
+    for (int it = 0; it < 8; it++) {
+        double db = b0diff(P.a, P.asq, r), dq = q0diff(P.asq, r);
+        double num = (b - b0(P.a, P.asq, r)) * db + (q - q0(P.asq, r)) * dq;
+        r = fmin(fmax(r + num / (db * db + dq * dq), lo), hi);
+    }
+    return hypot(b - b0(P.a, P.asq, r), q - q0(P.asq, r));
 }
 
 // Metric.h:110-131
@@ -125,8 +160,9 @@ __device__ __forceinline__ bool check_rup(const PoseK& P, double rV, double bV, 
 
 // Metric.h:169-174: true when the ray escapes (is integrated).
 __device__ __forceinline__ bool check_celest(const PoseK& P, double pR, double bV, double qV,
-                                             int cap, unsigned& newton, unsigned& golden) {
-    bool check1 = check_bq(P, bV, qV, cap, newton);
+                                             int cap, unsigned& newton, unsigned& golden,
+                                             double* r0 = nullptr) {
+    bool check1 = check_bq(P, bV, qV, cap, newton, r0);
     bool check2 = !check1 && (pR < 0);
     bool check3 = check1 && check_rup(P, P.r, bV, qV, golden);
     return check2 || check3;
@@ -166,9 +202,9 @@ __device__ __forceinline__ void derivs(double a, double asq, const double* v, do
 }
 
 // odeint1's loop body (Metric.h:292-297): derivs, scale, rkqs (Metric.h:253-270)
-// with rkck (Metric.h:228-251).  Updates var/z/h in place.
-__device__ __forceinline__ void attempt(double a, double asq, double* var, double& z, double& h,
-                                        double eps, double b, double q) {
+// with rkck (Metric.h:228-251).  Updates var/z/h in place; returns errmax.
+__device__ __forceinline__ double attempt(double a, double asq, double* var, double& z, double& h,
+                                          double eps, double b, double q) {
     double dvdz[5], scal[5], k2[5], k3[5], k4[5], k5[5], k6[5], t[5];
     derivs(a, asq, var, dvdz, b, q);
 #pragma unroll
@@ -210,6 +246,7 @@ __device__ __forceinline__ void attempt(double a, double asq, double* var, doubl
     } else {
         h = fmin(SAFETY * h * pow(errmax, PSHRNK), 0.1 * h);
     }
+    return errmax;
 }
 
 // Metric.h:176-185 (the while loops are kept: repeated addition of 2*PI
@@ -252,6 +289,88 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
+}
+
+// Wave-cooperative claim of queue positions for the lanes in ballot mask m
+// (every lane of the wave calls it with the same m).  A wave reserves
+// `chunk` (>= 64) consecutive positions per atomic on *head and hands them
+// out in order; next/end describe the unused part of its reservation and
+// are wave-uniform.  Returns this lane's position (meaningful when its bit
+// is set in m).  Positions >= the queue length mean "queue exhausted".
+__device__ __forceinline__ unsigned claim_positions(unsigned long long m, unsigned lane,
+                                                    unsigned* head, unsigned chunk,
+                                                    unsigned& next, unsigned& end) {
+    const unsigned cnt = (unsigned)__popcll(m);
+    const unsigned rank = (unsigned)__popcll(m & ((1ull << lane) - 1));
+    const unsigned avail = end - next;
+    unsigned my;
+    if (cnt <= avail) {
+        my = next + rank;
+        next += cnt;
+    } else {
+        int leader = __ffsll((long long)m) - 1;
+        unsigned base = 0;
+        if (lane == (unsigned)leader) base = atomicAdd(head, chunk);
+        base = __shfl(base, leader);
+        my = rank < avail ? next + rank : base + (rank - avail);
+        next = base + (cnt - avail);
+        end = base + chunk;
+    }
+    return my;
+}
+
+// Pose constants a lane needs while it traces one ray.
+struct LanePose {
+    double a, asq, r, theta, phi, r_cap;
+    __device__ __forceinline__ void load(const PoseK& P) {
+        a = P.a;
+        asq = P.asq;
+        r = P.r;
+        theta = P.theta;
+        phi = P.phi;
+        r_cap = P.r_cap;
+    }
+};
+
+// Launch shape of a persistent kernel: one resident set of workgroups
+// (Context::tune: block size, resident waves per SIMD), never more than the
+// rays need.
+struct Launch {
+    unsigned blocks, threads;
+};
+
+template <class K>
+int persistent_launch(Context* c, K kernel, int64_t n_max, Launch* L) {
+    const int threads = (int)c->tune.block;
+    if (threads != 64 && threads != 128 && threads != 256)
+        return fail(BT_ERR_INVALID, "tune: block size must be 64, 128 or 256");
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, 0) != hipSuccess || occ < 1)
+        occ = 1;
+    const int wpb = threads / 64;
+    int waves_cu = occ * wpb;
+    if (c->tune.waves > 0 && c->tune.waves * 4 < waves_cu) waves_cu = c->tune.waves * 4;
+    int bpc = waves_cu / wpb;
+    if (bpc < 1) bpc = 1;
+    int64_t blocks = (int64_t)c->num_cus * bpc;
+    int64_t need = ((n_max + 63) / 64 + wpb - 1) / wpb;
+    if (need < blocks) blocks = need;
+    if (blocks < 1) blocks = 1;
+    L->blocks = (unsigned)blocks;
+    L->threads = (unsigned)threads;
+    return BT_OK;
+}
+
+// Appends the lanes with `take` set to queue Q (one atomic per wave).  Every
+// lane of the wave must call it.
+__device__ __forceinline__ void queue_push(const QueueK& Q, bool take, uint32_t k, unsigned lane) {
+    unsigned long long m = __ballot(take);
+    if (!m) return;
+    int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane == (unsigned)leader) base = atomicAdd(&Q.counters[0], (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (take) Q.slots[base + (unsigned)__popcll(m & ((1ull << lane) - 1))] = k;
 }
 
 }  // namespace kerr

@@ -19,71 +19,121 @@ namespace {
 
 using namespace kerr;
 
-struct QueueK {
-    double* pr;
-    double* pt;
-    double* b;
-    double* q;
-    uint32_t* slots;    // queue of ray indices
-    unsigned* counters; // [0] = queue length, [1] = head
-};
-
 __global__ void zero_counters(unsigned* c, int n) {
     int i = threadIdx.x;
     if (i < n) c[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void ic_capture_f64(PoseK P, ParamsK prm, int64_t n, RayIO io,
-                                                      QueueK Q, StatsK* st) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Block-aggregated append to two queues: every thread of the block calls
+// it (uniformly); take0/take1 select the queue(s) ray k goes to.  One
+// atomic per queue per block and round.
+__device__ __forceinline__ void block_push2(const QueueK& Q0, bool take0, const QueueK& Q1,
+                                            bool take1, uint32_t k) {
+    __shared__ unsigned s_cnt[2][4];
+    __shared__ unsigned s_base[2];
+    const unsigned lane = lane_id(), wave = threadIdx.x >> 6;
+    const unsigned long long lt = (1ull << lane) - 1;
+    unsigned long long m0 = __ballot(take0), m1 = __ballot(take1);
+    if (lane == 0) {
+        s_cnt[0][wave] = (unsigned)__popcll(m0);
+        s_cnt[1][wave] = (unsigned)__popcll(m1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        unsigned t = 0;
+        for (unsigned w = 0; w < blockDim.x / 64; w++) t += s_cnt[threadIdx.x][w];
+        const QueueK& Qq = threadIdx.x ? Q1 : Q0;
+        s_base[threadIdx.x] = t ? atomicAdd(&Qq.counters[0], t) : 0;
+    }
+    __syncthreads();
+    unsigned off0 = s_base[0], off1 = s_base[1];
+    for (unsigned w = 0; w < wave; w++) {
+        off0 += s_cnt[0][w];
+        off1 += s_cnt[1][w];
+    }
+    if (take0) Q0.slots[off0 + (unsigned)__popcll(m0 & lt)] = k;
+    if (take1) Q1.slots[off1 + (unsigned)__popcll(m1 & lt)] = k;
+    __syncthreads();   // s_cnt/s_base are reused by the next round
+}
+
+// Initial conditions and a-priori capture test, grid-stride over the rays
+// (counters stay in registers until the end).  SPLIT = false: every
+// escaping ray goes to Q.  SPLIT = true (mixed precision): rays whose
+// (b, q) lie within crit_eps of the critical curve (Euclidean distance in
+// the (b, q) plane) go to Q64 and are traced in FP64 throughout; the rest
+// go to Q.  Queue entries and per-ray arrays use the global ray index.
+template <bool SPLIT, class Src>
+__global__ __launch_bounds__(256) void ic_capture(Src S, ParamsK prm, int64_t n, QueueK Q,
+                                                  QueueK Q64, StatsK* st) {
     const unsigned lane = lane_id();
-    unsigned newton = 0, golden = 0;
-    bool live = k < n;
-    bool escapes = false;
-    double pR = 0, pT = 0, b = 0, q = 0;
-    if (live) {
-        double th, ph;
-        ray_direction(io, k, th, ph);
-        initial_conditions(P, th, ph, pR, pT, b, q);
-        escapes = prm.apriori ? check_celest(P, pR, b, q, prm.newton_cap, newton, golden) : true;
-        if (!escapes) ray_store(io, k, -1, -1, 0);
+    unsigned long long c_rays = 0, c_int = 0, c_nw = 0, c_gd = 0, c_cr = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x; k0 < n; k0 += stride) {
+        const int64_t k = k0 + threadIdx.x;
+        unsigned newton = 0, golden = 0;
+        bool live = k < n;
+        bool escapes = false, crit = false;
+        double pR = 0, pT = 0, b = 0, q = 0;
+        if (live) {
+            const int sg = S.seg(k);
+            const PoseK& P = S.pose(sg);
+            const RayIO& io = S.rio(sg);
+            const int64_t kl = S.local(sg, k);
+            double th, ph;
+            ray_direction(io, kl, th, ph);
+            initial_conditions(P, th, ph, pR, pT, b, q);
+            double r0 = 2.0;
+            escapes = prm.apriori ? check_celest(P, pR, b, q, prm.newton_cap, newton, golden,
+                                                 SPLIT ? &r0 : nullptr)
+                                  : true;
+            if (!escapes) ray_store(io, kl, -1, -1, 0);
+            if (SPLIT && escapes) {
+                if (!prm.apriori && !P.a_zero) check_bq(P, b, q, prm.newton_cap, newton, &r0);
+                crit = critical_distance(P, b, q, r0) < prm.crit_eps;
+            }
+        }
+        if (escapes) {
+            Q.pr[k] = pR;
+            Q.pt[k] = pT;
+            Q.b[k] = b;
+            Q.q[k] = q;
+        }
+        block_push2(Q, escapes && !crit, Q64, SPLIT && crit, (uint32_t)k);
+        c_rays += live;
+        c_int += escapes;
+        c_nw += newton;
+        c_gd += golden;
+        c_cr += crit;
     }
-    unsigned long long m = __ballot(escapes);
-    unsigned base = 0;
-    if (m) {
-        int leader = __ffsll((long long)m) - 1;
-        if (lane == (unsigned)leader) base = atomicAdd(&Q.counters[0], (unsigned)__popcll(m));
-        base = __shfl(base, leader);
-    }
-    if (escapes) {
-        unsigned pos = base + (unsigned)__popcll(m & ((1ull << lane) - 1));
-        Q.slots[pos] = (uint32_t)k;
-        Q.pr[k] = pR;
-        Q.pt[k] = pT;
-        Q.b[k] = b;
-        Q.q[k] = q;
-    }
-    unsigned long long s_rays = wave_sum(live ? 1ull : 0ull);
-    unsigned long long s_int = wave_sum(escapes ? 1ull : 0ull);
-    unsigned long long s_nw = wave_sum((unsigned long long)newton);
-    unsigned long long s_gd = wave_sum((unsigned long long)golden);
+    unsigned long long s_rays = wave_sum(c_rays), s_int = wave_sum(c_int), s_nw = wave_sum(c_nw),
+                       s_gd = wave_sum(c_gd), s_cr = wave_sum(c_cr);
     if (lane == 0 && s_rays) {
         atomicAdd(&st->rays, s_rays);
         atomicAdd(&st->integrated, s_int);
         atomicAdd(&st->newton, s_nw);
         atomicAdd(&st->golden, s_gd);
+        if (s_cr) atomicAdd(&st->fallback, s_cr);
     }
 }
 
-__global__ __launch_bounds__(256) void rk_f64(PoseK P, ParamsK prm, RayIO io, QueueK Q,
-                                              StatsK* st) {
+// WARM = false: every ray runs to its end (the plain FP64 path).  WARM =
+// true (mixed precision): a ray runs in FP64 through the initial step-size
+// growth, where the FP64 error estimate is far below ERRCON and an FP32
+// estimate would only see rounding noise; after its first attempt with
+// errmax > ERRCON its state goes to cont.  cont.nstp[ray] = -1 marks a ray
+// that ended here.
+template <bool WARM, class Src>
+__global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, StatsK* st,
+                                              unsigned chunk, ContK cont) {
     const unsigned lane = lane_id();
-    const unsigned long long lt_mask = (1ull << lane) - 1;
     const unsigned qlen = Q.counters[0];
     unsigned* head = &Q.counters[1];
-    const double a = P.a, asq = P.asq;
+    unsigned res_next = 0, res_end = 0;   // this wave's reserved queue positions
 
     int64_t slot = -1;
+    int sg = 0;                 // segment (pose) of the lane's ray
+    LanePose lp;
+    if (!Src::kMulti) lp.load(S.pose(0));
     bool exhausted = false;
     double var[5];
     double z = 0, h = 0, b = 0, q = 0;
@@ -94,17 +144,17 @@ __global__ __launch_bounds__(256) void rk_f64(PoseK P, ParamsK prm, RayIO io, Qu
         bool need = (slot < 0) && !exhausted;
         unsigned long long m = __ballot(need);
         if (m) {
-            int leader = __ffsll((long long)m) - 1;
-            unsigned base = 0;
-            if (lane == (unsigned)leader) base = atomicAdd(head, (unsigned)__popcll(m));
-            base = __shfl(base, leader);
+            unsigned my = claim_positions(m, lane, head, chunk, res_next, res_end);
             if (need) {
-                unsigned my = base + (unsigned)__popcll(m & lt_mask);
                 if (my < qlen) {
                     slot = Q.slots[my];
-                    var[0] = P.r;
-                    var[1] = P.theta;
-                    var[2] = P.phi;
+                    if (Src::kMulti) {
+                        sg = S.seg(slot);
+                        lp.load(S.pose(sg));
+                    }
+                    var[0] = lp.r;
+                    var[1] = lp.theta;
+                    var[2] = lp.phi;
                     var[3] = Q.pr[slot];
                     var[4] = Q.pt[slot];
                     b = Q.b[slot];
@@ -119,12 +169,12 @@ __global__ __launch_bounds__(256) void rk_f64(PoseK P, ParamsK prm, RayIO io, Qu
         }
         if (__ballot(slot >= 0) == 0) break;
         if (slot >= 0) {
-            attempt(a, asq, var, z, h, prm.eps, b, q);
+            double em = attempt(lp.a, lp.asq, var, z, h, prm.eps, b, q);
             attempts++;
             const int step = nstp;
             nstp++;
             bool esc = z <= prm.z_end || (prm.r_escape > 0 && var[0] > prm.r_escape);
-            bool cap = !esc && P.r_cap > 0 && var[0] < P.r_cap;
+            bool cap = !esc && lp.r_cap > 0 && var[0] < lp.r_cap;
             bool out = !esc && !cap && nstp >= prm.max_steps;
             if (esc || cap || out) {
                 double th, ph;
@@ -133,11 +183,19 @@ __global__ __launch_bounds__(256) void rk_f64(PoseK P, ParamsK prm, RayIO io, Qu
                     ph = -1;
                 } else {
                     // out of attempts: varStart was never overwritten (Metric.h:298-302)
-                    th = esc ? var[1] : P.theta;
-                    ph = esc ? var[2] : P.phi;
+                    th = esc ? var[1] : lp.theta;
+                    ph = esc ? var[2] : lp.phi;
                     wrap_to_pi(th, ph);
                 }
-                ray_store(io, slot, th, ph, step);
+                ray_store(S.rio(sg), S.local(sg, slot), th, ph, step);
+                if (WARM) cont.nstp[slot] = -1;
+                slot = -1;
+            } else if (WARM && em > ERRCON) {
+#pragma unroll
+                for (int i = 0; i < 5; i++) cont.y[i * cont.stride + slot] = var[i];
+                cont.y[5 * cont.stride + slot] = z;
+                cont.h[slot] = (float)h;
+                cont.nstp[slot] = nstp;
                 slot = -1;
             }
         }
@@ -151,47 +209,81 @@ __global__ __launch_bounds__(256) void rk_f64(PoseK P, ParamsK prm, RayIO io, Qu
 
 }  // namespace
 
-int trace_f64(Context* c, const PoseK& pose, const ParamsK& prm, int64_t n, const RayIO& io,
-              StatsK* d_stats) {
-    if (n <= 0) return BT_OK;
-    if (n > 0x7fffffffLL) return fail(BT_ERR_INVALID, "trace_f64: too many rays in one batch");
-    QueueK Q;
+int queue_alloc(Context* c, int64_t n, QueueK* Q) {
     void* p;
     int rc;
-    if ((rc = c->buffer("f64_pr", sizeof(double) * n, &p))) return rc;
-    Q.pr = (double*)p;
-    if ((rc = c->buffer("f64_pt", sizeof(double) * n, &p))) return rc;
-    Q.pt = (double*)p;
-    if ((rc = c->buffer("f64_b", sizeof(double) * n, &p))) return rc;
-    Q.b = (double*)p;
-    if ((rc = c->buffer("f64_q", sizeof(double) * n, &p))) return rc;
-    Q.q = (double*)p;
-    if ((rc = c->buffer("f64_slots", sizeof(uint32_t) * n, &p))) return rc;
-    Q.slots = (uint32_t*)p;
-    if ((rc = c->buffer("f64_ctr", 64, &p))) return rc;
-    Q.counters = (unsigned*)p;
+    if ((rc = c->buffer("q_pr", sizeof(double) * n, &p))) return rc;
+    Q->pr = (double*)p;
+    if ((rc = c->buffer("q_pt", sizeof(double) * n, &p))) return rc;
+    Q->pt = (double*)p;
+    if ((rc = c->buffer("q_b", sizeof(double) * n, &p))) return rc;
+    Q->b = (double*)p;
+    if ((rc = c->buffer("q_q", sizeof(double) * n, &p))) return rc;
+    Q->q = (double*)p;
+    if ((rc = c->buffer("q_slots", sizeof(uint32_t) * n, &p))) return rc;
+    Q->slots = (uint32_t*)p;
+    if ((rc = c->buffer("q_ctr", 64, &p))) return rc;
+    Q->counters = (unsigned*)p;
+    return BT_OK;
+}
 
-    zero_counters<<<1, 64, 0, c->stream>>>(Q.counters, 2);
-    {
-        KTimer t(c, "ic_capture_f64");
-        int64_t blocks = (n + 255) / 256;
-        ic_capture_f64<<<(unsigned)blocks, 256, 0, c->stream>>>(pose, prm, n, io, Q, d_stats);
-    }
+template <class Src>
+int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const QueueK& Q,
+                const QueueK* Q64, StatsK* d_stats) {
+    zero_counters<<<1, 64, 0, c->stream>>>(Q.counters, 8);
+    KTimer t(c, Q64 ? "ic_split" : "ic_capture_f64");
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > (int64_t)c->num_cus * 16) blocks = (int64_t)c->num_cus * 16;
+    if (Q64)
+        ic_capture<true, Src><<<(unsigned)blocks, 256, 0, c->stream>>>(S, prm, n, Q, *Q64, d_stats);
+    else
+        ic_capture<false, Src><<<(unsigned)blocks, 256, 0, c->stream>>>(S, prm, n, Q, Q, d_stats);
     BT_HIP(hipGetLastError());
-    {
-        KTimer t(c, "rk_f64");
-        int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, rk_f64, 256, 0) != hipSuccess ||
-            occ < 1)
-            occ = 2;
-        int64_t waves_needed = (n + 63) / 64;
-        int64_t blocks = (int64_t)c->num_cus * occ;   // exactly one resident wave set
-        int64_t need_blocks = (waves_needed + 3) / 4;
-        if (need_blocks < blocks) blocks = need_blocks;
-        rk_f64<<<(unsigned)blocks, 256, 0, c->stream>>>(pose, prm, io, Q, d_stats);
+    return BT_OK;
+}
+
+template <class Src>
+int trace_f64_queue(Context* c, const Src& S, const ParamsK& prm, int64_t n_max, const QueueK& Q,
+                    StatsK* d_stats, const char* timer, const ContK* cont) {
+    KTimer t(c, timer);
+    Launch L;
+    if (cont) {
+        if (int rc = persistent_launch(c, rk_f64<true, Src>, n_max, &L)) return rc;
+        rk_f64<true, Src><<<L.blocks, L.threads, 0, c->stream>>>(S, prm, Q, d_stats, c->tune.chunk, *cont);
+    } else {
+        if (int rc = persistent_launch(c, rk_f64<false, Src>, n_max, &L)) return rc;
+        rk_f64<false, Src><<<L.blocks, L.threads, 0, c->stream>>>(S, prm, Q, d_stats, c->tune.chunk, ContK{});
     }
     BT_HIP(hipGetLastError());
     return BT_OK;
+}
+
+template int trace_setup<SingleSrc>(Context*, const SingleSrc&, const ParamsK&, int64_t, const QueueK&,
+                                    const QueueK*, StatsK*);
+template int trace_setup<MultiSrc>(Context*, const MultiSrc&, const ParamsK&, int64_t, const QueueK&,
+                                   const QueueK*, StatsK*);
+template int trace_f64_queue<SingleSrc>(Context*, const SingleSrc&, const ParamsK&, int64_t, const QueueK&,
+                                        StatsK*, const char*, const ContK*);
+template int trace_f64_queue<MultiSrc>(Context*, const MultiSrc&, const ParamsK&, int64_t, const QueueK&,
+                                       StatsK*, const char*, const ContK*);
+
+template <class Src>
+int trace_f64_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, StatsK* d_stats) {
+    if (n <= 0) return BT_OK;
+    if (n > 0x7fffffffLL) return fail(BT_ERR_INVALID, "trace_f64: too many rays in one batch");
+    QueueK Q;
+    int rc;
+    if ((rc = queue_alloc(c, n, &Q))) return rc;
+    if ((rc = trace_setup(c, S, prm, n, Q, nullptr, d_stats))) return rc;
+    return trace_f64_queue(c, S, prm, n, Q, d_stats, "rk_f64", nullptr);
+}
+
+template int trace_f64_src<MultiSrc>(Context*, const MultiSrc&, const ParamsK&, int64_t, StatsK*);
+
+int trace_f64(Context* c, const PoseK& pose, const ParamsK& prm, int64_t n, const RayIO& io,
+              StatsK* d_stats) {
+    SingleSrc S{pose, io};
+    return trace_f64_src(c, S, prm, n, d_stats);
 }
 
 }  // namespace bt

@@ -1,12 +1,333 @@
-// kerr_mixed.hip -- mixed-precision ray path (FP32 Cash-Karp bulk).
-#include "bt_internal.h"
+// kerr_mixed.hip -- mixed-precision ray path: the bulk of the Cash-Karp
+// attempts run in FP32, everything that decides or accumulates runs in FP64.
+//
+//   FP64  initial conditions, constants of motion (b, q), the capture test
+//         and the critical-curve test (ic_capture<true>, kerr_f64.hip);
+//         the initial step-size growth of every ray, until its first
+//         attempt with errmax > ERRCON (rk_f64<true>): there the FP64 error
+//         estimate is far below ERRCON while an FP32 one is rounding noise
+//         above it, and the two step sequences would part from the start;
+//         the state (r, theta, phi, pR, pTheta) and the affine parameter z,
+//         to which every accepted FP32 increment is added; at each stage
+//         point Delta = (r-1)^2 - (1-a^2), P = r^2 + a^2 - ab and the range
+//         reduction of theta, so that no FP32 cancellation happens near the
+//         horizon or the poles; every ray near the critical curve, and
+//         every ray whose step collapses (|h| < hmin_fp64) or which nears
+//         the attempt cap, is restarted from its initial state in FP64
+//         (rk_f64, the plain path's kernel) and so ends exactly as there.
+//   FP32  the six right-hand-side evaluations, stage combinations, error
+//         estimate and step-size control.
+//
+// The right-hand side is the reference's (Metric.h:198-226) regrouped with
+// iR = 1/rho^2, iD = 1/Delta and Ph = P/Delta, using partR + R/Delta =
+// P^2/Delta (R = P^2 - Delta*partR, Metric.h:96-98):
+//   dr/dz      = Delta iR pR
+//   dtheta/dz  = iR pTheta
+//   dphi/dz    = iR (a Ph - a + b / sin^2)
+//   dpR/dz     = iR (2 r Ph - (r-1)(Ph^2 + pR^2)) + r iR^2 X
+//   dpTheta/dz = iR cos (b^2 / sin^3 - a^2 sin) - a^2 cos sin iR^2 X
+// with X = pTheta^2 + pR^2 Delta - Theta - P Ph + partR (zero on the null
+// cone; kept so the flow is the reference's off the cone as well).
+// Compiled with -ffp-contract=fast.
+#include "kerr_dev.h"
 
 namespace bt {
+namespace {
+
+using namespace kerr;
+
+// Cash-Karp tableau (Metric.h:41-49), rounded to float.
+constexpr float fb21 = (float)b21, fb31 = (float)b31, fb32 = (float)b32, fb41 = (float)b41,
+                fb42 = (float)b42, fb43 = (float)b43, fb51 = (float)b51, fb52 = (float)b52,
+                fb53 = (float)b53, fb54 = (float)b54, fb61 = (float)b61, fb62 = (float)b62,
+                fb63 = (float)b63, fb64 = (float)b64, fb65 = (float)b65;
+constexpr float fc1 = (float)c1, fc3 = (float)c3, fc4 = (float)c4, fc6 = (float)c6;
+constexpr float fdc1 = (float)dc1, fdc3 = (float)dc3, fdc4 = (float)dc4, fdc5 = (float)dc5,
+                fdc6 = (float)dc6;
+constexpr double PIO2 = PI / 2;
+constexpr double TWO_OVER_PI = 2 / PI;
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// x^p for x > 0 through the hardware log2/exp2 (the step-size control only).
+__device__ __forceinline__ float powf_pos(float x, float p) {
+    return __builtin_amdgcn_exp2f(p * __builtin_amdgcn_logf(x));
+}
+
+// sin and cos of theta (FP64) with FP32 polynomials: theta = n*pi/2 + v,
+// |v| <= pi/4, reduced in FP64 so that sin stays relatively accurate near
+// both poles.  Taylor series to v^9 / v^10: truncation < 2e-9 on |v| <= pi/4.
+__device__ __forceinline__ void sincos_theta(double th, float& s, float& c) {
+    double n = rint(th * TWO_OVER_PI);
+    float v = (float)fma(-n, PIO2, th);
+    float v2 = v * v;
+    float sv = v * (1.f + v2 * (-1.f / 6 + v2 * (1.f / 120 + v2 * (-1.f / 5040 + v2 * (1.f / 362880)))));
+    float cv = 1.f + v2 * (-0.5f + v2 * (1.f / 24 + v2 * (-1.f / 720 + v2 * (1.f / 40320 + v2 * (-1.f / 3628800)))));
+    int qd = (int)n & 3;
+    float s0 = (qd & 1) ? cv : sv;
+    float c0 = (qd & 1) ? sv : cv;
+    s = (qd & 2) ? -s0 : s0;
+    c = ((qd + 1) & 2) ? -c0 : c0;
+}
+
+// Per-ray constants.
+struct RayC {
+    float a, asq, b, bsq, q, partR;
+    double e;      // 1 - a^2
+    double Poff;   // a^2 - a b
+};
+
+// Right-hand side at the stage point (r, theta) in FP64 and (pR, pTheta) in FP32.
+__device__ __forceinline__ void derivs_m(const RayC& C, double r_d, double th_d, float pr, float pt,
+                                         float* o) {
+    double rm1_d = r_d - 1.0;
+    double D_d = fma(rm1_d, rm1_d, -C.e);
+    double P_d = fma(r_d, r_d, C.Poff);
+    float r = (float)r_d, rm1 = (float)rm1_d, D = (float)D_d, P = (float)P_d;
+    float s, c;
+    sincos_theta(th_d, s, c);
+    float c2 = c * c;
+    float rosq = r * r + C.asq * c2;
+    float iR = rcp(rosq), iD = rcp(D), is = rcp(s);
+    float is2 = is * is;
+    float Ph = P * iD;
+    float Theta = C.q - c2 * (C.bsq * is2 - C.asq);
+    float prsq = pr * pr;
+    float X = pt * pt - Theta + C.partR + prsq * D - P * Ph;
+    float iR2X = iR * iR * X;
+    o[0] = D * iR * pr;
+    o[1] = iR * pt;
+    o[2] = iR * (C.a * Ph - C.a + C.b * is2);
+    o[3] = iR * (2.f * r * Ph - rm1 * (Ph * Ph + prsq)) + r * iR2X;
+    o[4] = iR * c * (C.bsq * is2 * is - C.asq * s) - C.asq * c * s * iR2X;
+}
+
+// One Cash-Karp attempt with the reference's step control (Metric.h:253-297)
+// in FP32; an accepted step adds the FP32 increment to the FP64 state.
+__device__ __forceinline__ void attempt_m(const RayC& C, double* y, double& z, float& h, float inv_eps) {
+    float k1[5], k2[5], k3[5], k4[5], k5[5], k6[5];
+    const float pr0 = (float)y[3], pt0 = (float)y[4];
+    const float hh = h;
+    derivs_m(C, y[0], y[1], pr0, pt0, k1);
+    float d[5];
+#define STAGE_IN(expr)                                                                      \
+    _Pragma("unroll") for (int i = 0; i < 5; i++) d[i] = hh * (expr);
+#define STAGE_EVAL(out) derivs_m(C, y[0] + (double)d[0], y[1] + (double)d[1], pr0 + d[3], pt0 + d[4], out)
+    STAGE_IN(fb21 * k1[i]);
+    STAGE_EVAL(k2);
+    STAGE_IN(fb31 * k1[i] + fb32 * k2[i]);
+    STAGE_EVAL(k3);
+    STAGE_IN(fb41 * k1[i] + fb42 * k2[i] + fb43 * k3[i]);
+    STAGE_EVAL(k4);
+    STAGE_IN(fb51 * k1[i] + fb52 * k2[i] + fb53 * k3[i] + fb54 * k4[i]);
+    STAGE_EVAL(k5);
+    STAGE_IN(fb61 * k1[i] + fb62 * k2[i] + fb63 * k3[i] + fb64 * k4[i] + fb65 * k5[i]);
+    STAGE_EVAL(k6);
+#undef STAGE_IN
+#undef STAGE_EVAL
+    float errmax = 0.f;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        d[i] = hh * (fc1 * k1[i] + fc3 * k3[i] + fc4 * k4[i] + fc6 * k6[i]);
+        float err = hh * (fdc1 * k1[i] + fdc3 * k3[i] + fdc4 * k4[i] + fdc5 * k5[i] + fdc6 * k6[i]);
+        float scal = fabsf((float)y[i]) + fabsf(k1[i] * hh) + (float)TINY;
+        errmax = fmaxf(errmax, fabsf(err) * rcp(scal));
+    }
+    errmax *= inv_eps;
+    if (errmax <= 1.0f) {
+        z += (double)hh;
+#pragma unroll
+        for (int i = 0; i < 5; i++) y[i] += (double)d[i];
+        h = errmax > (float)ERRCON ? (float)SAFETY * hh * powf_pos(errmax, (float)PGROW)
+                                   : (float)ADAPTIVE * hh;
+    } else {
+        h = fminf((float)SAFETY * hh * powf_pos(errmax, (float)PSHRNK), 0.1f * hh);
+    }
+}
+
+// Continues the rays queued in Q (the warm-up queue, walked a second time
+// with its own head) from their warm-up state in cont, skipping rays that
+// ended in the warm-up; rays whose step collapses or that near the attempt
+// cap go to Q64.
+template <class Src>
+__global__ __launch_bounds__(256) void rk_mixed(Src S, ParamsK prm, QueueK Q, unsigned* head,
+                                                QueueK Q64, ContK cont, StatsK* st, int restart_at,
+                                                unsigned chunk) {
+    const unsigned lane = lane_id();
+    const unsigned qlen = Q.counters[0];
+    unsigned res_next = 0, res_end = 0;   // this wave's reserved queue positions
+    const float inv_eps = (float)(1.0 / prm.eps);
+    const float hmin = (float)prm.hmin_fp64;
+
+    int64_t slot = -1;
+    int sg = 0;
+    double r_cap = 0;
+    bool exhausted = false;
+    double y[5];
+    double z = 0;
+    float h = 0;
+    RayC C;
+    if (!Src::kMulti) {
+        const PoseK& P = S.pose(0);
+        C.a = (float)P.a;
+        C.asq = (float)P.asq;
+        C.e = 1.0 - P.asq;
+        r_cap = P.r_cap;
+    }
+    int nstp = 0;
+    unsigned long long attempts = 0, restarts = 0;
+
+    while (true) {
+        bool need = (slot < 0) && !exhausted;
+        unsigned long long m = __ballot(need);
+        if (m) {
+            unsigned my = claim_positions(m, lane, head, chunk, res_next, res_end);
+            if (need) {
+                if (my < qlen) {
+                    slot = Q.slots[my];
+                    nstp = cont.nstp[slot];
+                }
+                if (my >= qlen) {
+                    exhausted = true;
+                } else if (nstp < 0) {
+                    slot = -1;   // ended in the warm-up; claim again next round
+                } else {
+                    if (Src::kMulti) {
+                        sg = S.seg(slot);
+                        const PoseK& P = S.pose(sg);
+                        C.a = (float)P.a;
+                        C.asq = (float)P.asq;
+                        C.e = 1.0 - P.asq;
+                        r_cap = P.r_cap;
+                    }
+                    const double pa = Src::kMulti ? S.pose(sg).a : S.pose(0).a;
+                    const double pasq = Src::kMulti ? S.pose(sg).asq : S.pose(0).asq;
+#pragma unroll
+                    for (int i = 0; i < 5; i++) y[i] = cont.y[i * cont.stride + slot];
+                    double b = Q.b[slot], q = Q.q[slot];
+                    C.b = (float)b;
+                    C.bsq = (float)(b * b);
+                    C.q = (float)q;
+                    C.partR = (float)(q + (pa - b) * (pa - b));
+                    C.Poff = pasq - pa * b;
+                    z = cont.y[5 * cont.stride + slot];
+                    h = cont.h[slot];
+                }
+            }
+        }
+        // lanes that drew a ray which ended in the warm-up hold no slot but
+        // are not exhausted: they claim again next round
+        if (__ballot(slot >= 0 || !exhausted) == 0) break;
+        bool restart = false, done = false;
+        if (slot >= 0) {
+            attempt_m(C, y, z, h, inv_eps);
+            attempts++;
+            const int step = nstp;
+            nstp++;
+            bool esc = z <= prm.z_end || (prm.r_escape > 0 && y[0] > prm.r_escape);
+            bool cap = !esc && r_cap > 0 && y[0] < r_cap;
+            restart = !esc && !cap && (nstp >= restart_at || !(fabsf(h) >= hmin));
+            done = esc || cap || restart;
+            if (esc || cap) {
+                double th = -1, ph = -1;
+                if (esc) {
+                    th = y[1];
+                    ph = y[2];
+                    wrap_to_pi(th, ph);
+                }
+                ray_store(S.rio(sg), S.local(sg, slot), th, ph, step);
+            }
+        }
+        queue_push(Q64, restart, (uint32_t)slot, lane);
+        if (restart) restarts++;
+        if (done) slot = -1;
+    }
+    unsigned long long s = wave_sum(attempts);
+    unsigned long long r = wave_sum(restarts);
+    if (lane == 0) {
+        if (s) atomicAdd(&st->attempts, s);
+        if (r) atomicAdd(&st->fallback, r);
+    }
+}
+
+}  // namespace
+
+template <class Src>
+int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, StatsK* d_stats) {
+    if (n <= 0) return BT_OK;
+    if (n > 0x7fffffffLL) return fail(BT_ERR_INVALID, "trace_mixed: too many rays in one batch");
+    if (prm.eps < 1e-9)
+        return fail(BT_ERR_INVALID, "mixed precision needs eps >= 1e-9 (FP32 error estimate); use BT_PREC_FP64");
+    // Qw: warm-up (FP64) then FP32 queue, Q64: FP64 from the start
+    QueueK Qw, Q64;
+    int rc;
+    if ((rc = queue_alloc(c, n, &Qw))) return rc;
+    void* p;
+    Q64 = Qw;
+    if ((rc = c->buffer("q_slots64", sizeof(uint32_t) * n, &p))) return rc;
+    Q64.slots = (uint32_t*)p;
+    Q64.counters = Qw.counters + 2;   // [2] = length, [3] = head
+    unsigned* head32 = Qw.counters + 4;
+    ContK cont;
+    cont.stride = n;
+    if ((rc = c->buffer("cont_y", sizeof(double) * 6 * n, &p))) return rc;
+    cont.y = (double*)p;
+    if ((rc = c->buffer("cont_h", sizeof(float) * n, &p))) return rc;
+    cont.h = (float*)p;
+    if ((rc = c->buffer("cont_nstp", sizeof(int32_t) * n, &p))) return rc;
+    cont.nstp = (int32_t*)p;
+
+    if ((rc = trace_setup(c, S, prm, n, Qw, &Q64, d_stats))) return rc;
+    if ((rc = trace_f64_queue(c, S, prm, n, Qw, d_stats, "rk_f64_warmup", &cont))) return rc;
+    {
+        KTimer t(c, "rk_mixed");
+        Launch L;
+        if ((rc = persistent_launch(c, rk_mixed<Src>, n, &L))) return rc;
+        // rays that near the attempt cap restart in FP64, so that a capped
+        // ray ends exactly as in the FP64 path
+        int margin = prm.max_steps / 16 > 16 ? prm.max_steps / 16 : 16;
+        int restart_at = prm.max_steps - margin > 1 ? prm.max_steps - margin : 1;
+        rk_mixed<Src><<<L.blocks, L.threads, 0, c->stream>>>(S, prm, Qw, head32, Q64, cont, d_stats,
+                                                             restart_at, c->tune.chunk);
+        BT_HIP(hipGetLastError());
+    }
+    return trace_f64_queue(c, S, prm, n, Q64, d_stats, "rk_f64_fallback", nullptr);
+}
+
+template int trace_mixed_src<MultiSrc>(Context*, const MultiSrc&, const ParamsK&, int64_t, StatsK*);
 
 int trace_mixed(Context* c, const PoseK& pose, const ParamsK& prm, int64_t n, const RayIO& io,
                 StatsK* d_stats) {
-    (void)c; (void)pose; (void)prm; (void)n; (void)io; (void)d_stats;
-    return fail(BT_ERR_STATE, "mixed-precision path not built yet");
+    SingleSrc S{pose, io};
+    return trace_mixed_src(c, S, prm, n, d_stats);
+}
+
+int trace_batch(Context* c, const BatchHost& b, const ParamsK& prm, StatsK* d_stats) {
+    const int nseg = (int)b.poses.size();
+    if (nseg == 0) return BT_OK;
+    if ((int)b.ios.size() != nseg || (int)b.counts.size() != nseg)
+        return fail(BT_ERR_INVALID, "trace_batch: inconsistent segment lists");
+    std::vector<int64_t> start(nseg + 1, 0);
+    for (int s = 0; s < nseg; s++) start[s + 1] = start[s] + b.counts[s];
+    const int64_t n = start[nseg];
+    if (n == 0) return BT_OK;
+    void* p;
+    int rc;
+    MultiSrc S;
+    S.nseg = nseg;
+    if ((rc = c->buffer("batch_poses", sizeof(PoseK) * nseg, &p))) return rc;
+    S.poses = (const PoseK*)p;
+    BT_HIP(hipMemcpyAsync(p, b.poses.data(), sizeof(PoseK) * nseg, hipMemcpyHostToDevice, c->stream));
+    if ((rc = c->buffer("batch_ios", sizeof(RayIO) * nseg, &p))) return rc;
+    S.ios = (const RayIO*)p;
+    BT_HIP(hipMemcpyAsync(p, b.ios.data(), sizeof(RayIO) * nseg, hipMemcpyHostToDevice, c->stream));
+    if ((rc = c->buffer("batch_start", sizeof(int64_t) * (nseg + 1), &p))) return rc;
+    S.start = (const int64_t*)p;
+    BT_HIP(hipMemcpyAsync(p, start.data(), sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice, c->stream));
+    // pageable sources: wait until the copies have consumed the host vectors
+    BT_HIP(hipStreamSynchronize(c->stream));
+    return prm.precision == BT_PREC_MIXED ? trace_mixed_src(c, S, prm, n, d_stats)
+                                          : trace_f64_src(c, S, prm, n, d_stats);
 }
 
 }  // namespace bt

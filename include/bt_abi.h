@@ -132,6 +132,17 @@ void bt_grid_params_default(bt_grid_params* p);
 int  bt_grid_build(bt_context* ctx, const bt_pose* pose, const bt_integ_params* prm,
                    int max_level, int start_level, int angle, const bt_grid_params* gp,
                    bt_grid** out);
+/* Several grids at once (no counterpart in the reference, which builds one
+ * Grid at a time, Grid.h:605): n poses with their own max/start levels and
+ * shared integration parameters.  The grids advance level by level in
+ * lockstep and the rays of each level of every grid are traced together in
+ * one set of persistent launches over a shared device work queue.  out
+ * receives n grids (each freed with bt_grid_free); stats (nullable) the
+ * batch's counters; bt_grid_stats of a batched grid holds only its ray
+ * count.  1 <= n <= 128. */
+int  bt_grid_build_batch(bt_context* ctx, int n, const bt_pose* poses, const bt_integ_params* prm,
+                         const int* max_levels, const int* start_levels, int angle,
+                         const bt_grid_params* gp, bt_grid** out, bt_trace_stats* stats);
 int  bt_grid_info(const bt_grid* g, int* N, int* M, int* max_level, int* start_level,
                   int* equafactor, int64_t* rays_traced);
 int  bt_grid_stats(const bt_grid* g, bt_trace_stats* stats);

@@ -1,0 +1,101 @@
+"""Mixed-precision ray path (BT_PREC_MIXED) against the plain FP64 path.
+
+What is exact: capture flags (the a-priori test runs in FP64 in both
+paths); every ray sent to FP64 -- near the critical curve, step collapse,
+attempt cap -- ends exactly as in the FP64 path.  What is not: rays
+traced in FP32 take a step sequence of their own, so their celestial angle
+differs from the FP64 path's by up to that path's own truncation error; the
+bound tested here is statistical (mean, p99) with the maximum reported.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import blacktracer_amd as bt
+
+pytestmark = pytest.mark.gpu
+
+ARCSEC = math.pi / 180 / 3600
+
+
+def _lattice(L):
+    N = 2 ** L + 1
+    M = 2 * (N - 1)
+    i, j = np.meshgrid(np.arange(N), np.arange(M), indexing="ij")
+    return (i / (N - 1) * math.pi).ravel(), (j / M * 2 * math.pi).ravel()
+
+
+def _gc(t1, p1, t2, p2):
+    h = np.sin((t2 - t1) / 2) ** 2 + np.sin(t1) * np.sin(t2) * np.sin((p2 - p1) / 2) ** 2
+    return 2 * np.arcsin(np.sqrt(np.clip(h, 0, 1)))
+
+
+POSES = [(0.5, 3.0), (0.999, 5.0), (0.9, 20.0)]
+
+
+@pytest.mark.parametrize("a,r", POSES)
+def test_mixed_vs_fp64_error_and_flags(ctx, a, r):
+    print(f"mixed vs fp64 a={a} r={r}", flush=True)
+    cam = bt.Camera(math.pi / 2, 0.0, r, black_hole=bt.BlackHole(a))
+    t, p = _lattice(8)
+    t64, p64, s64 = bt.trace_rays(cam, t, p, ctx=ctx)
+    tm, pm, sm, stm = bt.trace_rays(cam, t, p, params=bt.IntegParams(bt.BT_PREC_MIXED), ctx=ctx,
+                                    return_stats=True)
+    cap64, capm = t64 == -1, tm == -1
+    assert np.array_equal(cap64, capm)
+    e = _gc(t64[~cap64], p64[~cap64], tm[~cap64], pm[~cap64]) / ARCSEC
+    print(f"  arcsec: mean {e.mean():.3f} p99 {np.quantile(e, 0.99):.3f} max {e.max():.1f}; "
+          f"fp64 share {stm['attempts_fp64'] / stm['attempts']:.3f}", flush=True)
+    assert e.mean() < MEAN_BOUND
+    assert np.quantile(e, 0.99) < P99_BOUND
+    assert stm["attempts_fp64"] < stm["attempts"]
+
+
+def test_mixed_all_critical_is_fp64_exactly(ctx):
+    """crit_eps larger than any distance: every ray runs in FP64, results
+    are the FP64 path's bit for bit."""
+    cam = bt.Camera(math.pi / 2, 0.0, 5.0, black_hole=bt.BlackHole(0.999))
+    t, p = _lattice(6)
+    t64, p64, s64 = bt.trace_rays(cam, t, p, ctx=ctx)
+    tm, pm, sm, st = bt.trace_rays(cam, t, p, params=bt.IntegParams(bt.BT_PREC_MIXED, crit_eps=1e30),
+                                   ctx=ctx, return_stats=True)
+    assert np.array_equal(t64, tm) and np.array_equal(p64, pm) and np.array_equal(s64, sm)
+    assert st["attempts_fp64"] == st["attempts"]
+    assert st["fallback_rays"] == st["integrated"]
+
+
+def test_mixed_attempt_cap_restarts_in_fp64(ctx):
+    """Rays that near the attempt cap restart in FP64, so a capped ray
+    returns the FP64 path's sentinel (the camera's own angles)."""
+    cam = bt.Camera(math.pi / 2, 0.0, 5.0, black_hole=bt.BlackHole(0.999))
+    t, p = _lattice(6)
+    prm64 = bt.IntegParams(bt.BT_PREC_FP64, max_steps=40)
+    prmm = bt.IntegParams(bt.BT_PREC_MIXED, max_steps=40)
+    t64, p64, s64 = bt.trace_rays(cam, t, p, params=prm64, ctx=ctx)
+    tm, pm, sm, st = bt.trace_rays(cam, t, p, params=prmm, ctx=ctx, return_stats=True)
+    capped = s64 == 39
+    assert capped.sum() > 10
+    assert np.array_equal(t64[capped], tm[capped]) and np.array_equal(p64[capped], pm[capped])
+    assert st["fallback_rays"] >= capped.sum()
+
+
+def test_mixed_rejects_tiny_eps(ctx):
+    cam = bt.Camera(math.pi / 2, 0.0, 5.0, black_hole=bt.BlackHole(0.999))
+    with pytest.raises(bt.BtError):
+        bt.trace_rays(cam, np.array([1.0]), np.array([1.0]),
+                      params=bt.IntegParams(bt.BT_PREC_MIXED, eps=1e-12), ctx=ctx)
+
+
+def test_mixed_grid_build(ctx):
+    """The adaptive grid built with the mixed path: same layout, capture
+    pattern equal wherever both grids hold a point."""
+    bh = bt.BlackHole(0.999)
+    cam = bt.Camera(math.pi / 2, 0.0, 5.0, black_hole=bh)
+    g64 = bt.Grid(8, 1, True, cam, bh, ctx=ctx)
+    gm = bt.Grid(8, 1, True, cam, bh, params=bt.IntegParams(bt.BT_PREC_MIXED), ctx=ctx)
+    assert (g64.N, g64.M) == (gm.N, gm.M)
+    both = (g64.cel[..., 0] != -2) & (gm.cel[..., 0] != -2)
+    c64, cm = g64.cel[both], gm.cel[both]
+    assert np.array_equal(c64[:, 0] == -1, cm[:, 0] == -1)
+    assert abs(g64.rays_traced - gm.rays_traced) <= 0.01 * g64.rays_traced

@@ -54,6 +54,17 @@ def test_trace_rays_fp64_matches_oracle(ctx, pose):
     print(f"  max |d angle| = {err:.3g} rad", flush=True)
 
 
+def test_trace_rays_fp64_matches_reference_vectors(ctx):
+    """Against the reference's own outputs (tests/golden/ref_rays.npz)."""
+    import os
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_rays.npz"))
+    for k, (a, th0, ph0, r) in enumerate(gold["poses"]):
+        print(f"reference vectors pose {k}", flush=True)
+        cam = bt.Camera(th0, ph0, r, black_hole=bt.BlackHole(a))
+        th, ph, st = bt.trace_rays(cam, gold[f"p{k}_in_theta"], gold[f"p{k}_in_phi"], ctx=ctx)
+        _compare(th, ph, st, gold[f"p{k}_theta"], gold[f"p{k}_phi"], gold[f"p{k}_step"])
+
+
 def test_trace_rays_edge_cases(ctx):
     cam = bt.Camera(math.pi / 2, 0.0, 5.0, black_hole=bt.BlackHole(0.999))
     # empty batch
