@@ -31,6 +31,8 @@ struct bt_grid {
     int N = 0, M = 0, eq = 0, max_level = 0, start_level = 0;
     int64_t rays = 0;
     size_t bytes = 0;
+    void* block = nullptr;      // the device block holding the arrays (context pool)
+    size_t block_bytes = 0;
     bt::GridK k{};
     bt::PoseK pose{};
     bt::ParamsK prm{};

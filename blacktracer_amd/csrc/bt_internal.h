@@ -129,6 +129,7 @@ struct BatchHost {
 struct PendingTimer {
     std::string name;
     hipEvent_t start, stop;
+    hipStream_t stream;
 };
 
 struct Buffer {
@@ -148,6 +149,8 @@ struct Context {
     Tune tune;
     int num_cus = 256;
     hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;                 // overlapping work, joined before a call returns
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::recursive_mutex mu;
     bool timing = false;
     std::vector<PendingTimer> pending;
@@ -155,10 +158,22 @@ struct Context {
     std::map<std::string, std::pair<int64_t, double>> timers;
     std::map<std::string, Buffer> scratch;   // grow-only named work buffers
     unsigned long long* d_counters = nullptr;  // small device scratch for counters
+    // Device blocks of freed grids, kept for the next grid of a similar size
+    // (a grid build would otherwise pay hipMalloc/hipFree of its arrays).
+    std::multimap<size_t, void*> pool;
+    size_t pool_bytes = 0;
+    static constexpr size_t kPoolCap = size_t(32) << 30;
+
+    // A block of at least `bytes` (reused when one of up to 2x that size is
+    // pooled); *got receives its actual size.
+    int pool_alloc(size_t bytes, void** out, size_t* got);
+    // Returns a block to the pool (or frees it beyond kPoolCap); the caller
+    // has synchronised the stream.
+    void pool_free(void* p, size_t bytes);
 
     // Returns a device buffer of at least `bytes` for `name`.
     int buffer(const std::string& name, size_t bytes, void** out);
-    int timer_begin(const char* name, int* slot);
+    int timer_begin(const char* name, int* slot, hipStream_t st = nullptr);
     int timer_end(int slot);
     int flush_timers();
 };
@@ -167,8 +182,8 @@ struct Context {
 struct KTimer {
     Context* c;
     int slot = -1;
-    KTimer(Context* ctx, const char* name) : c(ctx) {
-        if (c->timing) c->timer_begin(name, &slot);
+    KTimer(Context* ctx, const char* name, hipStream_t st = nullptr) : c(ctx) {
+        if (c->timing) c->timer_begin(name, &slot, st);
     }
     ~KTimer() {
         if (slot >= 0) c->timer_end(slot);
@@ -215,7 +230,8 @@ int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const Q
                 const QueueK* Q64, StatsK* d_stats);
 template <class Src>
 int trace_f64_queue(Context* c, const Src& S, const ParamsK& prm, int64_t n_max, const QueueK& Q,
-                    StatsK* d_stats, const char* timer, const ContK* cont);
+                    StatsK* d_stats, const char* timer, const ContK* cont,
+                    hipStream_t stream = nullptr);
 template <class Src>
 int trace_f64_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, StatsK* d_stats);
 // kerr_mixed.hip: FP32 bulk with FP64 set-up, warm-up, fallback and accumulation.

@@ -40,9 +40,43 @@ int Context::buffer(const std::string& name, size_t bytes, void** out) {
     return BT_OK;
 }
 
-int Context::timer_begin(const char* name, int* slot) {
+int Context::pool_alloc(size_t bytes, void** out, size_t* got) {
+    auto it = pool.lower_bound(bytes);
+    if (it != pool.end() && it->first <= 2 * bytes) {
+        *out = it->second;
+        *got = it->first;
+        pool_bytes -= it->first;
+        pool.erase(it);
+        return BT_OK;
+    }
+    if (hipMalloc(out, bytes) != hipSuccess) {
+        // give the pooled blocks back and try once more
+        for (auto& kv : pool) hipFree(kv.second);
+        pool.clear();
+        pool_bytes = 0;
+        if (hipMalloc(out, bytes) != hipSuccess) {
+            *out = nullptr;
+            return fail(BT_ERR_NOMEM, "hipMalloc failed for a grid of " + std::to_string(bytes) + " bytes");
+        }
+    }
+    *got = bytes;
+    return BT_OK;
+}
+
+void Context::pool_free(void* p, size_t bytes) {
+    if (!p) return;
+    if (pool_bytes + bytes > kPoolCap) {
+        hipFree(p);
+        return;
+    }
+    pool.emplace(bytes, p);
+    pool_bytes += bytes;
+}
+
+int Context::timer_begin(const char* name, int* slot, hipStream_t st) {
     PendingTimer t;
     t.name = name;
+    t.stream = st ? st : stream;
     for (hipEvent_t* e : {&t.start, &t.stop}) {
         if (!free_events.empty()) {
             *e = free_events.back();
@@ -51,20 +85,21 @@ int Context::timer_begin(const char* name, int* slot) {
             BT_HIP(hipEventCreate(e));
         }
     }
-    BT_HIP(hipEventRecord(t.start, stream));
+    BT_HIP(hipEventRecord(t.start, t.stream));
     pending.push_back(t);
     *slot = (int)pending.size() - 1;
     return BT_OK;
 }
 
 int Context::timer_end(int slot) {
-    BT_HIP(hipEventRecord(pending[slot].stop, stream));
+    BT_HIP(hipEventRecord(pending[slot].stop, pending[slot].stream));
     return BT_OK;
 }
 
 int Context::flush_timers() {
     if (pending.empty()) return BT_OK;
     BT_HIP(hipStreamSynchronize(stream));
+    BT_HIP(hipStreamSynchronize(aux));
     for (auto& t : pending) {
         float ms = 0.f;
         BT_HIP(hipEventElapsedTime(&ms, t.start, t.stop));
@@ -222,7 +257,12 @@ int bt_context_create(int device, bt_context** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c.num_cus = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
+    // two streams at the default priority: the main one and an auxiliary one
+    // for work that overlaps it (joined back before a call returns)
+    if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming) != hipSuccess) {
         delete h;
         return fail(BT_ERR_HIP, "hipStreamCreate failed");
     }
@@ -235,13 +275,18 @@ void bt_context_destroy(bt_context* h) {
     Context& c = h->impl;
     hipSetDevice(c.device);
     hipStreamSynchronize(c.stream);
+    hipStreamSynchronize(c.aux);
     for (auto& kv : c.scratch)
         if (kv.second.ptr) hipFree(kv.second.ptr);
+    for (auto& kv : c.pool) hipFree(kv.second);
     for (auto& t : c.pending) {
         hipEventDestroy(t.start);
         hipEventDestroy(t.stop);
     }
     for (auto e : c.free_events) hipEventDestroy(e);
+    hipEventDestroy(c.ev_fork);
+    hipEventDestroy(c.ev_join);
+    hipStreamDestroy(c.aux);
     hipStreamDestroy(c.stream);
     delete h;
 }
@@ -317,7 +362,7 @@ void bt_integ_params_default(bt_integ_params* p) {
     p->r_capture_factor = 0.0;
     p->newton_cap = 100000;
     p->precision = BT_PREC_FP64;
-    p->crit_eps = 1e-4;
+    p->crit_eps = 0.01;
     p->hmin_fp64 = 1e-6;
 }
 

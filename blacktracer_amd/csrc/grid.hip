@@ -267,29 +267,40 @@ __device__ __forceinline__ bool block_fixable(const GridK& G, uint32_t i, uint32
 }
 
 // fixTvertices (Grid.h:103-119) for every block of level L, in key order
-// along each line (see the header comment).
-__global__ void fix_level(GridK G, int L, unsigned nrows, unsigned npairs) {
-    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+// along each line (see the header comment).  One wavefront per line: the
+// lanes test 64 candidate blocks at a time (block of this level with a
+// valid corner, and the edge midpoint present -- nothing a fix on this
+// level changes), then lane 0 runs checkAdjacentBlock for the hits in
+// ascending key order (ballot bit order), which keeps the sequential
+// order of the fixes.
+__device__ __forceinline__ void fix_line(const GridK& G, int L, unsigned nrows, unsigned npairs, unsigned t) {
+    const unsigned lane = threadIdx.x & 63;
     const uint32_t M = (uint32_t)G.M, N = (uint32_t)G.N;
     const int gap = 1 << (G.max_level - L);
-    const uint32_t ug = (uint32_t)gap;
+    const uint32_t ug = (uint32_t)gap, half = ug / 2;
     if (t < nrows) {
-        uint32_t y = t * ug;
-        if (y >= ug) {
-            // bottom edges (k_j, k_l) of blocks keyed on row y - gap
-            for (uint32_t j = 0; j < M; j += ug)
-                if (block_fixable(G, y - ug, j, L))
-                    check_adjacent(G, Edge{y, j, y, (j + ug) % M, L, gap}, 0);
-        }
-        if (y + ug <= N - 1) {
-            // top edges (ij, i_l) of blocks keyed on row y
-            for (uint32_t j = 0; j < M; j += ug)
-                if (block_fixable(G, y, j, L))
-                    check_adjacent(G, Edge{y, j, y, (j + ug) % M, L, gap}, 0);
+        const uint32_t y = t * ug;
+        for (int pass = 0; pass < 2; pass++) {
+            // pass 0: bottom edges (k_j, k_l) of blocks keyed on row y - gap;
+            // pass 1: top edges (ij, i_l) of blocks keyed on row y
+            if (pass == 0 && y < ug) continue;
+            if (pass == 1 && y + ug > N - 1) continue;
+            const uint32_t row = pass == 0 ? y - ug : y;
+            for (uint32_t j0 = 0; j0 < M; j0 += 64 * ug) {
+                const uint32_t j = j0 + lane * ug;
+                bool hit = j < M && block_fixable(G, row, j, L) && present_at(G, y, j + half);
+                unsigned long long m = __ballot(hit);
+                while (m) {
+                    const int src = __ffsll((long long)m) - 1;
+                    m &= m - 1;
+                    const uint32_t jj = j0 + (uint32_t)src * ug;
+                    if (lane == 0) check_adjacent(G, Edge{y, jj, y, (jj + ug) % M, L, gap}, 0);
+                }
+            }
         }
     } else if (t - nrows < npairs) {
-        uint32_t c = (t - nrows) * ug;
-        uint32_t c2 = c + M / 2;
+        const uint32_t c = (t - nrows) * ug;
+        const uint32_t c2 = c + M / 2;
         uint32_t cand[4] = {c, (c - ug + M) % M, c2, (c2 - ug + M) % M};
         // ascending j (4-element sort network)
 #define BT_SWAP(x, y)                   \
@@ -300,21 +311,50 @@ __global__ void fix_level(GridK G, int L, unsigned nrows, unsigned npairs) {
     }
         BT_SWAP(0, 1) BT_SWAP(2, 3) BT_SWAP(0, 2) BT_SWAP(1, 3) BT_SWAP(1, 2)
 #undef BT_SWAP
-        for (uint32_t i = 0; i + ug <= N - 1; i += ug) {
-            for (int s = 0; s < 4; s++) {
-                uint32_t j = cand[s];
-                if (!block_fixable(G, i, j, L)) continue;
-                if (j == c || j == c2) {
-                    // left edge (ij, k_j)
-                    check_adjacent(G, Edge{i, j, i + ug, j, L, gap}, 1);
-                } else {
-                    // right edge (i_l, k_l)
-                    uint32_t l = (j + ug) % M;
-                    check_adjacent(G, Edge{i, l, i + ug, l, L, gap}, 1);
-                }
+        // lane = 4 * (row offset) + candidate: bit order = rows ascending,
+        // then candidates ascending, the sequential loop's order
+        const uint32_t s = lane & 3;
+        const uint32_t js = cand[s];
+        const bool left = (js == c || js == c2);
+        const uint32_t col = left ? js : (js + ug) % M;   // the edge's column
+        for (uint32_t i0 = 0; i0 + ug <= N - 1; i0 += 16 * ug) {
+            const uint32_t i = i0 + (lane >> 2) * ug;
+            bool hit = i + ug <= N - 1 && block_fixable(G, i, js, L) && present_at(G, i + half, col);
+            unsigned long long m = __ballot(hit);
+            while (m) {
+                const int src = __ffsll((long long)m) - 1;
+                m &= m - 1;
+                const uint32_t ii = i0 + (uint32_t)(src >> 2) * ug;
+                const uint32_t cc = __shfl(col, src);
+                if (lane == 0) check_adjacent(G, Edge{ii, cc, ii + ug, cc, L, gap}, 1);
             }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void fix_level(GridK G, int L, unsigned nrows, unsigned npairs) {
+    fix_line(G, L, nrows, npairs, (blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+}
+
+// The same for several grids in one launch: grid g owns line items
+// [item0[g], item0[g+1]) (rows first, then meridian pairs, as above).
+struct FixJob {
+    GridK G;
+    unsigned nrows, npairs, item0;
+    int L;
+};
+
+__global__ __launch_bounds__(256) void fix_level_multi(const FixJob* jobs, int njobs) {
+    const unsigned t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].item0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const FixJob& J = jobs[lo];
+    if (t < J.item0) return;
+    fix_line(J.G, J.L, J.nrows, J.npairs, t - J.item0);
 }
 
 unsigned blocks_for(int64_t n, int tpb) { return (unsigned)((n + tpb - 1) / tpb); }
@@ -328,9 +368,48 @@ int grid_fix_tvertices(Context* c, const GridK& G) {
         unsigned npairs = (unsigned)((G.M / 2) / gap);
         unsigned n = nrows + npairs;
         KTimer t(c, "fix_tvertices");
-        fix_level<<<blocks_for(n, 64), 64, 0, c->stream>>>(G, L, nrows, npairs);
+        fix_level<<<blocks_for((int64_t)n * 64, 256), 256, 0, c->stream>>>(G, L, nrows, npairs);
         BT_HIP(hipGetLastError());
     }
+    return BT_OK;
+}
+
+// fixTvertices of several grids, one launch per level for all of them.
+int grid_fix_tvertices_multi(Context* c, const std::vector<bt_grid*>& gs) {
+    int lmin = 99, lmax = 0;
+    for (bt_grid* g : gs) {
+        lmin = std::min(lmin, g->k.start_level);
+        lmax = std::max(lmax, g->k.max_level);
+    }
+    void* p;
+    if (int rc = c->buffer("fix_jobs", sizeof(FixJob) * gs.size(), &p)) return rc;
+    FixJob* d_jobs = (FixJob*)p;
+    std::vector<FixJob> jobs;
+    for (int L = lmin; L < lmax; L++) {
+        jobs.clear();
+        unsigned items = 0;
+        for (bt_grid* g : gs) {
+            const GridK& G = g->k;
+            if (L < G.start_level || L >= G.max_level) continue;
+            int gap = 1 << (G.max_level - L);
+            FixJob j;
+            j.G = G;
+            j.L = L;
+            j.nrows = (unsigned)((G.N - 1) / gap + 1);
+            j.npairs = (unsigned)((G.M / 2) / gap);
+            j.item0 = items;
+            items += j.nrows + j.npairs;
+            jobs.push_back(j);
+        }
+        if (jobs.empty()) continue;
+        // the previous level's launch may still read d_jobs
+        BT_HIP(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(FixJob) * jobs.size(), hipMemcpyHostToDevice, c->stream));
+        BT_HIP(hipStreamSynchronize(c->stream));
+        KTimer t(c, "fix_tvertices");
+        fix_level_multi<<<blocks_for((int64_t)items * 64, 256), 256, 0, c->stream>>>(d_jobs, (int)jobs.size());
+        BT_HIP(hipGetLastError());
+    }
+    BT_HIP(hipStreamSynchronize(c->stream));
     return BT_OK;
 }
 
@@ -582,10 +661,7 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
             q.nb = h_ctr[2 * k + 1];
         }
     }
-    if (fix) {
-        for (int k = 0; k < G; k++)
-            if ((rc = grid_fix_tvertices(c, gs[k]->k))) return rc;
-    }
+    if (fix && (rc = grid_fix_tvertices_multi(c, gs))) return rc;
     StatsK s;
     BT_HIP(hipMemcpyAsync(&s, d_stats, sizeof(s), hipMemcpyDeviceToHost, c->stream));
     BT_HIP(hipStreamSynchronize(c->stream));
@@ -602,21 +678,32 @@ int grid_alloc(Context* c, bt_grid* g) {
     G.eq = g->eq;
     G.max_level = g->max_level;
     G.start_level = g->start_level;
-    BT_HIP(hipMalloc((void**)&G.cel, sizeof(double) * 2 * NM));
-    BT_HIP(hipMalloc((void**)&G.steps, sizeof(int32_t) * NM));
-    BT_HIP(hipMalloc((void**)&G.level, sizeof(int8_t) * NM));
-    BT_HIP(hipMalloc((void**)&G.present, sizeof(uint32_t) * nwords));
-    g->bytes = sizeof(double) * 2 * NM + sizeof(int32_t) * NM + NM + sizeof(uint32_t) * nwords;
-    (void)c;
+    // one block: cel | steps | level | present, each 256-byte aligned
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t b_cel = up(sizeof(double) * 2 * NM), b_steps = up(sizeof(int32_t) * NM),
+                 b_level = up((size_t)NM), b_present = up(sizeof(uint32_t) * nwords);
+    void* p;
+    size_t got;
+    if (int rc = c->pool_alloc(b_cel + b_steps + b_level + b_present, &p, &got)) return rc;
+    g->block = p;
+    g->block_bytes = got;
+    char* base = (char*)p;
+    G.cel = (double*)base;
+    G.steps = (int32_t*)(base + b_cel);
+    G.level = (int8_t*)(base + b_cel + b_steps);
+    G.present = (uint32_t*)(base + b_cel + b_steps + b_level);
+    g->bytes = b_cel + b_steps + b_level + b_present;
     return BT_OK;
 }
 
 void grid_release(bt_grid* g) {
+    if (g->block) {
+        if (g->ctx) g->ctx->pool_free(g->block, g->block_bytes);
+        else hipFree(g->block);
+    }
+    g->block = nullptr;
+    g->block_bytes = 0;
     GridK& G = g->k;
-    if (G.cel) hipFree(G.cel);
-    if (G.steps) hipFree(G.steps);
-    if (G.level) hipFree(G.level);
-    if (G.present) hipFree(G.present);
     G.cel = nullptr;
     G.steps = nullptr;
     G.level = nullptr;

@@ -244,15 +244,16 @@ int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const Q
 
 template <class Src>
 int trace_f64_queue(Context* c, const Src& S, const ParamsK& prm, int64_t n_max, const QueueK& Q,
-                    StatsK* d_stats, const char* timer, const ContK* cont) {
-    KTimer t(c, timer);
+                    StatsK* d_stats, const char* timer, const ContK* cont, hipStream_t stream) {
+    if (!stream) stream = c->stream;
+    KTimer t(c, timer, stream);
     Launch L;
     if (cont) {
         if (int rc = persistent_launch(c, rk_f64<true, Src>, n_max, &L)) return rc;
-        rk_f64<true, Src><<<L.blocks, L.threads, 0, c->stream>>>(S, prm, Q, d_stats, c->tune.chunk, *cont);
+        rk_f64<true, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, *cont);
     } else {
         if (int rc = persistent_launch(c, rk_f64<false, Src>, n_max, &L)) return rc;
-        rk_f64<false, Src><<<L.blocks, L.threads, 0, c->stream>>>(S, prm, Q, d_stats, c->tune.chunk, ContK{});
+        rk_f64<false, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, ContK{});
     }
     BT_HIP(hipGetLastError());
     return BT_OK;
@@ -263,9 +264,9 @@ template int trace_setup<SingleSrc>(Context*, const SingleSrc&, const ParamsK&, 
 template int trace_setup<MultiSrc>(Context*, const MultiSrc&, const ParamsK&, int64_t, const QueueK&,
                                    const QueueK*, StatsK*);
 template int trace_f64_queue<SingleSrc>(Context*, const SingleSrc&, const ParamsK&, int64_t, const QueueK&,
-                                        StatsK*, const char*, const ContK*);
+                                        StatsK*, const char*, const ContK*, hipStream_t);
 template int trace_f64_queue<MultiSrc>(Context*, const MultiSrc&, const ParamsK&, int64_t, const QueueK&,
-                                       StatsK*, const char*, const ContK*);
+                                       StatsK*, const char*, const ContK*, hipStream_t);
 
 template <class Src>
 int trace_f64_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, StatsK* d_stats) {

@@ -258,15 +258,21 @@ int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, Sta
     if (n > 0x7fffffffLL) return fail(BT_ERR_INVALID, "trace_mixed: too many rays in one batch");
     if (prm.eps < 1e-9)
         return fail(BT_ERR_INVALID, "mixed precision needs eps >= 1e-9 (FP32 error estimate); use BT_PREC_FP64");
-    // Qw: warm-up (FP64) then FP32 queue, Q64: FP64 from the start
-    QueueK Qw, Q64;
+    // Qw: warm-up (FP64) then FP32 queue; Qc: near-critical rays, FP64 from
+    // the start, traced on the auxiliary stream while the main stream runs
+    // the warm-up and the FP32 bulk; Qr: rays the FP32 kernel hands back
+    // (step collapse, attempt cap), FP64 from the start on the main stream.
+    QueueK Qw, Qc, Qr;
     int rc;
     if ((rc = queue_alloc(c, n, &Qw))) return rc;
     void* p;
-    Q64 = Qw;
-    if ((rc = c->buffer("q_slots64", sizeof(uint32_t) * n, &p))) return rc;
-    Q64.slots = (uint32_t*)p;
-    Q64.counters = Qw.counters + 2;   // [2] = length, [3] = head
+    Qc = Qr = Qw;
+    if ((rc = c->buffer("q_slots_crit", sizeof(uint32_t) * n, &p))) return rc;
+    Qc.slots = (uint32_t*)p;
+    Qc.counters = Qw.counters + 2;   // [2] = length, [3] = head
+    if ((rc = c->buffer("q_slots_restart", sizeof(uint32_t) * n, &p))) return rc;
+    Qr.slots = (uint32_t*)p;
+    Qr.counters = Qw.counters + 6;   // [6] = length, [7] = head
     unsigned* head32 = Qw.counters + 4;
     ContK cont;
     cont.stride = n;
@@ -277,21 +283,47 @@ int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, Sta
     if ((rc = c->buffer("cont_nstp", sizeof(int32_t) * n, &p))) return rc;
     cont.nstp = (int32_t*)p;
 
-    if ((rc = trace_setup(c, S, prm, n, Qw, &Q64, d_stats))) return rc;
-    if ((rc = trace_f64_queue(c, S, prm, n, Qw, d_stats, "rk_f64_warmup", &cont))) return rc;
+    if ((rc = trace_setup(c, S, prm, n, Qw, &Qc, d_stats))) return rc;
+    // fork: critical rays on the auxiliary stream, at most one workgroup per CU
+    BT_HIP(hipEventRecord(c->ev_fork, c->stream));
+    BT_HIP(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+    const int64_t crit_lanes = (int64_t)c->num_cus * c->tune.block;
+    rc = trace_f64_queue(c, S, prm, n < crit_lanes ? n : crit_lanes, Qc, d_stats, "rk_f64_critical",
+                         nullptr, c->aux);
+    // join (also on error, so that nothing is left running on the aux stream)
+    auto join = [&]() {
+        hipEventRecord(c->ev_join, c->aux);
+        hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    };
+    if (rc) {
+        join();
+        return rc;
+    }
+    if ((rc = trace_f64_queue(c, S, prm, n, Qw, d_stats, "rk_f64_warmup", &cont))) {
+        join();
+        return rc;
+    }
     {
         KTimer t(c, "rk_mixed");
         Launch L;
-        if ((rc = persistent_launch(c, rk_mixed<Src>, n, &L))) return rc;
+        if ((rc = persistent_launch(c, rk_mixed<Src>, n, &L))) {
+            join();
+            return rc;
+        }
         // rays that near the attempt cap restart in FP64, so that a capped
         // ray ends exactly as in the FP64 path
         int margin = prm.max_steps / 16 > 16 ? prm.max_steps / 16 : 16;
         int restart_at = prm.max_steps - margin > 1 ? prm.max_steps - margin : 1;
-        rk_mixed<Src><<<L.blocks, L.threads, 0, c->stream>>>(S, prm, Qw, head32, Q64, cont, d_stats,
+        rk_mixed<Src><<<L.blocks, L.threads, 0, c->stream>>>(S, prm, Qw, head32, Qr, cont, d_stats,
                                                              restart_at, c->tune.chunk);
-        BT_HIP(hipGetLastError());
+        if (hipGetLastError() != hipSuccess) {
+            join();
+            return fail(BT_ERR_HIP, "rk_mixed launch failed");
+        }
     }
-    return trace_f64_queue(c, S, prm, n, Q64, d_stats, "rk_f64_fallback", nullptr);
+    rc = trace_f64_queue(c, S, prm, n, Qr, d_stats, "rk_f64_fallback", nullptr);
+    join();
+    return rc;
 }
 
 template int trace_mixed_src<MultiSrc>(Context*, const MultiSrc&, const ParamsK&, int64_t, StatsK*);

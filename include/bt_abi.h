@@ -90,7 +90,8 @@ typedef struct {
     double r_capture_factor;  /* >0: stop as captured once r < f * r_horizon */
     int    newton_cap;        /* cap on checkB_Q's Newton loop (reference: none) */
     int    precision;         /* BT_PREC_FP64 / BT_PREC_MIXED */
-    double crit_eps;          /* mixed: FP64 for rays with |q - q0(b)| < crit_eps */
+    double crit_eps;          /* mixed: FP64 for rays whose (b, q) lie within crit_eps of the
+                                 critical curve (distance in the (b, q) plane; 0.01) */
     double hmin_fp64;         /* mixed: switch a ray to FP64 once |h| < hmin_fp64 */
 } bt_integ_params;
 

@@ -17,6 +17,12 @@ import blacktracer_amd as bt
 pytestmark = pytest.mark.gpu
 
 ARCSEC = math.pi / 180 / 3600
+# Bounds on mixed-vs-FP64 great-circle error (arcsec) over integrated rays.
+# Measured on MI355X over nine precision-config poses (a in {0.5, 0.9,
+# 0.999}, r in {3, 5, 20}): mean 0.014-0.070, p99 0.055-0.33; the bounds
+# leave a factor of about two.
+MEAN_BOUND = 0.15
+P99_BOUND = 0.6
 
 
 def _lattice(L):
