@@ -454,3 +454,73 @@ def make_image(stars, star_tree_arr, cam_param, magnitude, tree_level, M, N, ste
         ts.ctypes.data_as(_lib.I32P), ot.size // 2, ht.size // 2, C.byref(flags),
         out.ctypes.data_as(_lib.U8P)))
     return out
+
+
+# ---------------------------------------------------------------- serving
+
+
+class Server:
+    """Continuous-batching render server (bt_server_*): requests of mixed
+    resolution and pose, grids cached in HBM, admission by HBM footprint.
+
+    ``submit`` is non-blocking; the frame lands in the given uint8 array
+    (height x width x 4, BGRA) before its completion is returned by ``poll``
+    or counted by ``drain``."""
+
+    def __init__(self, scene, hbm_budget=16 << 30, slots=4, max_level=10, start_level=1, params=None,
+                 step=1, ctx=None):
+        self.ctx = ctx or default_context()
+        L = _lib.load()
+        cfg = _lib.BtServerConfig()
+        L.bt_server_config_default(C.byref(cfg))
+        cfg.hbm_budget = hbm_budget
+        cfg.slots = slots
+        cfg.max_level = max_level
+        cfg.start_level = start_level
+        if params is not None:
+            cfg.integ = params
+        cfg.step = step
+        h = C.c_void_p()
+        check(L.bt_server_create(self.ctx.handle, scene.handle, C.byref(cfg), C.byref(h)))
+        self.handle = h
+        self.scene = scene          # must outlive the server
+        self._outs = {}
+
+    def submit(self, a, r, theta, offset, out, tag, star=True, map=True, lr=True):
+        """out: uint8 array (height, width, 4), kept alive until completion."""
+        assert out.dtype == np.uint8 and out.ndim == 3 and out.shape[2] == 4 and out.flags["C_CONTIGUOUS"]
+        req = _lib.BtRequest(a, r, theta, offset, out.shape[1], out.shape[0], int(star), int(map), int(lr),
+                             out.ctypes.data, tag)
+        check(_lib.load().bt_server_submit(self.handle, C.byref(req)))
+        self._outs[tag] = out
+
+    def poll(self, max_n=256):
+        buf = (_lib.BtCompletion * max_n)()
+        n = C.c_int()
+        check(_lib.load().bt_server_poll(self.handle, buf, max_n, C.byref(n)))
+        out = []
+        for k in range(n.value):
+            c = buf[k]
+            self._outs.pop(c.tag, None)
+            out.append({"tag": c.tag, "status": c.status, "grid_hit": bool(c.grid_hit),
+                        "latency_ms": c.latency_ms, "queue_ms": c.queue_ms, "frame_ms": c.frame_ms})
+        return out
+
+    def drain(self, timeout_s=0.0):
+        check(_lib.load().bt_server_drain(self.handle, timeout_s))
+
+    def stats(self):
+        st = _lib.BtServerStats()
+        check(_lib.load().bt_server_stats_get(self.handle, C.byref(st)))
+        return {n: getattr(st, n) for n, _ in _lib.BtServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().bt_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -58,8 +58,40 @@ class BtRenderFlags(C.Structure):
                 ("ver", C.c_float), ("offset", C.c_float)]
 
 
+class BtServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("slots", C.c_int), ("max_level", C.c_int),
+                ("start_level", C.c_int), ("integ", BtIntegParams), ("step", C.c_int),
+                ("view_angle", C.c_double), ("off_up", C.c_double), ("off_right", C.c_double)]
+
+
+class BtRequest(C.Structure):
+    _fields_ = [("a", C.c_double), ("r", C.c_double), ("theta", C.c_double), ("offset", C.c_float),
+                ("width", C.c_int), ("height", C.c_int), ("star", C.c_int), ("map", C.c_int),
+                ("lr", C.c_int), ("out_bgra", C.c_void_p), ("tag", C.c_uint64)]
+
+
+class BtCompletion(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("status", C.c_int), ("grid_hit", C.c_int),
+                ("latency_ms", C.c_double), ("queue_ms", C.c_double), ("frame_ms", C.c_double)]
+
+
+class BtServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("grid_builds", C.c_int64), ("grid_hits", C.c_int64), ("evictions", C.c_int64),
+                ("peak_inflight", C.c_int), ("peak_bytes", C.c_size_t), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("frames_per_s", C.c_double),
+                ("wall_s", C.c_double)]
+
+
 # name -> (restype, argtypes); every symbol here is declared in include/bt_abi.h
 SIGNATURES = {
+    "bt_server_config_default": (None, [C.POINTER(BtServerConfig)]),
+    "bt_server_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BtServerConfig), C.POINTER(C.c_void_p)]),
+    "bt_server_submit": (C.c_int, [C.c_void_p, C.POINTER(BtRequest)]),
+    "bt_server_poll": (C.c_int, [C.c_void_p, C.POINTER(BtCompletion), C.c_int, C.POINTER(C.c_int)]),
+    "bt_server_drain": (C.c_int, [C.c_void_p, C.c_double]),
+    "bt_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(BtServerStats)]),
+    "bt_server_destroy": (None, [C.c_void_p]),
     "bt_last_error": (C.c_char_p, []),
     "bt_abi_version": (C.c_int, []),
     "bt_context_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),

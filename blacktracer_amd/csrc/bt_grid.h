@@ -33,6 +33,11 @@ struct bt_grid {
     size_t bytes = 0;
     void* block = nullptr;      // the device block holding the arrays (context pool)
     size_t block_bytes = 0;
+    // FP32 dense grid the render pass reads (makeGrid's product), made once
+    // on first use; guarded by gf_mu (frames on several streams may ask).
+    mutable float* gridf = nullptr;
+    mutable int gridf_rows = 0;
+    mutable std::mutex gf_mu;
     bt::GridK k{};
     bt::PoseK pose{};
     bt::ParamsK prm{};

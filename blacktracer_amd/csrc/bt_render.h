@@ -61,5 +61,39 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
 // Host-side inputs (Viewer.h, StarProcessor.h).
 int viewer_matrix(double view_angle, double off_up, double off_right, int w, int h, int sphere, float* out);
 int star_tree(const float* stars4, int S, int tree_level, int32_t* tree, float* pos, float* mag);
+// Camera::getParamArray (Camera.h:67-78) of a pose.
+void cam7_of(const PoseK& P, float* cam7);
 
+}  // namespace bt
+
+// Device-resident render inputs behind the bt_scene / bt_view handles.
+struct bt_scene {
+    bt::Context* ctx = nullptr;
+    int32_t* tree = nullptr;
+    float* pos = nullptr;
+    float* mag = nullptr;
+    float* table = nullptr;
+    int32_t* bin_start = nullptr;
+    float* bin_pos = nullptr;
+    float* bin_mag = nullptr;
+    int tree_level = 0, S = 0, rows = 0, cols = 0;
+    size_t bytes = 0;
+    bt::SceneK k() const {
+        return bt::SceneK{tree, tree_level, pos, mag, S, table, rows, cols, bin_start, bin_pos, bin_mag};
+    }
+};
+
+struct bt_view {
+    bt::Context* ctx = nullptr;
+    float* view = nullptr;
+    int W = 0, H = 0;
+    size_t bytes = 0;
+};
+
+namespace bt {
+struct GridF;
+// The FP32 render grid of a device grid (made once per grid, thread-safe).
+int grid_f32_of(Context& c, const bt_grid* g, GridF* out);
+// Device bytes a frame of w x h pixels uses at star-splat step `step`.
+size_t frame_bytes(int w, int h, int step);
 }  // namespace bt

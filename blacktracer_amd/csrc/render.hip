@@ -934,6 +934,14 @@ int render_grid_from_cel(Context* c, const double* d_cel, int GN, int GM, float*
     return BT_OK;
 }
 
+size_t frame_bytes(int w, int h, int step) {
+    const size_t NP = (size_t)w * h, NC = (size_t)(w + 1) * (h + 1);
+    const size_t fw = 2 * (size_t)step + 1;
+    // thphi, gap | bh, area x2, star and map images, per-tap star light,
+    // heavy-pixel list, BGRA output
+    return NC * (8 + 4) + NP * (1 + 4 + 4 + 4 + 4 + 12 * fw * fw + 4 + 4) + 4096;
+}
+
 int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, const float* cam7,
                  const bt_render_flags& f, int step, uint8_t* d_out, FrameDebug* dbg) {
     const int W = v.W, H = v.H;

@@ -140,7 +140,7 @@ void star_bins(const float* pos, const float* mag, int S, int32_t* bin_start, fl
     }
 }
 
-static void cam7_of(const PoseK& P, float* cam7) {
+void cam7_of(const PoseK& P, float* cam7) {
     // Camera::getParamArray, Camera.h:67-78
     cam7[0] = (float)P.speed;
     cam7[1] = (float)P.alpha;
@@ -154,29 +154,6 @@ static void cam7_of(const PoseK& P, float* cam7) {
 }  // namespace bt
 
 using namespace bt;
-
-struct bt_scene {
-    Context* ctx = nullptr;
-    int32_t* tree = nullptr;
-    float* pos = nullptr;
-    float* mag = nullptr;
-    float* table = nullptr;
-    int32_t* bin_start = nullptr;
-    float* bin_pos = nullptr;
-    float* bin_mag = nullptr;
-    int tree_level = 0, S = 0, rows = 0, cols = 0;
-    size_t bytes = 0;
-    SceneK k() const {
-        return SceneK{tree, tree_level, pos, mag, S, table, rows, cols, bin_start, bin_pos, bin_mag};
-    }
-};
-
-struct bt_view {
-    Context* ctx = nullptr;
-    float* view = nullptr;
-    int W = 0, H = 0;
-    size_t bytes = 0;
-};
 
 static void scene_release(bt_scene* s) {
     if (s->tree) hipFree(s->tree);
@@ -343,19 +320,36 @@ void bt_view_free(bt_view* v) {
     delete v;
 }
 
-// Converts (and caches per grid) the FP64 device grid into the FP32 dense
-// grid the render kernels read.
-static int grid_f32_of(Context& c, const bt_grid* g, GridF* out) {
-    void* p;
-    int rc;
-    int sym = (float)g->M / (float)g->N > 3 ? 1 : 0;
-    int GN1 = sym ? 2 * g->N - 1 : g->N;
-    if ((rc = c.buffer("r_gridf", sizeof(float) * 2 * (size_t)GN1 * g->M, &p))) return rc;
-    int gn1 = 0;
-    if ((rc = render_grid_from_cel(&c, g->k.cel, g->N, g->M, (float*)p, &gn1))) return rc;
-    *out = GridF{(const float*)p, gn1, g->M, g->max_level};
+}  // extern "C"
+
+namespace bt {
+// The FP32 dense grid the render kernels read, converted from the FP64
+// device grid once per grid (on c's stream, which is synchronised before
+// the grid is shared) and kept with it.
+int grid_f32_of(Context& c, const bt_grid* g, GridF* out) {
+    std::lock_guard<std::mutex> lk(g->gf_mu);
+    if (!g->gridf) {
+        int sym = (float)g->M / (float)g->N > 3 ? 1 : 0;
+        int GN1 = sym ? 2 * g->N - 1 : g->N;
+        float* p = nullptr;
+        if (hipMalloc((void**)&p, sizeof(float) * 2 * (size_t)GN1 * g->M) != hipSuccess)
+            return fail(BT_ERR_NOMEM, "hipMalloc failed for the FP32 render grid");
+        int gn1 = 0;
+        int rc = render_grid_from_cel(&c, g->k.cel, g->N, g->M, p, &gn1);
+        if (rc == BT_OK && hipStreamSynchronize(c.stream) != hipSuccess) rc = fail(BT_ERR_HIP, "grid conversion failed");
+        if (rc) {
+            hipFree(p);
+            return rc;
+        }
+        g->gridf = p;
+        g->gridf_rows = gn1;
+    }
+    *out = GridF{g->gridf, g->gridf_rows, g->M, g->max_level};
     return BT_OK;
 }
+}  // namespace bt
+
+extern "C" {
 
 int bt_render_frame_stages(bt_context* h, const bt_grid* g, const bt_scene* s, const bt_view* v, const float* cam7,
                            const bt_render_flags* flags, int step, uint8_t* out_bgra, void* d_out_bgra,

@@ -17,6 +17,9 @@
  *   bt_grid_build       Grid::Grid(maxLevel, startLevel, ...)   Grid.h:605-624
  *   bt_grid_export_psh  Grid::saveAsGpuHash / PSHOffsetTable    Grid.h:642-654, PSHOffsetTable.h:350-398
  *   bt_make_image       makeImage(...)                          MakeImageKernel.cu:1206-1236, kernel.cuh:28-33
+ * Entry points with no counterpart in the reference (declared with their
+ * reason): bt_trace_rays_device, bt_grid_build_batch, the resident render
+ * objects (bt_scene, bt_view, bt_render_frame) and the server (bt_server_*).
  */
 #ifndef BT_ABI_H
 #define BT_ABI_H
@@ -222,6 +225,70 @@ int  bt_render_frame_stages(bt_context* ctx, const bt_grid* g, const bt_scene* s
                             const bt_view* v, const float* cam7, const bt_render_flags* flags,
                             int step, uint8_t* out_bgra, void* d_out_bgra, float* thphi,
                             int32_t* gap, uint8_t* bh, float* area, float* star);
+
+/* ---- serving: continuous batching of render requests (no counterpart in
+ * the reference, whose frame loop renders one frame at a time,
+ * MakeImageKernel.cu:975-1204) ----
+ * A server renders requests of any resolution and pose from one scene.  A
+ * builder thread builds and caches in HBM the grid of each new pose
+ * (Grid(max_level, start_level) at the configured integrator settings) and
+ * the view of each new resolution; a dispatcher admits pending requests in
+ * arrival order whenever one of `slots` frame slots is free and the frame's
+ * working set fits the HBM budget next to the cached grids (idle grids are
+ * evicted least recently used first).  Each slot has its own stream, so
+ * admitted frames overlap on the device.  A request's result does not
+ * depend on what else is in flight. */
+typedef struct {
+    size_t hbm_budget;        /* bytes for cached grids + views + frames in flight (16 GiB) */
+    int    slots;             /* frames in flight at most, 1..16 (4) */
+    int    max_level, start_level;   /* grids of new poses (10, 1) */
+    bt_integ_params integ;    /* integrator settings of those grids (defaults) */
+    int    step;              /* star splat step (1) */
+    double view_angle, off_up, off_right;   /* Viewer of every resolution (pi/2, 0, pi/16) */
+} bt_server_config;
+
+typedef struct {
+    double   a, r, theta;     /* pose: camera on the circular orbit at (r, theta) of spin a */
+    float    offset;          /* orbit position: the frame's celestial phi offset (>= 0) */
+    int      width, height;
+    int      star, map, lr;   /* passes, as bt_render_flags */
+    uint8_t* out_bgra;        /* width*height*4 bytes, written before completion is posted */
+    uint64_t tag;             /* caller's id, returned with the completion */
+} bt_request;
+
+typedef struct {
+    uint64_t tag;
+    int      status;          /* BT_OK or the error of the request */
+    int      grid_hit;        /* the pose's grid was already resident on arrival */
+    double   latency_ms;      /* arrival (submit) -> frame in the caller's buffer */
+    double   queue_ms;        /* arrival -> admission */
+    double   frame_ms;        /* admission -> frame in the caller's buffer */
+} bt_completion;
+
+typedef struct {
+    int64_t submitted, completed, failed;
+    int64_t grid_builds, grid_hits, evictions;
+    int     peak_inflight;
+    size_t  peak_bytes;       /* peak of cached grids + views + reserved frame bytes */
+    double  p50_ms, p99_ms, mean_ms;   /* latency of completed requests */
+    double  frames_per_s;     /* completed / (last completion - first arrival) */
+    double  wall_s;
+} bt_server_stats;
+
+typedef struct bt_server bt_server;
+
+void bt_server_config_default(bt_server_config* cfg);
+/* The scene must outlive the server. */
+int  bt_server_create(bt_context* ctx, const bt_scene* scene, const bt_server_config* cfg,
+                      bt_server** out);
+/* Non-blocking.  BT_ERR_BUDGET if the frame alone exceeds the budget. */
+int  bt_server_submit(bt_server* s, const bt_request* req);
+/* Takes up to max completions (oldest first); *n receives the count. */
+int  bt_server_poll(bt_server* s, bt_completion* out, int max, int* n);
+/* Blocks until every submitted request has completed (timeout_s <= 0: no limit). */
+int  bt_server_drain(bt_server* s, double timeout_s);
+int  bt_server_stats_get(bt_server* s, bt_server_stats* stats);
+void bt_server_destroy(bt_server* s);
 
 #ifdef __cplusplus
 }
