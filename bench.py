@@ -22,6 +22,10 @@ Also reported:
   batch      the batching config: 64 seeded poses (a in [0, 0.999], r in
              [3, 50], theta in [0.1, pi - 0.1], max level 6..11), built in
              one bt_grid_build_batch call, mixed and FP64.
+  serving    the serving config: a Poisson stream (200 requests/s) of
+             1280x720 / 1920x1080 / 3840x2160 frames (60/30/10%) of four
+             orbiting cameras through the continuous-batching server; frames/s
+             and p50/p99 latency (arrival -> frame in host memory).
 `roofline`: the dominant kernel of a step, rk_mixed (FP32 VALU): algorithmic
 FLOPs (FLOPS_PER_ATTEMPT x its attempts) per second of its time, HIP events
 on the library's stream, against the FP32 vector peak.  `render_roofline`:
@@ -29,7 +33,7 @@ algorithmic bytes of a 1080p frame per second of render-kernel time against
 8 TB/s.  `cpu_baseline`: the C oracle tracing a bounded sample of the same
 sky (every 64th ray) on the host cores.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-batch] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-batch] [--no-serve] [--no-cpu]
 """
 import argparse
 import json
@@ -58,7 +62,7 @@ ARCSEC = math.pi / 180 / 3600
 POSE = dict(a=0.999, theta=math.pi / 2, phi=0.0, r=5.0)
 LEVEL = 11
 W, H = 1920, 1080
-RENDER_KERNELS = ("pix_interp", "bh_area", "smooth", "star_light", "star_heavy", "star_sum", "env_map",
+RENDER_KERNELS = ("pix_interp", "bh_area", "smooth", "star_light", "star_heavy", "star_sum", "env_map", "env_heavy",
                   "composite")
 MIXED_KERNELS = ("ic_split", "rk_f64_warmup", "rk_mixed", "rk_f64_fallback")
 
@@ -71,6 +75,8 @@ def parse():
     ap.add_argument("--frames", type=int, default=60, help="frames for p50/p99")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-pose batch")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
 
@@ -197,6 +203,65 @@ def batch(bt, ctx):
     return out
 
 
+def serve(bt, S, ctx, seconds, rate=200.0):
+    """Poisson stream of render requests (mean `rate`/s) over four orbiting
+    cameras, resolutions 1280x720 / 1920x1080 / 3840x2160 at 60/30/10%,
+    stars and sky of the full-frame config, through the serving loop.  The
+    four grids are built (and so cached) by a first request each; the
+    timed stream then runs against a warm cache."""
+    stars = S.synthetic_stars(100_000, seed=1234)
+    sky = S.synthetic_sky(4096, seed=99)
+    sc = bt.Scene(stars, sky, tree_level=8, ctx=ctx)
+    srv = bt.Server(sc, hbm_budget=64 << 30, slots=4, max_level=10, start_level=1, ctx=ctx)
+    rng = np.random.default_rng(7)
+    traj = [(0.999, 5.0, math.pi / 2), (0.9, 8.0, 1.3), (0.5, 12.0, 1.8), (0.99, 6.0, 1.0)]
+    speed = [1.0 / float(bt.Camera(th, 0.0, r, black_hole=bt.BlackHole(a)).getParamArray()[0])
+             for a, r, th in traj]
+    res = [(1280, 720), (1920, 1080), (3840, 2160)]
+    bufs = {wh: [np.zeros((wh[1], wh[0], 4), dtype=np.uint8) for _ in range(24)] for wh in res}
+    t0 = time.perf_counter()
+    for k, pose in enumerate(traj):                       # cold: one request per pose
+        srv.submit(*pose, 0.0, bufs[res[0]][k], 10_000 + k)
+    srv.drain(300)
+    cold_s = time.perf_counter() - t0
+    srv.poll(64)
+    warm = srv.stats()
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 1.2)))
+    arrivals = arrivals[arrivals < seconds]
+    kinds = rng.choice(3, size=arrivals.size, p=[0.6, 0.3, 0.1])
+    who = rng.integers(0, len(traj), size=arrivals.size)
+    frame_no = [0] * len(traj)
+    lat, qms, fms = [], [], []
+    start = time.perf_counter()
+    for i, t_arr in enumerate(arrivals):
+        while True:
+            now = time.perf_counter() - start
+            if now >= t_arr:
+                break
+            time.sleep(min(0.0005, t_arr - now))
+        j = int(who[i])
+        wh = res[int(kinds[i])]
+        q = frame_no[j]
+        frame_no[j] += 1
+        off = 2 * math.pi * q / (0.25 * speed[j] * wh[0])   # MakeImageKernel.cu:1412-1413
+        srv.submit(*traj[j], off, bufs[wh][i % 24], i)
+        for c in srv.poll(256):
+            lat.append(c["latency_ms"]); qms.append(c["queue_ms"]); fms.append(c["frame_ms"])
+    srv.drain(600)
+    for c in srv.poll(4096):
+        lat.append(c["latency_ms"]); qms.append(c["queue_ms"]); fms.append(c["frame_ms"])
+    wall = time.perf_counter() - start
+    st = srv.stats()
+    srv.close()
+    return {"workload": f"Poisson {rate:g} req/s for {seconds:g} s, 4 orbiting cameras (Grid(10,1) each), "
+                        "1280x720/1920x1080/3840x2160 at 60/30/10%, 100k stars, 4096x2048 sky, 4 slots",
+            "requests": len(lat), "offered_per_s": arrivals.size / seconds, "frames_per_s": len(lat) / wall,
+            "latency_ms_p50": pct(lat, 0.5), "latency_ms_p99": pct(lat, 0.99), "latency_ms_mean": float(np.mean(lat)),
+            "queue_ms_p50": pct(qms, 0.5), "frame_ms_p50": pct(fms, 0.5), "frame_ms_p99": pct(fms, 0.99),
+            "cold_start_s": cold_s, "grid_builds": st["grid_builds"], "grid_hits": st["grid_hits"] - warm["grid_hits"],
+            "peak_inflight": st["peak_inflight"], "peak_hbm_gib": st["peak_bytes"] / 2 ** 30}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -309,6 +374,8 @@ def main():
     out["roofline"]["fp64_path"]["frac"] = out["roofline"]["fp64_path"]["achieved"] / PEAK_FP64_TFLOPS
     if not args.no_batch:
         out["batch"] = batch(bt, ctx)
+    if not args.no_serve:
+        out["serving"] = serve(bt, S, ctx, args.serve_seconds)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))

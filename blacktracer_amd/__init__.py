@@ -18,7 +18,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (BT_PREC_FP64, BT_PREC_MIXED, BtError, BtGridParams, BtIntegParams, BtPose,
+from ._lib import (BT_ERR_BUDGET, BT_PREC_FP64, BT_PREC_MIXED, BtError, BtGridParams, BtIntegParams, BtPose,
                    BtRenderFlags, BtTraceStats, check)
 
 __all__ = ["Context", "BlackHole", "Camera", "Grid", "IntegParams", "trace_rays", "BtError",

@@ -15,6 +15,8 @@ struct GridK {
 };
 
 int grid_alloc(Context* c, bt_grid* g);
+// Device bytes of an N x M grid with its FP32 render grid.
+size_t grid_device_bytes(int N, int M);
 int grid_build(Context* c, bt_grid* g, const bt_grid_params* gp);
 int grid_fix_tvertices(Context* c, const GridK& G);
 // Builds all grids (allocated, same integration parameters) together; the

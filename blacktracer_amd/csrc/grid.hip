@@ -669,6 +669,17 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
     return BT_OK;
 }
 
+size_t grid_device_bytes(int N, int M) {
+    const int64_t NM = (int64_t)N * M;
+    const int64_t nwords = (NM + 31) / 32;
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    // the grid's block (see grid_alloc) + its FP32 render grid
+    const int sym = (float)M / (float)N > 3 ? 1 : 0;
+    const size_t gn1 = sym ? 2 * (size_t)N - 1 : (size_t)N;
+    return up(sizeof(double) * 2 * NM) + up(sizeof(int32_t) * NM) + up((size_t)NM) + up(sizeof(uint32_t) * nwords) +
+           sizeof(float) * 2 * gn1 * M;
+}
+
 int grid_alloc(Context* c, bt_grid* g) {
     const int64_t NM = (int64_t)g->N * g->M;
     const int64_t nwords = (NM + 31) / 32;

@@ -35,6 +35,7 @@ __constant__ int kTempRGB[1173] = {
 
 struct Cam7 {
     float v[7];
+    const float* colours;   // star_colours table, 391 x 6 floats
 };
 
 __device__ __forceinline__ F2 gload(const float* grid, int GN, int GM, int i, int j) {
@@ -374,10 +375,11 @@ __device__ void lensing_redshift(const float* t, const float* p, int W, int ind,
 //
 // Work split.  Near the shadow a pixel can project onto a large part of the
 // sky with tens of thousands of candidates; one lane would serialise them and
-// stall its wavefront.  star_map_light takes pixels with at most kHeavyStars
-// candidates, one lane each, and queues the others for star_map_heavy, where
-// a 256-lane workgroup splits one pixel's candidates and reduces the taps in
-// a fixed order.
+// stall its wavefront.  star_map_light takes pixels with at most kLightStars
+// candidates (about 92% of the pixels of a lensed frame, 6% of the
+// candidates), one lane each, and queues the others for star_map_wave, where
+// one wavefront splits a pixel's candidates and reduces the taps in a fixed
+// order.
 //
 // Per tap the reference evaluates 100 * 10^(-0.4 (m - 2.5 log10 x)) with
 // x = frac * gaussian(d) and converts HSP -> RGB at brightness P * that.
@@ -387,9 +389,23 @@ __device__ void lensing_redshift(const float* t, const float* p, int W, int ind,
 // reference) take the reference's own expression.  The star light agrees
 // with the reference arithmetic to a few ulps up to summation order; frames
 // to within 1/255.
-constexpr int kHeavyStars = 512;
+constexpr int kLightStars = 16;
+
+// Appends `value` to list for every lane with `take` set: one atomic per
+// wavefront.  Every active lane of the wave must call it.
+__device__ __forceinline__ void wave_append(uint32_t* list, unsigned* count, bool take, uint32_t value) {
+    const unsigned lane = __lane_id();
+    unsigned long long m = __ballot(take);
+    if (!m) return;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane == (unsigned)leader) base = atomicAdd(count, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (take) list[base + (unsigned)__popcll(m & ((1ull << lane) - 1))] = value;
+}
 
 struct StarPix {
+    const float* colours;   // star_colours table (6 floats per blackbody index)
     float t[4], p[4];
     float mn[2], mx[2];
     int sgn;
@@ -397,10 +413,48 @@ struct StarPix {
     float frac, redshft, red;
 };
 
+// Per blackbody index: rgb_to_hsp of the table colour (H, S, P) and
+// hsp_to_rgb of that at the same P (R, G, B) -- what every star of that
+// colour would compute (kernel.cuh:451-468), once.
+__global__ void star_colours(float* out) {
+    int index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index > 390) return;
+    float hh, ss, pp, r1, g1, b1;
+    rgb_to_hsp((float)kTempRGB[3 * index] / 255.f, (float)kTempRGB[3 * index + 1] / 255.f,
+               (float)kTempRGB[3 * index + 2] / 255.f, hh, ss, pp);
+    hsp_to_rgb(hh, ss, pp, r1, g1, b1);
+    float* o = out + 6 * index;
+    o[0] = hh;
+    o[1] = ss;
+    o[2] = pp;
+    o[3] = r1;
+    o[4] = g1;
+    o[5] = b1;
+}
+
+// Corners, bounding box and orientation of pixel (i, j) (no lensing).
+__device__ void star_pixel_geom(const float* thphi, int W, int i, int j, float offset, const Cam7& cam,
+                                StarPix& P) {
+    P.colours = cam.colours;
+    const int ind = i * (W + 1) + j;
+    P.picheck = false;
+    corners(thphi, W, ind, P.t, P.p, P.picheck, offset);
+    const float* t = P.t;
+    const float* p = P.p;
+    P.mx[0] = fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3]));
+    P.mx[1] = fmaxf(fmaxf(p[0], p[1]), fmaxf(p[2], p[3]));
+    P.mn[0] = fminf(fminf(t[0], t[1]), fminf(t[2], t[3]));
+    P.mn[1] = fminf(fminf(p[0], p[1]), fminf(p[2], p[3]));
+    float orient = (t[1] - t[0]) * (p[1] + p[0]) + (t[2] - t[1]) * (p[2] + p[1]) + (t[3] - t[2]) * (p[3] + p[2]) +
+                   (t[0] - t[3]) * (p[0] + p[3]);
+    P.sgn = orient < 0 ? -1 : 1;
+}
+
 __device__ bool star_pixel(const float* thphi, const uint8_t* bh, int W, int i, int j, float offset, int lr,
                            const float* area, const float* view, const Cam7& cam, StarPix& P) {
     const size_t ij = (size_t)i * W + j;
     if (bh[ij] != 0) return false;
+    P.colours = cam.colours;
     const int ind = i * (W + 1) + j;
     P.picheck = false;
     corners(thphi, W, ind, P.t, P.p, P.picheck, offset);
@@ -518,11 +572,9 @@ __device__ __forceinline__ void star_accumulate(const StarPix& P, const float* p
     float temp = 46.f / P.redshft * ((1.f / ((0.92f * mg.y) + 1.7f)) + (1.f / ((0.92f * mg.y) + 0.62f))) - 10.f;
     // clamp to the table's last entry (the reference's 1170 reads past it)
     int index = max(0, min(cvt_i32(floorf(temp)), 390));
-    float hh, ss, pp;
-    rgb_to_hsp((float)kTempRGB[3 * index] / 255.f, (float)kTempRGB[3 * index + 1] / 255.f,
-               (float)kTempRGB[3 * index + 2] / 255.f, hh, ss, pp);
-    float r1, g1, b1;
-    hsp_to_rgb(hh, ss, pp, r1, g1, b1);
+    // the blackbody colour's HSP and its RGB at unit brightness, per index
+    const float* col = P.colours + 6 * index;
+    const float hh = col[0], ss = col[1], pp = col[2], r1 = col[3], g1 = col[4], b1 = col[5];
     const float E = f_exp10(-.4f * part);
     const float maxDistSq = (STEP + .5f) * (STEP + .5f);
 #pragma unroll
@@ -554,11 +606,54 @@ __device__ __forceinline__ void star_accumulate(const StarPix& P, const float* p
         }
 }
 
+// Pixels with more than kLightStars candidates are queued with their
+// candidate count; their candidates are cut into chunks of CH (a multiple of
+// 64, at least kChunk, grown so that the chunks fit the scratch), one
+// wavefront per chunk, and each pixel's chunk partials are summed in chunk
+// order afterwards.  Every queued star is then one of a few candidates of
+// one lane, whatever the pixel it lands in: the long per-star chain
+// (bisection, taps) of a pixel with tens of thousands of candidates is
+// spread over many waves instead of serialised on a few lanes.
+constexpr int kChunk = 256;
+
+struct StarQueue {
+    uint32_t* pix;         // queued pixel indices
+    uint32_t* cnt;         // their candidate counts
+    uint32_t* cstart;      // first chunk of each queued pixel (exclusive scan)
+    uint32_t* owner;       // queued pixel of each chunk
+    float* lens;           // per queued pixel: frac, redshift, 4 log10(redshift)
+    unsigned* ctr;         // [0] queued pixels, [1] chunks, [2] chunk size
+    float* partial;        // per chunk: FW*FW*3 partial taps
+    unsigned cap;          // chunk capacity of `partial`
+};
+
+// All candidates of a pixel in order, into acc (the reference's loop).
+template <int STEP>
+__device__ __forceinline__ void star_seq(const StarPix& P, const SceneK& s,
+                                         float (&acc)[(2 * STEP + 1) * (2 * STEP + 1)][3]) {
+    const float* pos = P.picheck ? s.star_pos : s.bin_pos;
+    const float* mag = P.picheck ? s.star_mag : s.bin_mag;
+    for_ranges(P, s, [&](int b, int e) {
+        for (int q = b; q < e; q++) star_accumulate<STEP>(P, pos, mag, q, acc);
+    });
+}
+
+template <int STEP>
+__device__ __forceinline__ void store_taps(float* taps, size_t NP, size_t ij,
+                                           const float (&acc)[(2 * STEP + 1) * (2 * STEP + 1)][3]) {
+    // tap-major layout: taps[(tap * 3 + c) * NP + ij]
+#pragma unroll
+    for (int u = 0; u < (2 * STEP + 1) * (2 * STEP + 1); u++) {
+        taps[(size_t)(u * 3 + 0) * NP + ij] = acc[u][0];
+        taps[(size_t)(u * 3 + 1) * NP + ij] = acc[u][1];
+        taps[(size_t)(u * 3 + 2) * NP + ij] = acc[u][2];
+    }
+}
+
 template <int STEP>
 __global__ __launch_bounds__(256) void star_map_light(const float* thphi, const uint8_t* bh, SceneK s, Cam7 cam,
                                                       int W, int H, float offset, int lr, const float* area,
-                                                      const float* view, float* taps, uint32_t* heavy,
-                                                      unsigned* nheavy) {
+                                                      const float* view, float* taps, StarQueue Q) {
     constexpr int FW = 2 * STEP + 1;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y * blockDim.y + threadIdx.y;
@@ -569,93 +664,196 @@ __global__ __launch_bounds__(256) void star_map_light(const float* thphi, const 
 #pragma unroll
     for (int u = 0; u < FW * FW; u++) acc[u][0] = acc[u][1] = acc[u][2] = 0.f;
     StarPix P;
+    int count = 0;
     if (star_pixel(thphi, bh, W, i, j, offset, lr, area, view, cam, P)) {
-        int count = 0;
         for_ranges(P, s, [&](int b, int e) { count += e - b; });
-        if (count > kHeavyStars) {
-            heavy[atomicAdd(nheavy, 1u)] = (uint32_t)ij;
-        } else if (count > 0) {
-            const float* pos = P.picheck ? s.star_pos : s.bin_pos;
-            const float* mag = P.picheck ? s.star_mag : s.bin_mag;
-            for_ranges(P, s, [&](int b, int e) {
-                for (int q = b; q < e; q++) star_accumulate<STEP>(P, pos, mag, q, acc);
-            });
+        if (count <= kLightStars && count > 0) star_seq<STEP>(P, s, acc);
+    }
+    const bool queue = count > kLightStars;
+    // wave-aggregated append of (pixel, count)
+    const unsigned lane = __lane_id();
+    unsigned long long m = __ballot(queue);
+    if (m) {
+        const int leader = __ffsll((long long)m) - 1;
+        unsigned base = 0;
+        if (lane == (unsigned)leader) base = atomicAdd(&Q.ctr[0], (unsigned)__popcll(m));
+        base = __shfl(base, leader);
+        if (queue) {
+            unsigned k = base + (unsigned)__popcll(m & ((1ull << lane) - 1));
+            Q.pix[k] = (uint32_t)ij;
+            Q.cnt[k] = (uint32_t)count;
+            Q.lens[3 * k] = P.frac;
+            Q.lens[3 * k + 1] = P.redshft;
+            Q.lens[3 * k + 2] = P.red;
         }
     }
-    // tap-major layout: taps[(tap * 3 + c) * NP + ij]
-#pragma unroll
-    for (int u = 0; u < FW * FW; u++) {
-        taps[(size_t)(u * 3 + 0) * NP + ij] = acc[u][0];
-        taps[(size_t)(u * 3 + 1) * NP + ij] = acc[u][1];
-        taps[(size_t)(u * 3 + 2) * NP + ij] = acc[u][2];
+    if (!queue) store_taps<STEP>(taps, NP, ij, acc);
+}
+
+// One workgroup: chunk size (the smallest multiple of 64 >= kChunk that
+// keeps the chunk count within cap), and the exclusive scan of the chunk
+// counts of the queued pixels.
+__global__ __launch_bounds__(1024) void star_chunk_scan(StarQueue Q) {
+    __shared__ unsigned long long s_sum[1024];
+    const unsigned n = Q.ctr[0];
+    const unsigned t = threadIdx.x, T = blockDim.x;
+    const unsigned per = (n + T - 1) / T;
+    const unsigned lo = min(n, t * per), hi = min(n, lo + per);
+    // total candidates -> chunk size
+    unsigned long long c = 0;
+    for (unsigned k = lo; k < hi; k++) c += Q.cnt[k];
+    s_sum[t] = c;
+    __syncthreads();
+    for (unsigned off = T / 2; off > 0; off >>= 1) {
+        if (t < off) s_sum[t] += s_sum[t + off];
+        __syncthreads();
+    }
+    const unsigned long long total = s_sum[0];
+    __syncthreads();
+    unsigned long long ch = kChunk;
+    // chunks <= total / ch + n; grow ch until that fits cap (n <= cap - 64,
+    // so this ends once total / ch < 64)
+    while (total / ch + n > Q.cap) ch += 64;
+    // per-pixel chunk counts, exclusive scan
+    unsigned long long loc = 0;
+    for (unsigned k = lo; k < hi; k++) loc += (Q.cnt[k] + ch - 1) / ch;
+    s_sum[t] = loc;
+    __syncthreads();
+    for (unsigned off = 1; off < T; off <<= 1) {   // Hillis-Steele inclusive scan
+        unsigned long long v = t >= off ? s_sum[t - off] : 0ull;
+        __syncthreads();
+        s_sum[t] += v;
+        __syncthreads();
+    }
+    unsigned long long run = s_sum[t] - loc;
+    for (unsigned k = lo; k < hi; k++) {
+        Q.cstart[k] = (uint32_t)run;
+        run += (Q.cnt[k] + ch - 1) / ch;
+    }
+    if (t == T - 1) {
+        Q.ctr[1] = (unsigned)s_sum[T - 1];
+        Q.ctr[2] = (unsigned)ch;
     }
 }
 
-// One workgroup per queued pixel: lanes stride over the pixel's candidates,
-// partial taps are reduced in a fixed order.
+// owner[c] = the queued pixel of chunk c (one thread per queued pixel).
+__global__ __launch_bounds__(256) void star_chunk_owner(StarQueue Q) {
+    const unsigned n = Q.ctr[0], nch = Q.ctr[1];
+    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const unsigned c1 = k + 1 < n ? Q.cstart[k + 1] : nch;
+        for (unsigned c = Q.cstart[k]; c < c1; c++) Q.owner[c] = k;
+    }
+}
+
+// One wavefront per chunk (grid-stride): lane l takes the chunk's
+// candidates l, l + 64, ...; the taps are reduced over the wave by a fixed
+// xor butterfly into the chunk's partial.  For a binned pixel the wave
+// walks the footprint's bin rows 64 at a time: each lane loads one row's
+// range, a wave scan gives the rows' first candidate numbers (kept in LDS),
+// and each lane finds the row of each of its candidates by bisection.
 template <int STEP>
-__global__ __launch_bounds__(256) void star_map_heavy(const float* thphi, const uint8_t* bh, SceneK s, Cam7 cam,
-                                                      int W, int H, float offset, int lr, const float* area,
-                                                      const float* view, float* taps, const uint32_t* heavy,
-                                                      const unsigned* nheavy) {
+__global__ __launch_bounds__(256) void star_map_chunk(const float* thphi, const uint8_t* bh, SceneK s, Cam7 cam,
+                                                      int W, float offset, int lr, const float* area,
+                                                      const float* view, StarQueue Q) {
     constexpr int FW = 2 * STEP + 1;
     constexpr int NT = FW * FW * 3;
-    __shared__ int s_b[2048];
-    __shared__ int s_pref[2049];
-    __shared__ int s_n;
-    __shared__ float s_red[4][NT];
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6;
-    const size_t NP = (size_t)W * H;
-    const unsigned nh = *nheavy;
-    for (unsigned hidx = blockIdx.x; hidx < nh; hidx += gridDim.x) {
-        const uint32_t ij = heavy[hidx];
+    const unsigned lane = threadIdx.x & 63;
+    const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const unsigned nwaves = (gridDim.x * blockDim.x) >> 6;
+    const unsigned n = Q.ctr[0], nch = Q.ctr[1], CH = Q.ctr[2];
+    __shared__ int s_beg[4][64];
+    __shared__ unsigned s_pre[4][65];
+    const unsigned wslot = threadIdx.x >> 6;
+    (void)n;
+    for (unsigned c = wave; c < nch; c += nwaves) {
+        const unsigned lo = Q.owner[c];
+        const uint32_t ij = Q.pix[lo];
+        const unsigned g0 = (c - Q.cstart[lo]) * CH;
+        const unsigned g1 = min(g0 + CH, Q.cnt[lo]);
         const int i = (int)(ij / (uint32_t)W), j = (int)(ij % (uint32_t)W);
         StarPix P;
-        star_pixel(thphi, bh, W, i, j, offset, lr, area, view, cam, P);
-        if (tid == 0) {
-            int n = 0, acc_n = 0;
-            for_ranges(P, s, [&](int b, int e) {
-                if (n < 2048 && e > b) {
-                    s_b[n] = b;
-                    s_pref[n] = acc_n;
-                    acc_n += e - b;
-                    n++;
-                }
-            });
-            s_pref[n] = acc_n;
-            s_n = n;
-        }
-        __syncthreads();
-        const int n = s_n;
-        const int total = s_pref[n];
+        star_pixel_geom(thphi, W, i, j, offset, cam, P);
+        P.frac = Q.lens[3 * lo];
+        P.redshft = Q.lens[3 * lo + 1];
+        P.red = Q.lens[3 * lo + 2];
         const float* pos = P.picheck ? s.star_pos : s.bin_pos;
         const float* mag = P.picheck ? s.star_mag : s.bin_mag;
         float acc[FW * FW][3];
 #pragma unroll
         for (int u = 0; u < FW * FW; u++) acc[u][0] = acc[u][1] = acc[u][2] = 0.f;
-        int cur = 0;
-        for (int k = tid; k < total; k += blockDim.x) {
-            while (cur + 1 < n && s_pref[cur + 1] <= k) cur++;
-            star_accumulate<STEP>(P, pos, mag, s_b[cur] + (k - s_pref[cur]), acc);
+        if (P.picheck) {
+            unsigned base = 0;   // candidates before the current range
+            for_ranges(P, s, [&](int b, int e) {
+                const unsigned len = (unsigned)(e - b);
+                const unsigned a0 = max(base, g0), a1 = min(base + len, g1);
+                for (unsigned g = a0 + lane; g < a1; g += 64)
+                    star_accumulate<STEP>(P, pos, mag, b + (int)(g - base), acc);
+                base += len;
+            });
+        } else {
+            const int r0 = bin_of(P.mn[0], kBinDT, kBinT), r1 = bin_of(P.mx[0], kBinDT, kBinT);
+            const int p0 = bin_of(P.mn[1], kBinDP, kBinP), p1 = bin_of(P.mx[1], kBinDP, kBinP);
+            unsigned base = 0;
+            for (int rb = r0; rb <= r1 && base < g1; rb += 64) {
+                const int r = rb + (int)lane;
+                int b = 0, e = 0;
+                if (r <= r1) {
+                    b = s.bin_start[r * kBinP + p0];
+                    e = s.bin_start[r * kBinP + p1 + 1];
+                }
+                unsigned len = (unsigned)(e - b), inc = len;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    unsigned o = __shfl_up(inc, off);
+                    if ((int)lane >= off) inc += o;
+                }
+                s_beg[wslot][lane] = b;
+                s_pre[wslot][lane + 1] = inc;
+                if (lane == 0) s_pre[wslot][0] = 0;
+                const unsigned tot = __shfl(inc, 63);
+                __builtin_amdgcn_wave_barrier();
+                const unsigned a0 = max(base, g0), a1 = min(base + tot, g1);
+                for (unsigned g = a0 + lane; g < a1; g += 64) {
+                    const unsigned x = g - base;   // candidate number within this row block
+                    int lo2 = 0, hi2 = 63;         // row k with s_pre[k] <= x < s_pre[k+1]
+                    while (lo2 < hi2) {
+                        int mid = (lo2 + hi2 + 1) >> 1;
+                        if (s_pre[wslot][mid] <= x) lo2 = mid;
+                        else hi2 = mid - 1;
+                    }
+                    star_accumulate<STEP>(P, pos, mag, s_beg[wslot][lo2] + (int)(x - s_pre[wslot][lo2]), acc);
+                }
+                __builtin_amdgcn_wave_barrier();
+                base += tot;
+            }
         }
-        // xor butterfly within each wave, then the 4 waves in order
+        float* out = Q.partial + (size_t)c * NT;
 #pragma unroll
         for (int u = 0; u < FW * FW; u++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                float v = acc[u][c];
+            for (int cc = 0; cc < 3; cc++) {
+                float v = acc[u][cc];
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                if ((tid & 63) == 0) s_red[wave][3 * u + c] = v;
+                if (lane == 0) out[3 * u + cc] = v;
             }
-        __syncthreads();
-        if (tid < NT) {
-            int u = tid / 3, c = tid % 3;
-            float v = ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
-            taps[(size_t)(u * 3 + c) * NP + ij] = v;
+    }
+}
+
+// One thread per queued pixel: its chunk partials summed in chunk order.
+template <int STEP>
+__global__ __launch_bounds__(256) void star_chunk_reduce(int W, int H, float* taps, StarQueue Q) {
+    constexpr int NT = (2 * STEP + 1) * (2 * STEP + 1) * 3;
+    const unsigned n = Q.ctr[0], nch = Q.ctr[1];
+    const size_t NP = (size_t)W * H;
+    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const unsigned c0 = Q.cstart[k], c1 = k + 1 < n ? Q.cstart[k + 1] : nch;
+        const uint32_t ij = Q.pix[k];
+        for (int tix = 0; tix < NT; tix++) {
+            float v = 0.f;
+            for (unsigned c = c0; c < c1; c++) v += Q.partial[(size_t)c * NT + tix];
+            taps[(size_t)tix * NP + ij] = v;
         }
-        __syncthreads();
     }
 }
 
@@ -705,109 +903,253 @@ __global__ __launch_bounds__(256) void star_sum(const float* taps, int W, int H,
 }
 
 // distortEnvironmentMap (MakeImageKernel.cu:631-751)
-__global__ __launch_bounds__(256) void env_map(const float* thphi, const uint8_t* bh, SceneK s, Cam7 cam, int W,
-                                               int H, float offset, int lr, const float* area,
-                                               const float* view, uint8_t* img) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    int i = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= H || j >= W) return;
-    const size_t ij = (size_t)i * W + j;
-    const int rows = s.rows, cols = s.cols;
-    const float4* table = reinterpret_cast<const float4*>(s.table);
-    float cx = 0.f, cy = 0.f, cz = 0.f, cw = 0.f;
-    if (bh[ij] == 0) {
-        float t[4], p[4];
-        int ind = i * (W + 1) + j;
-        bool picheck = false;
-        corners(thphi, W, ind, t, p, picheck, offset);
-        if (ind > 0) {
-            float pixSize = BT_FPI / (float)rows;
-            float phMax = fmaxf(fmaxf(p[0], p[1]), fmaxf(p[2], p[3]));
-            float phMin = fminf(fminf(p[0], p[1]), fminf(p[2], p[3]));
-            int pixMax = cvt_i32(phMax / pixSize);
-            int pixMin = cvt_i32(phMin / pixSize);
-            int pixNum = pixMax - pixMin + 1;
-            if (pixNum > 1 && pixMax * pixSize > phMax) pixNum -= 1;
-            float thMax = fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3]));
-            float thMin = fminf(fminf(t[0], t[1]), fminf(t[2], t[3]));
-            if (pixNum == 1) {
-                int rmax = min(max(cvt_i32(thMax / pixSize), 0), rows - 1);
-                int rmin = cvt_i32(thMin / pixSize) - 1;
-                int col = min(max(pixMin, 0), cols - 1);
-                float4 mxc = table[(size_t)rmax * cols + col];
-                float4 mnc = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (rmin >= 0) mnc = table[(size_t)min(rmin, rows - 1) * cols + col];
-                cx += mxc.x - mnc.x;
-                cy += mxc.y - mnc.y;
-                cz += mxc.z - mnc.z;
-                cw += mxc.w - mnc.w;
-            } else {
-                float max1 = -100, min1 = cols + 1, max2 = -100, min2 = cols + 1;
-                if (pixNum < cols) {
-                    for (int sidx = 0; sidx <= pixNum; sidx++) {
-                        float cp = (pixMin + sidx) * pixSize;
-                        for (int q = 0; q < 4; q++) {
-                            float ap = p[q], bp = p[(q + 1) % 4];
-                            float at = t[q], bt = t[(q + 1) % 4];
-                            if ((cp - ap) * (cp - bp) <= 0) {
-                                float ct = at + (bt - at) / (bp - ap) * (cp - ap);
-                                min2 = ct < min2 ? ct : min2;
-                                max2 = ct > max2 ? ct : max2;
-                            }
-                        }
-                        if (sidx > 0) {
-                            float max_ = fmaxf(max1, max2);
-                            float min_ = fminf(min1, min2);
-                            int rmax = min(max(cvt_i32(max_ / pixSize), 0), rows - 1);
-                            int col = (pixMin + sidx) % cols;
-                            float4 mxc = table[(size_t)rmax * cols + col];
-                            int index2 = (cvt_i32(min_ / pixSize) - 1) * cols + col;
-                            float4 mnc = make_float4(0.f, 0.f, 0.f, 0.f);
-                            if (index2 > 0) {
-                                if (index2 >= rows * cols) index2 = (rows - 1) * cols + col;
-                                mnc = table[index2];
-                            }
-                            cx += mxc.x - mnc.x;
-                            cy += mxc.y - mnc.y;
-                            cz += mxc.z - mnc.z;
-                            cw += mxc.w - mnc.w;
-                        }
-                        max1 = max2;
-                        min1 = min2;
-                    }
-                } else {
-                    int thMaxPix = min(max(cvt_i32(thMax / pixSize), 0), rows - 1) * cols;
-                    int thMinPix = min(max(cvt_i32(thMin / pixSize), 0), rows - 1) * cols;
-                    for (int q = 0; q < pixNum; q++) {
-                        float4 mxc = table[thMaxPix + (pixMin + q) % cols];
-                        float4 mnc = table[thMinPix + (pixMin + q) % cols];
-                        cx += mxc.x - mnc.x;
-                        cy += mxc.y - mnc.y;
-                        cz += mxc.z - mnc.z;
-                        cw += mxc.w - mnc.w;
-                    }
-                }
-            }
-            cx = fminf(255.f, f_pow(cx / cw, 1.f / 2.2f));
-            cy = fminf(255.f, f_pow(cy / cw, 1.f / 2.2f));
-            cz = fminf(255.f, f_pow(cz / cw, 1.f / 2.2f));
-            float Hh, Ss, Pp;
-            rgb_to_hsp(cz / 255.f, cy / 255.f, cx / 255.f, Hh, Ss, Pp);
-            if (lr) {
-                float redshft, frac;
-                lensing_redshift(t, p, W, ind, cam, view, frac, redshft, area[ij]);
-                Pp *= frac;
-                Pp = f_pow(Pp, redshft);
-            }
-            hsp_to_rgb(Hh, Ss, fminf(1.f, Pp), cz, cy, cx);
+// ---- distortEnvironmentMap (MakeImageKernel.cu:631-751) ----
+// Pixels whose celestial footprint spans at most kHeavyCols sky columns
+// (about 90% of them) are done one per lane by env_map; those with up to
+// kLaneCols queue for env_map_lane (one per lane, compacted, so that a wave's
+// lanes have similar work) and the rest for env_map_heavy, one wavefront
+// per pixel, so that a pixel on the Einstein ring with thousands of columns
+// does not stall its neighbours' wave.
+
+constexpr int kHeavyCols = 32;
+
+struct EnvPix {
+    float t[4], p[4];
+    int ind, pixMin, pixNum;
+    float pixSize, thMax, thMin;
+};
+
+// Corners and column range of pixel (i, j); false for pixel index 0 (the
+// reference's `ind > 0`, MakeImageKernel.cu:647), which stays black.
+__device__ __forceinline__ bool env_setup(const float* thphi, int W, int i, int j, float offset, int rows,
+                                          EnvPix& E) {
+    bool picheck = false;
+    E.ind = i * (W + 1) + j;
+    corners(thphi, W, E.ind, E.t, E.p, picheck, offset);
+    if (E.ind <= 0) return false;
+    E.pixSize = BT_FPI / (float)rows;
+    float phMax = fmaxf(fmaxf(E.p[0], E.p[1]), fmaxf(E.p[2], E.p[3]));
+    float phMin = fminf(fminf(E.p[0], E.p[1]), fminf(E.p[2], E.p[3]));
+    int pixMax = cvt_i32(phMax / E.pixSize);
+    E.pixMin = cvt_i32(phMin / E.pixSize);
+    E.pixNum = pixMax - E.pixMin + 1;
+    if (E.pixNum > 1 && pixMax * E.pixSize > phMax) E.pixNum -= 1;
+    E.thMax = fmaxf(fmaxf(E.t[0], E.t[1]), fmaxf(E.t[2], E.t[3]));
+    E.thMin = fminf(fminf(E.t[0], E.t[1]), fminf(E.t[2], E.t[3]));
+    return true;
+}
+
+// The quad's theta span on the column boundary at phi = cp, folded into the
+// running (mn, mx) (the reference never resets it between boundaries).
+__device__ __forceinline__ void env_boundary(const EnvPix& E, float cp, float& mn, float& mx) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float ap = E.p[q], bp = E.p[(q + 1) % 4];
+        float at = E.t[q], bt = E.t[(q + 1) % 4];
+        if ((cp - ap) * (cp - bp) <= 0) {
+            float ct = at + (bt - at) / (bp - ap) * (cp - ap);
+            mn = ct < mn ? ct : mn;
+            mx = ct > mx ? ct : mx;
         }
     }
+}
+
+// Summed-table difference of column `col` over the running span.
+__device__ __forceinline__ float4 env_column(const float4* table, int rows, int cols, float pixSize, int col,
+                                             float mn, float mx) {
+    int rmax = min(max(cvt_i32(mx / pixSize), 0), rows - 1);
+    float4 mxc = table[(size_t)rmax * cols + col];
+    int index2 = (cvt_i32(mn / pixSize) - 1) * cols + col;
+    float4 mnc = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (index2 > 0) {
+        if (index2 >= rows * cols) index2 = (rows - 1) * cols + col;
+        mnc = table[index2];
+    }
+    return make_float4(mxc.x - mnc.x, mxc.y - mnc.y, mxc.z - mnc.z, mxc.w - mnc.w);
+}
+
+// Mean colour -> gamma -> HSP -> lensing/redshift -> BGRA (MakeImageKernel.cu:733-750).
+__device__ __forceinline__ uchar4 env_finish(const EnvPix& E, float cx, float cy, float cz, float cw, int W,
+                                             int lr, const Cam7& cam, const float* view, float area_ij) {
+    cx = fminf(255.f, f_pow(cx / cw, 1.f / 2.2f));
+    cy = fminf(255.f, f_pow(cy / cw, 1.f / 2.2f));
+    cz = fminf(255.f, f_pow(cz / cw, 1.f / 2.2f));
+    float Hh, Ss, Pp;
+    rgb_to_hsp(cz / 255.f, cy / 255.f, cx / 255.f, Hh, Ss, Pp);
+    if (lr) {
+        float redshft, frac;
+        lensing_redshift(E.t, E.p, W, E.ind, cam, view, frac, redshft, area_ij);
+        Pp *= frac;
+        Pp = f_pow(Pp, redshft);
+    }
+    hsp_to_rgb(Hh, Ss, fminf(1.f, Pp), cz, cy, cx);
     uchar4 o;
     o.x = (uint8_t)min(255, cvt_i32(cx * 255));
     o.y = (uint8_t)min(255, cvt_i32(cy * 255));
     o.z = (uint8_t)min(255, cvt_i32(cz * 255));
     o.w = 255;
+    return o;
+}
+
+// Column sums of a pixel in the reference's order (one lane).
+__device__ __forceinline__ float4 env_seq(const EnvPix& E, const float4* table, int rows, int cols) {
+    float cx = 0.f, cy = 0.f, cz = 0.f, cw = 0.f;
+    if (E.pixNum == 1) {
+        int rmax = min(max(cvt_i32(E.thMax / E.pixSize), 0), rows - 1);
+        int rmin = cvt_i32(E.thMin / E.pixSize) - 1;
+        int col = min(max(E.pixMin, 0), cols - 1);
+        float4 mxc = table[(size_t)rmax * cols + col];
+        float4 mnc = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (rmin >= 0) mnc = table[(size_t)min(rmin, rows - 1) * cols + col];
+        cx = mxc.x - mnc.x;
+        cy = mxc.y - mnc.y;
+        cz = mxc.z - mnc.z;
+        cw = mxc.w - mnc.w;
+    } else if (E.pixNum < cols) {
+        float mx = -100, mn = cols + 1;
+        for (int sidx = 0; sidx <= E.pixNum; sidx++) {
+            env_boundary(E, (E.pixMin + sidx) * E.pixSize, mn, mx);
+            if (sidx > 0) {
+                float4 d = env_column(table, rows, cols, E.pixSize, (E.pixMin + sidx) % cols, mn, mx);
+                cx += d.x;
+                cy += d.y;
+                cz += d.z;
+                cw += d.w;
+            }
+        }
+    } else {
+        int thMaxPix = min(max(cvt_i32(E.thMax / E.pixSize), 0), rows - 1) * cols;
+        int thMinPix = min(max(cvt_i32(E.thMin / E.pixSize), 0), rows - 1) * cols;
+        for (int q = 0; q < E.pixNum; q++) {
+            float4 mxc = table[thMaxPix + (E.pixMin + q) % cols];
+            float4 mnc = table[thMinPix + (E.pixMin + q) % cols];
+            cx += mxc.x - mnc.x;
+            cy += mxc.y - mnc.y;
+            cz += mxc.z - mnc.z;
+            cw += mxc.w - mnc.w;
+        }
+    }
+    return make_float4(cx, cy, cz, cw);
+}
+
+constexpr int kLaneCols = 256;
+
+__global__ __launch_bounds__(256) void env_map(const float* thphi, const uint8_t* bh, SceneK s, Cam7 cam, int W,
+                                               int H, float offset, int lr, const float* area,
+                                               const float* view, uint8_t* img, uint32_t* qlane, uint32_t* qwave,
+                                               unsigned* nq) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= H || j >= W) return;
+    const size_t ij = (size_t)i * W + j;
+    const float4* table = reinterpret_cast<const float4*>(s.table);
+    uchar4 o = make_uchar4(0, 0, 0, 255);
+    EnvPix E;
+    const bool live = bh[ij] == 0 && env_setup(thphi, W, i, j, offset, s.rows, E);
+    const int tier = !live || E.pixNum <= kHeavyCols ? 0 : E.pixNum <= kLaneCols ? 1 : 2;
+    wave_append(qlane, &nq[0], tier == 1, (uint32_t)ij);
+    wave_append(qwave, &nq[1], tier == 2, (uint32_t)ij);
+    if (tier != 0) return;   // env_map_lane / env_map_heavy write this pixel
+    if (live) {
+        float4 c4 = env_seq(E, table, s.rows, s.cols);
+        o = env_finish(E, c4.x, c4.y, c4.z, c4.w, W, lr, cam, view, area[ij]);
+    }
     reinterpret_cast<uchar4*>(img)[ij] = o;
+}
+
+// One queued pixel per lane, in the reference's column order.
+__global__ __launch_bounds__(256) void env_map_lane(const float* thphi, SceneK s, Cam7 cam, int W, float offset,
+                                                    int lr, const float* area, const float* view, uint8_t* img,
+                                                    const uint32_t* q, const unsigned* nq) {
+    const unsigned n = *nq;
+    const float4* table = reinterpret_cast<const float4*>(s.table);
+    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const uint32_t ij = q[e];
+        EnvPix E;
+        env_setup(thphi, W, (int)(ij / (uint32_t)W), (int)(ij % (uint32_t)W), offset, s.rows, E);
+        float4 c4 = env_seq(E, table, s.rows, s.cols);
+        reinterpret_cast<uchar4*>(img)[ij] = env_finish(E, c4.x, c4.y, c4.z, c4.w, W, lr, cam, view, area[ij]);
+    }
+}
+
+__device__ __forceinline__ float wave_xor_sum(float v) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// One wavefront per queued pixel (grid-stride over the queue).  Lanes take
+// contiguous chunks of the column boundaries; the running theta span is an
+// exact min/max, so a lane starts its chunk from the min/max of all earlier
+// boundaries (a wave scan) and sees the spans the sequential loop sees.
+// Column terms are summed per lane and reduced in a fixed order.
+__global__ __launch_bounds__(256) void env_map_heavy(const float* thphi, SceneK s, Cam7 cam, int W,
+                                                     float offset, int lr, const float* area, const float* view,
+                                                     uint8_t* img, const uint32_t* heavy, const unsigned* nheavy) {
+    const unsigned lane = threadIdx.x & 63;
+    const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const unsigned nwaves = (gridDim.x * blockDim.x) >> 6;
+    const unsigned n = *nheavy;
+    const int rows = s.rows, cols = s.cols;
+    const float4* table = reinterpret_cast<const float4*>(s.table);
+    for (unsigned e = wave; e < n; e += nwaves) {
+        const uint32_t ij = heavy[e];
+        const int i = (int)(ij / (uint32_t)W), j = (int)(ij % (uint32_t)W);
+        EnvPix E;
+        env_setup(thphi, W, i, j, offset, rows, E);
+        float cx = 0.f, cy = 0.f, cz = 0.f, cw = 0.f;
+        if (E.pixNum < cols) {
+            const int nb = E.pixNum + 1;   // boundaries 0..pixNum
+            const int per = (nb + 63) / 64;
+            const int lo = min((int)lane * per, nb), hi = min(lo + per, nb);
+            float mx = -100, mn = cols + 1;
+            for (int b = lo; b < hi; b++) env_boundary(E, (E.pixMin + b) * E.pixSize, mn, mx);
+            // exclusive scan of (mn, mx) over lanes
+            float pmn = mn, pmx = mx;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                float omn = __shfl_up(pmn, off), omx = __shfl_up(pmx, off);
+                if ((int)lane >= off) {
+                    pmn = fminf(pmn, omn);
+                    pmx = fmaxf(pmx, omx);
+                }
+            }
+            float emn = __shfl_up(pmn, 1), emx = __shfl_up(pmx, 1);
+            if (lane == 0) {
+                emn = cols + 1;
+                emx = -100;
+            }
+            mn = emn;
+            mx = emx;
+            for (int b = lo; b < hi; b++) {
+                env_boundary(E, (E.pixMin + b) * E.pixSize, mn, mx);
+                if (b > 0) {
+                    float4 d = env_column(table, rows, cols, E.pixSize, (E.pixMin + b) % cols, mn, mx);
+                    cx += d.x;
+                    cy += d.y;
+                    cz += d.z;
+                    cw += d.w;
+                }
+            }
+        } else {
+            int thMaxPix = min(max(cvt_i32(E.thMax / E.pixSize), 0), rows - 1) * cols;
+            int thMinPix = min(max(cvt_i32(E.thMin / E.pixSize), 0), rows - 1) * cols;
+            for (int q = lane; q < E.pixNum; q += 64) {
+                float4 mxc = table[thMaxPix + (E.pixMin + q) % cols];
+                float4 mnc = table[thMinPix + (E.pixMin + q) % cols];
+                cx += mxc.x - mnc.x;
+                cy += mxc.y - mnc.y;
+                cz += mxc.z - mnc.z;
+                cw += mxc.w - mnc.w;
+            }
+        }
+        cx = wave_xor_sum(cx);
+        cy = wave_xor_sum(cy);
+        cz = wave_xor_sum(cz);
+        cw = wave_xor_sum(cw);
+        if (lane == 0)
+            reinterpret_cast<uchar4*>(img)[ij] = env_finish(E, cx, cy, cz, cw, W, lr, cam, view, area[ij]);
+    }
 }
 
 // addStarsAndBackground (MakeImageKernel.cu:837-853)
@@ -938,8 +1280,9 @@ size_t frame_bytes(int w, int h, int step) {
     const size_t NP = (size_t)w * h, NC = (size_t)(w + 1) * (h + 1);
     const size_t fw = 2 * (size_t)step + 1;
     // thphi, gap | bh, area x2, star and map images, per-tap star light,
-    // heavy-pixel list, BGRA output
-    return NC * (8 + 4) + NP * (1 + 4 + 4 + 4 + 4 + 12 * fw * fw + 4 + 4) + 4096;
+    // star chunk partials, star queues (4) and lensing (3), env queues (2),
+    // BGRA output
+    return NC * (8 + 4) + NP * (1 + 4 + 4 + 4 + 4 + 12 * fw * fw + 12 * fw * fw + 16 + 12 + 8 + 4) + 4096;
 }
 
 int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, const float* cam7,
@@ -973,6 +1316,13 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
 
     Cam7 cam;
     for (int q = 0; q < 7; q++) cam.v[q] = cam7[q];
+    {
+        // blackbody colours per index, made once per context
+        bool fresh = c->scratch.find("r_star_colours") == c->scratch.end();
+        if ((rc = c->buffer("r_star_colours", sizeof(float) * 6 * 391, &p))) return rc;
+        cam.colours = (const float*)p;
+        if (fresh) star_colours<<<2, 256, 0, c->stream>>>((float*)p);
+    }
     GridV gv{g.grid, g.GN, g.GM};
     const dim3 blk(16, 16);
     {
@@ -993,37 +1343,44 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         const int fw = 2 * step + 1;
         if ((rc = c->buffer("r_taps", sizeof(float) * 3 * fw * fw * NP, &p))) return rc;
         float* taps = (float*)p;
-        if ((rc = c->buffer("r_heavy", sizeof(uint32_t) * NP + 64, &p))) return rc;
-        uint32_t* heavy = (uint32_t*)p;
-        if ((rc = c->buffer("r_heavy_n", 64, &p))) return rc;
-        unsigned* nheavy = (unsigned*)p;
-        zero_u32_r<<<1, 64, 0, c->stream>>>(nheavy, 1);
-        const unsigned nblk = (unsigned)c->num_cus * 8;
-        {
-            KTimer t(c, "star_light");
-            if (step == 1)
-                star_map_light<1><<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area,
-                                                                      v.view, taps, heavy, nheavy);
-            else if (step == 2)
-                star_map_light<2><<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area,
-                                                                      v.view, taps, heavy, nheavy);
-            else
-                star_map_light<3><<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area,
-                                                                      v.view, taps, heavy, nheavy);
-        }
-        {
-            KTimer t(c, "star_heavy");
-            if (step == 1) {
-                star_map_heavy<1><<<nblk, 256, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view,
-                                                               taps, heavy, nheavy);
-            } else if (step == 2) {
-                star_map_heavy<2><<<nblk, 256, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view,
-                                                               taps, heavy, nheavy);
-            } else {
-                star_map_heavy<3><<<nblk, 256, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view,
-                                                               taps, heavy, nheavy);
-            }
-        }
+        StarQueue Q;
+        if ((rc = c->buffer("r_star_lens", sizeof(float) * 3 * (NP + 64), &p))) return rc;
+        Q.lens = (float*)p;
+        if ((rc = c->buffer("r_star_q", sizeof(uint32_t) * 4 * (NP + 64), &p))) return rc;
+        Q.pix = (uint32_t*)p;
+        Q.cnt = Q.pix + NP + 64;
+        Q.cstart = Q.cnt + NP + 64;
+        Q.owner = Q.cstart + NP + 64;
+        if ((rc = c->buffer("r_star_qn", 64, &p))) return rc;
+        Q.ctr = (unsigned*)p;
+        const int NT = fw * fw * 3;
+        // at least one chunk per queued pixel fits (queued <= NP), so the
+        // chunk-size search in star_chunk_scan ends
+        Q.cap = (unsigned)(NP + 64);
+        if ((rc = c->buffer("r_star_part", sizeof(float) * NT * (size_t)Q.cap, &p))) return rc;
+        Q.partial = (float*)p;
+        zero_u32_r<<<1, 64, 0, c->stream>>>(Q.ctr, 3);
+        const unsigned nwave = (unsigned)c->num_cus * 16, nred = (unsigned)c->num_cus * 4;
+#define BT_STAR_LAUNCH(S)                                                                                    \
+    {                                                                                                       \
+        {                                                                                                   \
+            KTimer t(c, "star_light");                                                                      \
+            star_map_light<S><<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr,  \
+                                                                  area, v.view, taps, Q);                   \
+        }                                                                                                   \
+        KTimer t(c, "star_heavy");                                                                          \
+        star_chunk_scan<<<1, 1024, 0, c->stream>>>(Q);                                                      \
+        star_chunk_owner<<<nred, 256, 0, c->stream>>>(Q);                                                   \
+        star_map_chunk<S><<<nwave, 256, 0, c->stream>>>(thphi, bh, s, cam, W, f.offset, f.lr, area, v.view, Q); \
+        star_chunk_reduce<S><<<nred, 256, 0, c->stream>>>(W, H, taps, Q);                                   \
+    }
+        if (step == 1)
+            BT_STAR_LAUNCH(1)
+        else if (step == 2)
+            BT_STAR_LAUNCH(2)
+        else
+            BT_STAR_LAUNCH(3)
+#undef BT_STAR_LAUNCH
         BT_HIP(hipGetLastError());
         {
             KTimer t(c, "star_sum");
@@ -1038,8 +1395,25 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         BT_HIP(hipGetLastError());
     }
     if (f.map) {
-        KTimer t(c, "env_map");
-        env_map<<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view, img_map);
+        if ((rc = c->buffer("r_env_q", sizeof(uint32_t) * 2 * (NP + 64), &p))) return rc;
+        uint32_t* qlane = (uint32_t*)p;
+        uint32_t* qwave = (uint32_t*)p + NP + 64;
+        if ((rc = c->buffer("r_env_qn", 64, &p))) return rc;
+        unsigned* nq = (unsigned*)p;
+        zero_u32_r<<<1, 64, 0, c->stream>>>(nq, 2);
+        {
+            KTimer t(c, "env_map");
+            env_map<<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view,
+                                                         img_map, qlane, qwave, nq);
+            env_map_lane<<<(unsigned)c->num_cus * 8, 256, 0, c->stream>>>(thphi, s, cam, W, f.offset, f.lr, area,
+                                                                         v.view, img_map, qlane, nq);
+        }
+        BT_HIP(hipGetLastError());
+        {
+            KTimer t(c, "env_heavy");
+            env_map_heavy<<<(unsigned)c->num_cus * 8, 256, 0, c->stream>>>(thphi, s, cam, W, f.offset, f.lr, area,
+                                                                          v.view, img_map, qwave, nq + 1);
+        }
         BT_HIP(hipGetLastError());
     }
     {

@@ -110,6 +110,7 @@ struct bt_server {
     size_t used = 0, peak = 0;
     int64_t submitted = 0, completed = 0, failed = 0, builds = 0, hits = 0, evictions = 0;
     int inflight = 0, peak_inflight = 0;
+    size_t building = 0;                // bytes the builder has reserved for a grid in progress
     uint64_t tick = 0;
     bool stop = false;
     bool have_first = false;
@@ -186,6 +187,23 @@ void builder_main(bt_server* s) {
         }
         GridKey key = s->build_grids.front();
         s->build_grids.pop_front();
+        // reserve the grid's bytes first (Grid.h:612-615 sizes): wait while
+        // frames in flight hold the memory, fail if nothing can free it
+        const int N = (int)(uint32_t)std::round(std::pow(2, s->cfg.max_level) / (2 - 1) + 1);   // angle = 1
+        const size_t est = grid_device_bytes(N, 2 * (N - 1));
+        bool fits;
+        while (!(fits = make_room(s, est)) && s->inflight > 0 && !s->stop) s->cv.wait(lk);
+        if (s->stop) return;
+        if (!fits) {
+            CachedGrid& cg = s->grids[key];
+            cg.queued = false;
+            cg.status = fail(BT_ERR_BUDGET, "server: a grid does not fit the HBM budget");
+            s->cv.notify_all();
+            continue;
+        }
+        s->used += est;
+        s->building = est;
+        s->peak = std::max(s->peak, s->used);
         lk.unlock();
         bt_pose pose{key.a, key.theta, 0.0, key.r, 0.0, 0.0, 1.0, 0.0, 0};
         bt_grid_params gp;
@@ -201,11 +219,12 @@ void builder_main(bt_server* s) {
         size_t bytes = 0;
         if (rc == BT_OK) bytes = g->bytes + sizeof(float) * 2 * (size_t)g->gridf_rows * g->M;
         lk.lock();
+        s->used -= est;
+        s->building = 0;
         CachedGrid& cg = s->grids[key];
         cg.queued = false;
         cg.status = rc;
         if (rc == BT_OK) {
-            make_room(s, bytes);   // over budget: the dispatcher waits until frames drain
             cg.g = g;
             cg.bytes = bytes;
             cg.ready = true;
@@ -327,7 +346,18 @@ void dispatcher_main(bt_server* s) {
             cg.refs++;   // pin it while evicting
             bool fits = make_room(s, it->bytes);
             cg.refs--;
-            if (!fits) break;   // wait for frames to drain (keeps arrival order)
+            if (!fits) {
+                if (s->inflight == 0 && s->building == 0) {
+                    // nothing in flight, no grid being built and every idle
+                    // grid gone: this request (frame + its grid + the views)
+                    // can never fit
+                    complete(s, *it, BT_ERR_BUDGET, Clock::now(), Clock::now(), it->hit);
+                    it = s->pending.erase(it);
+                    progress = true;
+                    continue;
+                }
+                break;   // wait for frames to drain (keeps arrival order)
+            }
             Slot& sl = *free_slot;
             sl.req = *it;
             sl.grid = &cg;

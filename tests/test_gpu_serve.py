@@ -63,10 +63,10 @@ def test_alone_equals_in_mix_equals_direct(ctx, scene):
 
 
 def test_budget_admission_and_eviction(ctx, scene):
-    # room for one 640x360 frame (about 35 MiB of working set), its view and
-    # a couple of level-7 grids (about 1 MiB each): frames run one at a time
-    # and the three poses evict each other
-    budget = 40 << 20
+    # room for one 640x360 frame (about 62 MiB of working set), its view
+    # (2 MiB) and a few level-7 grids (about 1 MiB each): frames run one at
+    # a time and the three poses evict each other
+    budget = 66 << 20
     srv = bt.Server(scene, hbm_budget=budget, slots=4, max_level=LEVEL, ctx=ctx)
     reqs = [(POSES[k % 3], 0.1 * k, (640, 360)) for k in range(9)]
     outs, comps = _run(srv, reqs)
@@ -88,3 +88,15 @@ def test_budget_admission_and_eviction(ctx, scene):
         tiny = bt.Server(scene, hbm_budget=1 << 20, slots=1, max_level=LEVEL, ctx=ctx)
         tiny.submit(*POSES[0], 0.0, np.zeros((360, 640, 4), dtype=np.uint8), 0)
     tiny.close()
+
+
+def test_request_that_can_never_fit_fails(ctx, scene):
+    """Frame + view + grid above the budget with nothing in flight: the
+    request completes with BT_ERR_BUDGET instead of waiting forever."""
+    srv = bt.Server(scene, hbm_budget=63 << 20, slots=2, max_level=LEVEL, ctx=ctx)
+    out = np.zeros((360, 640, 4), dtype=np.uint8)
+    srv.submit(*POSES[0], 0.0, out, 7)
+    srv.drain(60)
+    comps = srv.poll(8)
+    assert len(comps) == 1 and comps[0]["status"] == bt.BT_ERR_BUDGET
+    srv.close()
