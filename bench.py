@@ -57,6 +57,10 @@ FLOPS_PER_ATTEMPT = 6 * 136 + 270 + 15 + 14
 PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector
 PEAK_HBM_GBS = 8000.0
+# HBM bytes of one rk_mixed launch over the 8.4M-ray sky: rocprofv3 FETCH_SIZE
+# and WRITE_SIZE of that dispatch, each scaled by the same counter on a 1 GiB
+# device copy (profiles/integrator_precision_config_counters.md).
+TRAFFIC_RK_MIXED_BYTES = 865.9e6
 ARCSEC = math.pi / 180 / 3600
 
 POSE = dict(a=0.999, theta=math.pi / 2, phi=0.0, r=5.0)
@@ -77,6 +81,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-pose batch")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--profile-run", action="store_true",
+                    help="a short fixed program for rocprofv3 (copy calibration, one build each, one frame)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
 
@@ -232,6 +238,17 @@ def serve(bt, S, ctx, seconds, rate=200.0):
     who = rng.integers(0, len(traj), size=arrivals.size)
     frame_no = [0] * len(traj)
     lat, qms, fms = [], [], []
+    by_res = {wh: [] for wh in res}
+    tag_res = {}
+
+    def take(comps):
+        for c in comps:
+            lat.append(c["latency_ms"])
+            qms.append(c["queue_ms"])
+            fms.append(c["frame_ms"])
+            if c["tag"] in tag_res:
+                by_res[tag_res.pop(c["tag"])].append(c["latency_ms"])
+
     start = time.perf_counter()
     for i, t_arr in enumerate(arrivals):
         while True:
@@ -244,12 +261,11 @@ def serve(bt, S, ctx, seconds, rate=200.0):
         q = frame_no[j]
         frame_no[j] += 1
         off = 2 * math.pi * q / (0.25 * speed[j] * wh[0])   # MakeImageKernel.cu:1412-1413
+        tag_res[i] = wh
         srv.submit(*traj[j], off, bufs[wh][i % 24], i)
-        for c in srv.poll(256):
-            lat.append(c["latency_ms"]); qms.append(c["queue_ms"]); fms.append(c["frame_ms"])
+        take(srv.poll(256))
     srv.drain(600)
-    for c in srv.poll(4096):
-        lat.append(c["latency_ms"]); qms.append(c["queue_ms"]); fms.append(c["frame_ms"])
+    take(srv.poll(4096))
     wall = time.perf_counter() - start
     st = srv.stats()
     srv.close()
@@ -258,8 +274,41 @@ def serve(bt, S, ctx, seconds, rate=200.0):
             "requests": len(lat), "offered_per_s": arrivals.size / seconds, "frames_per_s": len(lat) / wall,
             "latency_ms_p50": pct(lat, 0.5), "latency_ms_p99": pct(lat, 0.99), "latency_ms_mean": float(np.mean(lat)),
             "queue_ms_p50": pct(qms, 0.5), "frame_ms_p50": pct(fms, 0.5), "frame_ms_p99": pct(fms, 0.99),
+            "by_resolution": {f"{w}x{h}": {"requests": len(v), "latency_ms_p50": pct(v, 0.5),
+                                            "latency_ms_p99": pct(v, 0.99)} for (w, h), v in by_res.items() if v},
             "cold_start_s": cold_s, "grid_builds": st["grid_builds"], "grid_hits": st["grid_hits"] - warm["grid_hits"],
             "peak_inflight": st["peak_inflight"], "peak_hbm_gib": st["peak_bytes"] / 2 ** 30}
+
+
+def profile_run(bt, S, ctx):
+    """A short fixed program for rocprofv3 counter passes: a 1 GiB device copy
+    (the runtime's copy kernel; the bytes calibration of FETCH_SIZE /
+    WRITE_SIZE), two mixed and two FP64 builds of the precision-config sky
+    (read the second of each), 1080p frames."""
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    nbytes = 1 << 30
+    src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    if hip.hipMemcpy(dst.ptr, src.ptr, nbytes, 3) != 0:   # hipMemcpyDeviceToDevice
+        raise RuntimeError("calibration copy failed")
+    ctx.sync()
+    src.close()
+    dst.close()
+    bh = bt.BlackHole(POSE["a"])
+    cam = bt.Camera(POSE["theta"], POSE["phi"], POSE["r"], black_hole=bh)
+    out = {}
+    for rep in range(2):   # the second pass is the one to read (warm clocks and caches)
+        for name, prec in (("mixed", bt.BT_PREC_MIXED), ("fp64", bt.BT_PREC_FP64)):
+            g = bt.Grid(LEVEL, LEVEL, True, cam, bh, params=bt.IntegParams(prec), ctx=ctx)
+            out[name] = g.stats()
+            g.close()
+    fr, _ = frames(bt, S, ctx, 2)
+    out["frame_ms"] = fr["frame_ms_p50"]
+    print(json.dumps(out))
 
 
 def main():
@@ -271,6 +320,9 @@ def main():
     from blacktracer_amd import scene as S
 
     ctx = bt.default_context()
+    if args.profile_run:
+        profile_run(bt, S, ctx)
+        return
     bh = bt.BlackHole(POSE["a"])
     cam = bt.Camera(POSE["theta"], POSE["phi"], POSE["r"], black_hole=bh)
     p_mixed = bt.IntegParams(bt.BT_PREC_MIXED)
@@ -363,7 +415,8 @@ def main():
         "precision": prec,
         "frames": fr,
         "roofline": {"bound": "valu", "kernel": "rk_mixed", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRAFFIC_RK_MIXED_BYTES,
+                     "traffic_note": "HBM bytes of the 8.4M-ray rk_mixed launch from rocprofv3 FETCH_SIZE/WRITE_SIZE",
                      "flops_per_attempt": FLOPS_PER_ATTEMPT, "attempts_per_step": attempts // args.steps,
                      "fp64_attempts_per_step": fp64_att // args.steps,
                      "avg_launch_ms": rk_ms / max(1, n_rk),
