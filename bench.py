@@ -81,6 +81,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-pose batch")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--profile-run", action="store_true",
                     help="a short fixed program for rocprofv3 (copy calibration, one build each, one frame)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -331,7 +332,11 @@ def main():
     def build(prm):
         return bt.Grid(LEVEL, LEVEL, True, cam, bh, params=prm, ctx=ctx)
 
-    # ---- headline: mixed-precision build of the 2049 x 4096 sky
+    # ---- headline: mixed-precision build of the 2049 x 4096 sky, at the
+    # launch shape the autotuner picks for this pose and precision
+    shapes = {}
+    if not args.no_tune:
+        shapes["mixed"] = ctx.autotune(cam, params=p_mixed, n_rays=1 << 21)
     for _ in range(args.warmup):
         build(p_mixed).close()
     ctx.sync()
@@ -357,7 +362,9 @@ def main():
     n_rk, rk_ms = kt["rk_mixed"]
     achieved = FLOPS_PER_ATTEMPT * attempts / (rk_ms * 1e-3) / 1e12 if rk_ms > 0 else 0.0
 
-    # ---- the plain FP64 build of the same sky
+    # ---- the plain FP64 build of the same sky (tuned for FP64 in turn)
+    if not args.no_tune:
+        shapes["fp64"] = ctx.autotune(cam, params=p_fp64, n_rays=1 << 21)
     build(p_fp64).close()
     ctx.sync()
     ctx.timing(True)
@@ -375,6 +382,8 @@ def main():
     ctx.timing(False)
     fp64_value = f_rays * len(f_times) / sum(f_times)
 
+    if "mixed" in shapes:
+        ctx.set_tune(*(shapes["mixed"][k] for k in ("chunk", "block", "waves")))
     # ---- precision: mixed vs FP64 ray by ray; FP64 vs converged on the sub-lattice
     gm, g64 = build(p_mixed), build(p_fp64)
     prec = err_stats(g64.cel, gm.cel)
@@ -407,7 +416,8 @@ def main():
         "config": {"workload": "precision config: a=0.999, r=5M, theta=pi/2 circular orbit; uniform Grid(11,11) "
                                "2049x4096 camera sky (8,384,514 rays); eps 1e-5, z_end -1e7, MAXSTP 1000; "
                                "mixed FP32/FP64 integrator",
-                   "rays_per_step": rays // args.steps, "parallelism": "single GPU"},
+                   "rays_per_step": rays // args.steps, "parallelism": "single GPU",
+                   "launch_shape": shapes or "default (chunk 128, block 256, waves 0)"},
         "fp64_rays_per_s": fp64_value,
         "mixed_over_fp64": value / fp64_value,
         "kernel_ms_per_step": {n: v[1] / max(1, v[0]) for n, v in kt.items()},

@@ -50,6 +50,27 @@ class Context:
     def reset_timers(self):
         check(_lib.load().bt_timer_reset(self.handle))
 
+    def tune(self):
+        """The persistent integrators' launch shape: {chunk, block, waves}."""
+        t = _lib.BtTune()
+        check(_lib.load().bt_tune_get(self.handle, C.byref(t)))
+        return {"chunk": t.chunk, "block": t.block, "waves": t.waves}
+
+    def set_tune(self, chunk=128, block=256, waves=0):
+        t = _lib.BtTune(chunk, block, waves, 0.0)
+        check(_lib.load().bt_tune_set(self.handle, C.byref(t)))
+
+    def autotune(self, camera, params=None, n_rays=1 << 20):
+        """Times seeded rays of the camera's pose over launch shapes, keeps the
+        fastest; returns {chunk, block, waves, rays_per_lane, ms}."""
+        t = _lib.BtTune()
+        ms = C.c_double()
+        params = params or IntegParams()
+        check(_lib.load().bt_autotune(self.handle, C.byref(camera.pose()), C.byref(params), int(n_rays),
+                                      C.byref(t), C.byref(ms)))
+        return {"chunk": t.chunk, "block": t.block, "waves": t.waves,
+                "rays_per_lane": t.rays_per_lane, "ms": ms.value}
+
     def alloc(self, nbytes):
         """A device buffer owned by the caller (DeviceBuffer)."""
         return DeviceBuffer(self, nbytes)

@@ -83,8 +83,17 @@ class BtServerStats(C.Structure):
                 ("wall_s", C.c_double)]
 
 
+class BtTune(C.Structure):
+    _fields_ = [("chunk", C.c_int), ("block", C.c_int), ("waves", C.c_int),
+                ("rays_per_lane", C.c_double)]
+
+
 # name -> (restype, argtypes); every symbol here is declared in include/bt_abi.h
 SIGNATURES = {
+    "bt_tune_get": (C.c_int, [C.c_void_p, C.POINTER(BtTune)]),
+    "bt_tune_set": (C.c_int, [C.c_void_p, C.POINTER(BtTune)]),
+    "bt_autotune": (C.c_int, [C.c_void_p, C.POINTER(BtPose), C.POINTER(BtIntegParams), C.c_int64,
+                              C.POINTER(BtTune), DP]),
     "bt_server_config_default": (None, [C.POINTER(BtServerConfig)]),
     "bt_server_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BtServerConfig), C.POINTER(C.c_void_p)]),
     "bt_server_submit": (C.c_int, [C.c_void_p, C.POINTER(BtRequest)]),

@@ -147,6 +147,7 @@ struct Tune {
 struct Context {
     int device = 0;
     Tune tune;
+    int64_t launch_lanes = 0;   // widest persistent launch since last reset (lanes)
     int num_cus = 256;
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;                 // overlapping work, joined before a call returns

@@ -358,6 +358,7 @@ int persistent_launch(Context* c, K kernel, int64_t n_max, Launch* L) {
     if (blocks < 1) blocks = 1;
     L->blocks = (unsigned)blocks;
     L->threads = (unsigned)threads;
+    if (blocks * threads > c->launch_lanes) c->launch_lanes = blocks * threads;
     return BT_OK;
 }
 

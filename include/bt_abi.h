@@ -123,6 +123,27 @@ int bt_trace_rays_device(bt_context* ctx, const bt_pose* pose, const bt_integ_pa
                          int64_t n, double* d_theta, double* d_phi, int32_t* d_step,
                          bt_trace_stats* stats);
 
+/* ---- launch-shape tuning of the persistent integrators (no reference
+ * counterpart: the reference launches one thread per ray with a fixed block,
+ * integration_wrapper Grid.h:514-556) ---- */
+typedef struct {
+    int chunk;             /* queue positions a wave reserves per atomic (>= 64) */
+    int block;             /* threads per workgroup: 64, 128 or 256 */
+    int waves;             /* resident waves per SIMD, 0 = as many as registers allow */
+    double rays_per_lane;  /* bt_autotune only: calibration rays per lane of the widest launch */
+} bt_tune;
+
+int bt_tune_get(bt_context* ctx, bt_tune* out);
+/* BT_ERR_INVALID for a block other than 64/128/256, chunk outside
+ * [64, 2^20] or waves outside 0..8.  Results never depend on the shape. */
+int bt_tune_set(bt_context* ctx, const bt_tune* t);
+/* Times n_rays (<= 0: 2^20; at most 2^24) seeded uniform directions of the
+ * pose under prm for candidate shapes (coordinate descent over chunk
+ * {64..1024}, block {64,128,256}, waves {0..4}) and keeps the fastest in
+ * the context; out and best_ms (nullable) receive it and its time. */
+int bt_autotune(bt_context* ctx, const bt_pose* pose, const bt_integ_params* prm, int64_t n_rays,
+                bt_tune* out, double* best_ms);
+
 /* ---- adaptive grid: Grid(maxLevel, startLevel, angle, cam, bh) ---- */
 typedef struct bt_grid bt_grid;
 
