@@ -85,6 +85,8 @@ def parse():
     ap.add_argument("--profile-run", action="store_true",
                     help="a short fixed program for rocprofv3 (copy calibration, one build each, one frame)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--precision-sweep", action="store_true",
+                    help="mixed vs FP64 error and rays/s per pose of the 9-pose precision workload")
     return ap.parse_args()
 
 
@@ -312,6 +314,55 @@ def profile_run(bt, S, ctx):
     print(json.dumps(out))
 
 
+def precision_sweep(bt, ctx):
+    """Mixed against FP64, pose by pose: the 2049 x 4096 sky (uniform
+    Grid(11,11)) at a in {0.5, 0.9, 0.999} x r in {3, 5, 20} M, theta = pi/2,
+    reference integrator settings; then the radius-stop variant (rel tol 1e-8,
+    escape r > 1000M, capture r < 1.01 r+) on the 257 x 513 sky.  Each
+    precision runs at the launch shape the autotuner picks for it."""
+    rows = []
+
+    def timed(level, cam, bh, prm):
+        bt.Grid(level, level, True, cam, bh, params=prm, ctx=ctx).close()
+        ts = []
+        for _ in range(3):
+            ctx.sync()
+            t0 = time.perf_counter()
+            g = bt.Grid(level, level, True, cam, bh, params=prm, ctx=ctx)
+            ctx.sync()
+            ts.append(time.perf_counter() - t0)
+            if len(ts) < 3:
+                g.close()
+        return g, sorted(ts)[1]
+
+    cases = [(a, r, LEVEL, "reference") for a in (0.5, 0.9, 0.999) for r in (3.0, 5.0, 20.0)]
+    cases.append((0.999, 5.0, 8, "radius-stop"))
+    for a, r, level, kind in cases:
+        bh = bt.BlackHole(a)
+        cam = bt.Camera(math.pi / 2, 0.0, r, black_hole=bh)
+        mk = (lambda prec: bt.radius_stop_params(prec)) if kind == "radius-stop" else bt.IntegParams
+        res = {}
+        for name, prec in (("fp64", bt.BT_PREC_FP64), ("mixed", bt.BT_PREC_MIXED)):
+            ctx.autotune(cam, params=mk(prec), n_rays=1 << 20)
+            g, t = timed(level, cam, bh, mk(prec))
+            res[name] = (g, t)
+        g64, t64 = res["fp64"]
+        gm, tm = res["mixed"]
+        row = {"a": a, "r": r, "grid": f"Grid({level},{level})", "params": kind, "rays": gm.rays_traced}
+        row.update(err_stats(g64.cel, gm.cel))
+        st = gm.stats()
+        row["fp64_attempt_share"] = st["attempts_fp64"] / max(1, st["attempts"])
+        row["fallback_rays"] = st["fallback_rays"]
+        row["fp64_rays_per_s"] = g64.rays_traced / t64
+        row["mixed_rays_per_s"] = gm.rays_traced / tm
+        row["mixed_over_fp64"] = t64 / tm
+        g64.close()
+        gm.close()
+        print(json.dumps(row), flush=True)
+        rows.append(row)
+    return rows
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -323,6 +374,9 @@ def main():
     ctx = bt.default_context()
     if args.profile_run:
         profile_run(bt, S, ctx)
+        return
+    if args.precision_sweep:
+        precision_sweep(bt, ctx)
         return
     bh = bt.BlackHole(POSE["a"])
     cam = bt.Camera(POSE["theta"], POSE["phi"], POSE["r"], black_hole=bh)

@@ -183,8 +183,8 @@ def IntegParams(precision=BT_PREC_FP64, **kw):  # noqa: N802
     return p
 
 
-def baseline_params(precision=BT_PREC_FP64):
-    """The BASELINE wording of the grid build: rel tol 1e-8, escape at
+def radius_stop_params(precision=BT_PREC_FP64):
+    """The radius-stop variant of the full-sky build: rel tol 1e-8, escape at
     r > 1000M, capture at r < 1.01 r+ (no a-priori capture test)."""
     return IntegParams(precision, eps=1e-8, max_steps=10000, apriori_capture=0, r_escape=1000.0,
                        r_capture_factor=1.01)

@@ -252,12 +252,46 @@ __global__ __launch_bounds__(256) void rk_mixed(Src S, ParamsK prm, QueueK Q, un
 
 }  // namespace
 
+// Below this tolerance FP32 cannot carry the step control: the rounding of
+// the FP32 stage sums (unit roundoff 6e-8, relative to |y| + |h y'|) is no
+// longer small against eps, so the error estimate turns into noise, steps
+// collapse and the FP32 right-hand side's rounding accumulates into the
+// angles.  The mixed mode then traces every ray in FP64.
+constexpr double kMixedEpsMin = 1e-6;
+
+namespace {
+// phase 0: snapshot the integrated-ray count; phase 1: add the rays
+// integrated since to the fallback count.
+__global__ void stats_mark(StatsK* s, unsigned long long* snap, int phase) {
+    if (phase == 0)
+        snap[0] = s->integrated;
+    else
+        s->fallback += s->integrated - snap[0];
+}
+}  // namespace
+
 template <class Src>
 int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, StatsK* d_stats) {
     if (n <= 0) return BT_OK;
     if (n > 0x7fffffffLL) return fail(BT_ERR_INVALID, "trace_mixed: too many rays in one batch");
-    if (prm.eps < 1e-9)
-        return fail(BT_ERR_INVALID, "mixed precision needs eps >= 1e-9 (FP32 error estimate); use BT_PREC_FP64");
+    if (prm.eps < kMixedEpsMin) {
+        // every ray in FP64; the rays it integrates are counted as fallback rays
+        unsigned long long* snap = nullptr;
+        int rc;
+        if (d_stats) {
+            void* p;
+            if ((rc = c->buffer("stats_snap", sizeof(unsigned long long), &p))) return rc;
+            snap = (unsigned long long*)p;
+            stats_mark<<<1, 1, 0, c->stream>>>(d_stats, snap, 0);
+            BT_HIP(hipGetLastError());
+        }
+        if ((rc = trace_f64_src(c, S, prm, n, d_stats))) return rc;
+        if (d_stats) {
+            stats_mark<<<1, 1, 0, c->stream>>>(d_stats, snap, 1);
+            BT_HIP(hipGetLastError());
+        }
+        return BT_OK;
+    }
     // Qw: warm-up (FP64) then FP32 queue; Qc: near-critical rays, FP64 from
     // the start, traced on the auxiliary stream while the main stream runs
     // the warm-up and the FP32 bulk; Qr: rays the FP32 kernel hands back

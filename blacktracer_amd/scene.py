@@ -1,6 +1,6 @@
 """Seeded synthetic scene inputs for the render pass (stars and sky map).
 
-Stars: uniform on the sphere, magnitude uniform in [0, 8],
+Stars of the 1080p full-frame workload: uniform on the sphere, magnitude uniform in [0, 8],
 temperature log-uniform in [3000, 30000] K.  The reference stores a B-V
 colour per star and derives temperature with Ballesteros' formula
 (MakeImageKernel.cu:295-297, index = 46/z (1/(0.92 BV + 1.7) + 1/(0.92 BV +

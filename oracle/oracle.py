@@ -140,8 +140,8 @@ def params(eps=1e-5, h1=0.01, z_end=-1e7, max_steps=1000, apriori_capture=1,
     return p
 
 
-def baseline_params():
-    """The BASELINE wording: rel tol 1e-8, stop r>1000M or r<1.01 r+."""
+def radius_stop_params():
+    """The radius-stop variant: rel tol 1e-8, stop r>1000M or r<1.01 r+."""
     return params(eps=1e-8, max_steps=10000, apriori_capture=0, r_escape=1000.0,
                   r_capture_factor=1.01)
 

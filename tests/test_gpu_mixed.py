@@ -86,11 +86,21 @@ def test_mixed_attempt_cap_restarts_in_fp64(ctx):
     assert st["fallback_rays"] >= capped.sum()
 
 
-def test_mixed_rejects_tiny_eps(ctx):
+def test_mixed_below_fp32_tolerance_runs_fp64(ctx):
+    """eps below what FP32 step control can carry (1e-6): the mixed mode
+    traces every ray in FP64, identical to the FP64 path, and counts them
+    all as fallback rays."""
     cam = bt.Camera(math.pi / 2, 0.0, 5.0, black_hole=bt.BlackHole(0.999))
-    with pytest.raises(bt.BtError):
-        bt.trace_rays(cam, np.array([1.0]), np.array([1.0]),
-                      params=bt.IntegParams(bt.BT_PREC_MIXED, eps=1e-12), ctx=ctx)
+    t, p = _lattice(5)
+    for eps in (1e-8, 1e-12):
+        print(f"eps={eps}", flush=True)
+        t64, p64, s64, st64 = bt.trace_rays(cam, t, p, params=bt.IntegParams(bt.BT_PREC_FP64, eps=eps),
+                                            ctx=ctx, return_stats=True)
+        tm, pm, sm, stm = bt.trace_rays(cam, t, p, params=bt.IntegParams(bt.BT_PREC_MIXED, eps=eps),
+                                        ctx=ctx, return_stats=True)
+        assert np.array_equal(t64, tm) and np.array_equal(p64, pm) and np.array_equal(s64, sm)
+        assert stm["attempts_fp64"] == stm["attempts"] == st64["attempts"]
+        assert stm["fallback_rays"] == stm["integrated"] > 0
 
 
 def test_mixed_grid_build(ctx):

@@ -94,11 +94,11 @@ def test_trace_rays_max_steps_sentinel(ctx):
     assert np.allclose(th[esc], 1.0) and np.allclose(ph[esc], 0.5)
 
 
-def test_trace_rays_baseline_wording(ctx):
+def test_trace_rays_radius_stop_variant(ctx):
     """rel tol 1e-8, stop at r > 1000M (escaped) or r < 1.01 r+ (captured)."""
     cam = bt.Camera(math.pi / 2, 0.0, 5.0, black_hole=bt.BlackHole(0.999))
     t, p = _dirs(3000, 8)
-    th, ph, st = bt.trace_rays(cam, t, p, params=bt.baseline_params(), ctx=ctx)
+    th, ph, st = bt.trace_rays(cam, t, p, params=bt.radius_stop_params(), ctx=ctx)
     ps = O.pose(0.999, math.pi / 2, 0.0, 5.0)
-    oth, oph, ost, _ = O.trace_rays(ps, O.baseline_params(), t, p)
+    oth, oph, ost, _ = O.trace_rays(ps, O.radius_stop_params(), t, p)
     _compare(th, ph, st, oth, oph, ost, max_step_diff_frac=0.002)
