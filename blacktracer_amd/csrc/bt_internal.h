@@ -224,7 +224,8 @@ int trace_f64(Context* c, const PoseK& pose, const ParamsK& prm, int64_t n, cons
 // Q64 instead.  trace_f64_queue: traces every ray queued in Q in FP64 from
 // its initial state (at most n_max rays); with cont given (warm-up mode) a
 // ray leaves after its first attempt with errmax > ERRCON and its state
-// goes to cont (cont.nstp = -1 for a ray that ended during the warm-up).
+// goes to cont (cont.nstp = -1 for a ray that ended during the warm-up);
+// prio is the waves' base issue priority (0..3).
 int queue_alloc(Context* c, int64_t n, QueueK* Q);
 template <class Src>
 int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const QueueK& Q,
@@ -232,7 +233,7 @@ int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const Q
 template <class Src>
 int trace_f64_queue(Context* c, const Src& S, const ParamsK& prm, int64_t n_max, const QueueK& Q,
                     StatsK* d_stats, const char* timer, const ContK* cont,
-                    hipStream_t stream = nullptr);
+                    hipStream_t stream = nullptr, int prio = 0);
 template <class Src>
 int trace_f64_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, StatsK* d_stats);
 // kerr_mixed.hip: FP32 bulk with FP64 set-up, warm-up, fallback and accumulation.

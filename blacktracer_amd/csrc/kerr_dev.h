@@ -362,6 +362,39 @@ int persistent_launch(Context* c, K kernel, int64_t n_max, Launch* L) {
     return BT_OK;
 }
 
+// Wave issue priority (s_setprio) of a persistent integrator wave: base,
+// raised by one once a lane's ray passes kLongRay attempts and by two past
+// kVeryLongRay.  The few rays near the photon orbit run up to the attempt
+// cap while the bulk takes tens of attempts; a launch ends with its longest
+// ray, so its wave gets first claim on the SIMD's issue slots and the short
+// rays of other waves fill the rest.  Only the order of issue changes.
+constexpr int kLongRay = 128;
+constexpr int kVeryLongRay = 512;
+
+__device__ __forceinline__ void set_wave_prio(int p) {
+    p = __builtin_amdgcn_readfirstlane(p);   // uniform: a scalar branch
+    if (p <= 0)
+        __builtin_amdgcn_s_setprio(0);
+    else if (p == 1)
+        __builtin_amdgcn_s_setprio(1);
+    else if (p == 2)
+        __builtin_amdgcn_s_setprio(2);
+    else
+        __builtin_amdgcn_s_setprio(3);
+}
+
+// Updates the wave's priority from its lanes' attempt counts; cur is the
+// priority last set (wave-uniform).
+__device__ __forceinline__ void update_wave_prio(int base, bool live, int nstp, int& cur) {
+    const int boost = __ballot(live && nstp >= kVeryLongRay) ? 2 : __ballot(live && nstp >= kLongRay) ? 1 : 0;
+    int want = base + boost;
+    if (want > 3) want = 3;
+    if (want != cur) {
+        set_wave_prio(want);
+        cur = want;
+    }
+}
+
 // Appends the lanes with `take` set to queue Q (one atomic per wave).  Every
 // lane of the wave must call it.
 __device__ __forceinline__ void queue_push(const QueueK& Q, bool take, uint32_t k, unsigned lane) {

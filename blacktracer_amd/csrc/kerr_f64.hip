@@ -124,8 +124,10 @@ __global__ __launch_bounds__(256) void ic_capture(Src S, ParamsK prm, int64_t n,
 // that ended here.
 template <bool WARM, class Src>
 __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, StatsK* st,
-                                              unsigned chunk, ContK cont) {
+                                              unsigned chunk, ContK cont, int prio) {
     const unsigned lane = lane_id();
+    int cur_prio = 0;
+    update_wave_prio(prio, false, 0, cur_prio);
     const unsigned qlen = Q.counters[0];
     unsigned* head = &Q.counters[1];
     unsigned res_next = 0, res_end = 0;   // this wave's reserved queue positions
@@ -168,6 +170,7 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
             }
         }
         if (__ballot(slot >= 0) == 0) break;
+        update_wave_prio(prio, slot >= 0, nstp, cur_prio);
         if (slot >= 0) {
             double em = attempt(lp.a, lp.asq, var, z, h, prm.eps, b, q);
             attempts++;
@@ -244,16 +247,17 @@ int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const Q
 
 template <class Src>
 int trace_f64_queue(Context* c, const Src& S, const ParamsK& prm, int64_t n_max, const QueueK& Q,
-                    StatsK* d_stats, const char* timer, const ContK* cont, hipStream_t stream) {
+                    StatsK* d_stats, const char* timer, const ContK* cont, hipStream_t stream, int prio) {
     if (!stream) stream = c->stream;
     KTimer t(c, timer, stream);
     Launch L;
     if (cont) {
         if (int rc = persistent_launch(c, rk_f64<true, Src>, n_max, &L)) return rc;
-        rk_f64<true, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, *cont);
+        rk_f64<true, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, *cont, prio);
     } else {
         if (int rc = persistent_launch(c, rk_f64<false, Src>, n_max, &L)) return rc;
-        rk_f64<false, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, ContK{});
+        rk_f64<false, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, ContK{},
+                                                               prio);
     }
     BT_HIP(hipGetLastError());
     return BT_OK;
@@ -264,9 +268,9 @@ template int trace_setup<SingleSrc>(Context*, const SingleSrc&, const ParamsK&, 
 template int trace_setup<MultiSrc>(Context*, const MultiSrc&, const ParamsK&, int64_t, const QueueK&,
                                    const QueueK*, StatsK*);
 template int trace_f64_queue<SingleSrc>(Context*, const SingleSrc&, const ParamsK&, int64_t, const QueueK&,
-                                        StatsK*, const char*, const ContK*, hipStream_t);
+                                        StatsK*, const char*, const ContK*, hipStream_t, int);
 template int trace_f64_queue<MultiSrc>(Context*, const MultiSrc&, const ParamsK&, int64_t, const QueueK&,
-                                       StatsK*, const char*, const ContK*, hipStream_t);
+                                       StatsK*, const char*, const ContK*, hipStream_t, int);
 
 template <class Src>
 int trace_f64_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, StatsK* d_stats) {

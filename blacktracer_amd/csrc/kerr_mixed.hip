@@ -175,6 +175,7 @@ __global__ __launch_bounds__(256) void rk_mixed(Src S, ParamsK prm, QueueK Q, un
         r_cap = P.r_cap;
     }
     int nstp = 0;
+    int cur_prio = 0;
     unsigned long long attempts = 0, restarts = 0;
 
     while (true) {
@@ -218,6 +219,7 @@ __global__ __launch_bounds__(256) void rk_mixed(Src S, ParamsK prm, QueueK Q, un
         // lanes that drew a ray which ended in the warm-up hold no slot but
         // are not exhausted: they claim again next round
         if (__ballot(slot >= 0 || !exhausted) == 0) break;
+        update_wave_prio(0, slot >= 0, nstp, cur_prio);
         bool restart = false, done = false;
         if (slot >= 0) {
             attempt_m(C, y, z, h, inv_eps);
@@ -322,8 +324,9 @@ int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, Sta
     BT_HIP(hipEventRecord(c->ev_fork, c->stream));
     BT_HIP(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
     const int64_t crit_lanes = (int64_t)c->num_cus * c->tune.block;
+    // the critical rays' waves issue ahead of the bulk's (base priority 1)
     rc = trace_f64_queue(c, S, prm, n < crit_lanes ? n : crit_lanes, Qc, d_stats, "rk_f64_critical",
-                         nullptr, c->aux);
+                         nullptr, c->aux, 1);
     // join (also on error, so that nothing is left running on the aux stream)
     auto join = [&]() {
         hipEventRecord(c->ev_join, c->aux);
