@@ -417,6 +417,11 @@ class View:
             pass
 
 
+def frame_bytes(width, height, step=1):
+    """Device bytes one frame of this size works in (what the server reserves)."""
+    return int(_lib.load().bt_frame_bytes(int(width), int(height), int(step)))
+
+
 def render_frame_device(grid, scene, view, out_buffer, flags=None, cam7=None, step=1, ctx=None):
     """One frame into a device buffer (asynchronous; no host copy)."""
     ctx = ctx or grid.ctx

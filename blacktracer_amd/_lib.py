@@ -90,6 +90,7 @@ class BtTune(C.Structure):
 
 # name -> (restype, argtypes); every symbol here is declared in include/bt_abi.h
 SIGNATURES = {
+    "bt_frame_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
     "bt_tune_get": (C.c_int, [C.c_void_p, C.POINTER(BtTune)]),
     "bt_tune_set": (C.c_int, [C.c_void_p, C.POINTER(BtTune)]),
     "bt_autotune": (C.c_int, [C.c_void_p, C.POINTER(BtPose), C.POINTER(BtIntegParams), C.c_int64,

@@ -25,6 +25,10 @@ struct SceneK {
     const int32_t* bin_start;
     const float* bin_pos;
     const float* bin_mag;
+    // every star's magnitude lies in [-10, 60] (no NaN): with a pixel's
+    // lensing in range, no tap of any star needs the reference's slow
+    // expression
+    int mag_tame;
 };
 
 constexpr int kBinT = 256;
@@ -77,9 +81,10 @@ struct bt_scene {
     float* bin_pos = nullptr;
     float* bin_mag = nullptr;
     int tree_level = 0, S = 0, rows = 0, cols = 0;
+    int mag_tame = 1;
     size_t bytes = 0;
     bt::SceneK k() const {
-        return bt::SceneK{tree, tree_level, pos, mag, S, table, rows, cols, bin_start, bin_pos, bin_mag};
+        return bt::SceneK{tree, tree_level, pos, mag, S, table, rows, cols, bin_start, bin_pos, bin_mag, mag_tame};
     }
 };
 

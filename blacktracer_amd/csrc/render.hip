@@ -410,6 +410,7 @@ struct StarPix {
     float mn[2], mx[2];
     int sgn;
     bool picheck;
+    bool odd;   // lensing out of the range where every tap takes the fast form
     float frac, redshft, red;
 };
 
@@ -536,6 +537,45 @@ __device__ void search_tree(const int32_t* tree, int treeLevel, const float* mn,
 
 __device__ __forceinline__ int bin_of(float v, float d, int n) { return min(max(cvt_i32(v / d), 0), n - 1); }
 
+// The phi bins [q0, q1] of bin row r (theta in [r, r + 1] * kBinDT) that can
+// hold a star passing star_in_polygon, within the pixel's bounding-box bins
+// [p0, p1]; q0 > q1 when none can.  Each edge test of star_in_polygon,
+// A (sp - p_b) - B (st - t_b) > 0 with A = sgn (t_a - t_b) and
+// B = sgn (p_a - p_b), bounds phi by a line in theta whose extremes over the
+// row are at its two ends; the four bounds are intersected.  Conservative:
+// the row is widened by 1e-3 of a bin in theta, the span by one bin on each
+// side (the rounding of the FP32 edge tests is far below a bin), and an edge
+// that is steep in this frame (|B| > 1000 |A|, where that rounding divided by
+// |A| could reach a bin) bounds nothing.  On a strongly lensed pixel (a thin
+// arc of the Einstein ring) this tests a small fraction of its bounding
+// box's stars.
+__device__ __forceinline__ void row_span(const StarPix& P, int r, int p0, int p1, int& q0, int& q1) {
+    const float ta = ((float)r - 1e-3f) * kBinDT, tb = ((float)r + 1.001f) * kBinDT;
+    float lo = P.mn[1], hi = P.mx[1];
+    const float sg = (float)P.sgn;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int kb = (k + 1) & 3;
+        const float A = sg * (P.t[k] - P.t[kb]);
+        const float B = sg * (P.p[k] - P.p[kb]);
+        if (!(fabsf(B) <= 1000.f * fabsf(A))) continue;   // also A == 0, NaN
+        const float g = B / A;
+        const float la = P.p[kb] + g * (ta - P.t[kb]);
+        const float lb = P.p[kb] + g * (tb - P.t[kb]);
+        if (A > 0.f)
+            lo = fmaxf(lo, fminf(la, lb));
+        else
+            hi = fminf(hi, fmaxf(la, lb));
+    }
+    if (!(lo <= hi)) {
+        q0 = 1;
+        q1 = 0;
+        return;
+    }
+    q0 = max(p0, bin_of(lo, kBinDP, kBinP) - 1);
+    q1 = min(p1, bin_of(hi, kBinDP, kBinP) + 1);
+}
+
 // f(begin, end) for every candidate range of a pixel; ranges index
 // ranges_pos/ranges_mag (tree order for picheck pixels, bin order otherwise).
 template <class F>
@@ -549,23 +589,60 @@ __device__ void for_ranges(const StarPix& P, const SceneK& s, F&& f) {
     } else {
         const int r0 = bin_of(P.mn[0], kBinDT, kBinT), r1 = bin_of(P.mx[0], kBinDT, kBinT);
         const int p0 = bin_of(P.mn[1], kBinDP, kBinP), p1 = bin_of(P.mx[1], kBinDP, kBinP);
-        for (int r = r0; r <= r1; r++) f(s.bin_start[r * kBinP + p0], s.bin_start[r * kBinP + p1 + 1]);
+        for (int r = r0; r <= r1; r++) {
+            int q0, q1;
+            row_span(P, r, p0, p1, q0, q1);
+            if (q0 <= q1) f(s.bin_start[r * kBinP + q0], s.bin_start[r * kBinP + q1 + 1]);
+        }
     }
 }
 
-// One candidate star of one pixel (the body of MakeImageKernel.cu:285-322).
-template <int STEP>
-__device__ __forceinline__ void star_accumulate(const StarPix& P, const float* pos, const float* mag, int q,
-                                                float (&acc)[(2 * STEP + 1) * (2 * STEP + 1)][3]) {
-    constexpr int FW = 2 * STEP + 1;
+// The inside test of one candidate star (MakeImageKernel.cu:285-292): st,
+// sp receive its position (phi shifted by 2 pi for the retest of a quad
+// across phi = 2 pi).
+__device__ __forceinline__ bool star_test(const StarPix& P, const float* pos, int q, float& st, float& sp) {
     const float2 sp2 = reinterpret_cast<const float2*>(pos)[q];
-    float st = sp2.x, sp = sp2.y;
+    st = sp2.x;
+    sp = sp2.y;
     bool in = star_in_polygon(P.t, P.p, st, sp, P.sgn);
     if (P.picheck && !in && sp < BT_FPI2 * .2f) {
         sp += BT_FPI2;
         in = star_in_polygon(P.t, P.p, st, sp, P.sgn);
     }
-    if (!in) return;
+    return in;
+}
+
+// 10^x in FP32 to a few ulps: 2^(x log2 10), the product carried in two
+// parts (log2 10 = L1 + L2, the rounding of x L1 recovered by an FMA), so
+// that the exponent's rounding (large for large |x|) does not reach the
+// result.
+__device__ __forceinline__ float exp10_f32(float x) {
+    constexpr float L1 = 3.32192802429199219f;     // log2(10) rounded to float
+    constexpr float L2 = 7.05797954e-08f;          // log2(10) - L1
+    const float hi = x * L1;
+    const float lo = fmaf(x, L1, -hi) + x * L2;
+    return exp2f(hi) * (1.f + lo * 0.693147180559945309f);
+}
+
+// Whether every tap of every star takes the fast form in this pixel: the
+// tap weight x = frac * gaussian(d) lies in (0, 3.4e38] for frac in
+// [1e-30, 1e30] (gaussian(d) in [9e-4, 1] over the footprint), and
+// E = 10^(-0.4 (m + red)) stays finite for m >= -10, |red| <= 80.
+__device__ __forceinline__ bool star_pixel_odd(const StarPix& P, const SceneK& s) {
+    return !(s.mag_tame && P.frac >= 1e-30f && P.frac <= 1e30f && P.red >= -80.f && P.red <= 80.f);
+}
+
+// The light of one star inside the pixel (MakeImageKernel.cu:293-322).
+// Per tap the reference evaluates 100 * 10^(-0.4 (part - 2.5 log10 x)) with
+// x = frac * gaussian(d); for finite x > 0 and finite E = 10^(-0.4 part)
+// that is 100 x E.  ODD = false (pixels where star_pixel_odd is false):
+// every tap takes that fast form.  ODD = true: each tap checks, and takes
+// the reference's own expression (kernel.cuh:390, MakeImageKernel.cu:310-318)
+// where the fast form does not apply.
+template <int STEP, bool ODD>
+__device__ __forceinline__ void star_splat(const StarPix& P, const float* mag, int q, float st, float sp,
+                                           float (&acc)[(2 * STEP + 1) * (2 * STEP + 1)][3]) {
+    constexpr int FW = 2 * STEP + 1;
     interpolate_star(P.t[0], P.t[1], P.t[2], P.t[3], P.p[0], P.p[1], P.p[2], P.p[3], st, sp, P.sgn);
     const float2 mg = reinterpret_cast<const float2*>(mag)[q];
     const float part = mg.x + P.red;
@@ -574,8 +651,8 @@ __device__ __forceinline__ void star_accumulate(const StarPix& P, const float* p
     int index = max(0, min(cvt_i32(floorf(temp)), 390));
     // the blackbody colour's HSP and its RGB at unit brightness, per index
     const float* col = P.colours + 6 * index;
-    const float hh = col[0], ss = col[1], pp = col[2], r1 = col[3], g1 = col[4], b1 = col[5];
-    const float E = f_exp10(-.4f * part);
+    const float r1 = col[3], g1 = col[4], b1 = col[5];
+    const float E = ODD ? f_exp10(-.4f * part) : exp10_f32(-.4f * part);
     const float maxDistSq = (STEP + .5f) * (STEP + .5f);
 #pragma unroll
     for (int u = 0; u < FW; u++)
@@ -587,7 +664,7 @@ __device__ __forceinline__ void star_accumulate(const StarPix& P, const float* p
             if (dist > maxDistSq) continue;
             float x = P.frac * gaussian(dist);
             float rr, gg, bb;
-            if (x > 0.f && x <= 3.4e38f && fabsf(E) <= 3.4e38f) {
+            if (!ODD || (x > 0.f && x <= 3.4e38f && fabsf(E) <= 3.4e38f)) {
                 float bright = 100.f * x * E;
                 rr = r1 * bright;
                 gg = g1 * bright;
@@ -595,7 +672,7 @@ __device__ __forceinline__ void star_accumulate(const StarPix& P, const float* p
             } else {
                 float appMag = part - 2.5f * f_log10(x);
                 float brightness = 100 * f_exp10(-.4f * appMag);
-                hsp_to_rgb(hh, ss, pp * brightness, rr, gg, bb);
+                hsp_to_rgb(col[0], col[1], col[2] * brightness, rr, gg, bb);
             }
             rr *= 255.f;
             gg *= 255.f;
@@ -606,36 +683,80 @@ __device__ __forceinline__ void star_accumulate(const StarPix& P, const float* p
         }
 }
 
+// One candidate star of one pixel (the body of MakeImageKernel.cu:285-322).
+template <int STEP, bool ODD>
+__device__ __forceinline__ void star_accumulate(const StarPix& P, const float* pos, const float* mag, int q,
+                                                float (&acc)[(2 * STEP + 1) * (2 * STEP + 1)][3]) {
+    float st, sp;
+    if (star_test(P, pos, q, st, sp)) star_splat<STEP, ODD>(P, mag, q, st, sp, acc);
+}
+
 // Pixels with more than kLightStars candidates are queued with their
-// candidate count; their candidates are cut into chunks of CH (a multiple of
-// 64, at least kChunk, grown so that the chunks fit the scratch), one
-// wavefront per chunk, and each pixel's chunk partials are summed in chunk
-// order afterwards.  Every queued star is then one of a few candidates of
-// one lane, whatever the pixel it lands in: the long per-star chain
-// (bisection, taps) of a pixel with tens of thousands of candidates is
-// spread over many waves instead of serialised on a few lanes.
+// candidate count and cut into chunks of kChunk candidates, one wavefront
+// per chunk; their chunk indices are taken by an atomic when the pixel is
+// queued (contiguous per pixel).  A pixel's chunk partials are summed in
+// chunk order, so the result does not depend on which waves ran which
+// chunks.  Every queued star is then
+// one of a few candidates of one lane, whatever the pixel it lands in: the
+// long per-star chain (bisection, taps) of a pixel with tens of thousands of
+// candidates is spread over many waves instead of serialised on a few lanes.
 constexpr int kChunk = 256;
+
+// A queued count with this bit set is a pixel whose candidates come from
+// the star tree (a quad across phi = 2 pi).  The tree is walked once for
+// such a pixel (star_tree_ranges) and its listed node ranges are stored, so
+// that every chunk of the pixel walks the stored ranges as a binned pixel
+// walks its bin rows.
+constexpr uint32_t kTreePixel = 0x80000000u;
+// A queued count with this bit set is an odd pixel (star_pixel_odd): its
+// chunks are traced by the ODD instance of star_map_chunk.
+constexpr uint32_t kOddPixel = 0x40000000u;
+
+__device__ __forceinline__ uint32_t queued_count(uint32_t v) { return v & ~(kTreePixel | kOddPixel); }
+__device__ __forceinline__ uint32_t queued_chunks(uint32_t v) { return (queued_count(v) + kChunk - 1) / kChunk; }
+
+// owner of a chunk index that holds no work (taken by a pixel whose chunks
+// did not all fit `cap`, or whose tree ranges did not fit `rcap`; that
+// pixel is traced by its own lane instead)
+constexpr uint32_t kNoOwner = 0xffffffffu;
 
 struct StarQueue {
     uint32_t* pix;         // queued pixel indices
-    uint32_t* cnt;         // their candidate counts
-    uint32_t* cstart;      // first chunk of each queued pixel (exclusive scan)
-    uint32_t* owner;       // queued pixel of each chunk
+    uint32_t* cnt;         // their candidate counts (| kTreePixel)
+    uint32_t* nnode;       // tree pixels: listed node count
+    uint32_t* rbase;       // tree pixels: first stored range
+    int2* ranges;          // stored node ranges (begin, end) of tree pixels
+    unsigned rcap;         // capacity of `ranges`
+    uint32_t* cstart;      // first chunk of each queued pixel
+    uint32_t* owner;       // queued pixel of each chunk (or kNoOwner)
     float* lens;           // per queued pixel: frac, redshift, 4 log10(redshift)
-    unsigned* ctr;         // [0] queued pixels, [1] chunks, [2] chunk size
+    uint32_t* odd;         // pixels for star_map_odd
+    unsigned* ctr;         // [0] queued pixels, [1] chunk indices taken, [2] odd pixels, [3] stored ranges
     float* partial;        // per chunk: FW*FW*3 partial taps
     unsigned cap;          // chunk capacity of `partial`
 };
 
-// All candidates of a pixel in order, into acc (the reference's loop).
-template <int STEP>
-__device__ __forceinline__ void star_seq(const StarPix& P, const SceneK& s,
+// All candidates of a pixel in order, into acc (the reference's loop),
+// kLightStars at a time: the window's candidate indices are collected into
+// this thread's LDS slots `cand`, then traced, so that the kernel holds one
+// copy of the splat whichever of the tree or the bins lists the candidates.
+template <int STEP, bool ODD>
+__device__ __forceinline__ void star_seq(const StarPix& P, const SceneK& s, int count, int* cand,
                                          float (&acc)[(2 * STEP + 1) * (2 * STEP + 1)][3]) {
     const float* pos = P.picheck ? s.star_pos : s.bin_pos;
     const float* mag = P.picheck ? s.star_mag : s.bin_mag;
-    for_ranges(P, s, [&](int b, int e) {
-        for (int q = b; q < e; q++) star_accumulate<STEP>(P, pos, mag, q, acc);
-    });
+    for (int w = 0; w < count; w += kLightStars) {
+        int nq = 0, g = 0;
+        for_ranges(P, s, [&](int b, int e) {
+            const int len = e - b;
+            if (g + len > w && g < w + kLightStars) {
+                const int q0 = b + max(0, w - g), q1 = b + min(len, w + kLightStars - g);
+                for (int q = q0; q < q1; q++) cand[nq++] = q;
+            }
+            g += len;
+        });
+        for (int k = 0; k < nq; k++) star_accumulate<STEP, ODD>(P, pos, mag, cand[k], acc);
+    }
 }
 
 template <int STEP>
@@ -655,121 +776,165 @@ __global__ __launch_bounds__(256) void star_map_light(const float* thphi, const 
                                                       int W, int H, float offset, int lr, const float* area,
                                                       const float* view, float* taps, StarQueue Q) {
     constexpr int FW = 2 * STEP + 1;
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    int i = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= H || j >= W) return;
+    __shared__ int s_cand[256][kLightStars];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    // every lane stays to the end: the queue append below scans over the wave
+    const bool valid = i < H && j < W;
     const size_t NP = (size_t)W * H;
-    const size_t ij = (size_t)i * W + j;
+    const size_t ij = valid ? (size_t)i * W + j : 0;
     float acc[FW * FW][3];
 #pragma unroll
     for (int u = 0; u < FW * FW; u++) acc[u][0] = acc[u][1] = acc[u][2] = 0.f;
     StarPix P;
-    int count = 0;
-    if (star_pixel(thphi, bh, W, i, j, offset, lr, area, view, cam, P)) {
-        for_ranges(P, s, [&](int b, int e) { count += e - b; });
-        if (count <= kLightStars && count > 0) star_seq<STEP>(P, s, acc);
+    int count = 0, nodes = 0;
+    bool odd = false;
+    if (valid && star_pixel(thphi, bh, W, i, j, offset, lr, area, view, cam, P)) {
+        for_ranges(P, s, [&](int b, int e) {
+            count += e - b;
+            nodes++;
+        });
+        odd = count > 0 && star_pixel_odd(P, s);
     }
+    // pixels of many candidates go to the chunks; odd pixels of few to
+    // star_map_odd
     const bool queue = count > kLightStars;
-    // wave-aggregated append of (pixel, count)
+    bool to_odd = odd && !queue;
+    bool self = valid && !queue && !odd;   // this lane writes the pixel's taps
+    bool seq = self && count > 0;          // ... after tracing its candidates
+    // wave-aggregated append of (pixel, count) and of the pixel's chunks
     const unsigned lane = __lane_id();
-    unsigned long long m = __ballot(queue);
+    const unsigned long long m = __ballot(queue);
     if (m) {
         const int leader = __ffsll((long long)m) - 1;
-        unsigned base = 0;
-        if (lane == (unsigned)leader) base = atomicAdd(&Q.ctr[0], (unsigned)__popcll(m));
+        const unsigned nchunk = queue ? ((unsigned)count + kChunk - 1) / kChunk : 0u;
+        const unsigned nrng = queue && P.picheck ? (unsigned)nodes : 0u;
+        unsigned inc = nchunk, rinc = nrng;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            unsigned o = __shfl_up(inc, off), ro = __shfl_up(rinc, off);
+            if ((int)lane >= off) {
+                inc += o;
+                rinc += ro;
+            }
+        }
+        const unsigned tot = __shfl(inc, 63), rtot = __shfl(rinc, 63);
+        unsigned base = 0, cbase = 0, rbase = 0;
+        if (lane == (unsigned)leader) {
+            base = atomicAdd(&Q.ctr[0], (unsigned)__popcll(m));
+            cbase = atomicAdd(&Q.ctr[1], tot);
+            if (rtot) rbase = atomicAdd(&Q.ctr[3], rtot);
+        }
         base = __shfl(base, leader);
+        cbase = __shfl(cbase, leader);
+        rbase = __shfl(rbase, leader);
         if (queue) {
-            unsigned k = base + (unsigned)__popcll(m & ((1ull << lane) - 1));
+            const unsigned k = base + (unsigned)__popcll(m & ((1ull << lane) - 1));
+            const unsigned long long c0 = (unsigned long long)cbase + inc - nchunk;
+            const unsigned long long r0 = (unsigned long long)rbase + rinc - nrng;
             Q.pix[k] = (uint32_t)ij;
-            Q.cnt[k] = (uint32_t)count;
+            Q.nnode[k] = (uint32_t)nodes;
             Q.lens[3 * k] = P.frac;
             Q.lens[3 * k + 1] = P.redshft;
             Q.lens[3 * k + 2] = P.red;
+            Q.cstart[k] = (uint32_t)c0;
+            Q.rbase[k] = (uint32_t)r0;
+            if (c0 + nchunk <= Q.cap && r0 + nrng <= Q.rcap) {
+                Q.cnt[k] = (uint32_t)count | (P.picheck ? kTreePixel : 0u) | (odd ? kOddPixel : 0u);
+                for (unsigned c = 0; c < nchunk; c++) Q.owner[c0 + c] = k;
+            } else {
+                // its chunks or tree ranges do not fit: no chunk work, this
+                // lane (or star_map_odd) traces it
+                Q.cnt[k] = 0;
+                for (unsigned long long c = c0; c < Q.cap && c < c0 + nchunk; c++) Q.owner[c] = kNoOwner;
+                if (odd)
+                    to_odd = true;
+                else
+                    seq = self = true;
+            }
         }
     }
-    if (!queue) store_taps<STEP>(taps, NP, ij, acc);
+    wave_append(Q.odd, &Q.ctr[2], to_odd, (uint32_t)ij);
+    if (seq) star_seq<STEP, false>(P, s, count, s_cand[threadIdx.y * blockDim.x + threadIdx.x], acc);
+    if (self) store_taps<STEP>(taps, NP, ij, acc);
 }
 
-// One workgroup: chunk size (the smallest multiple of 64 >= kChunk that
-// keeps the chunk count within cap), and the exclusive scan of the chunk
-// counts of the queued pixels.
-__global__ __launch_bounds__(1024) void star_chunk_scan(StarQueue Q) {
-    __shared__ unsigned long long s_sum[1024];
-    const unsigned n = Q.ctr[0];
-    const unsigned t = threadIdx.x, T = blockDim.x;
-    const unsigned per = (n + T - 1) / T;
-    const unsigned lo = min(n, t * per), hi = min(n, lo + per);
-    // total candidates -> chunk size
-    unsigned long long c = 0;
-    for (unsigned k = lo; k < hi; k++) c += Q.cnt[k];
-    s_sum[t] = c;
-    __syncthreads();
-    for (unsigned off = T / 2; off > 0; off >>= 1) {
-        if (t < off) s_sum[t] += s_sum[t + off];
-        __syncthreads();
-    }
-    const unsigned long long total = s_sum[0];
-    __syncthreads();
-    unsigned long long ch = kChunk;
-    // chunks <= total / ch + n; grow ch until that fits cap (n <= cap - 64,
-    // so this ends once total / ch < 64)
-    while (total / ch + n > Q.cap) ch += 64;
-    // per-pixel chunk counts, exclusive scan
-    unsigned long long loc = 0;
-    for (unsigned k = lo; k < hi; k++) loc += (Q.cnt[k] + ch - 1) / ch;
-    s_sum[t] = loc;
-    __syncthreads();
-    for (unsigned off = 1; off < T; off <<= 1) {   // Hillis-Steele inclusive scan
-        unsigned long long v = t >= off ? s_sum[t - off] : 0ull;
-        __syncthreads();
-        s_sum[t] += v;
-        __syncthreads();
-    }
-    unsigned long long run = s_sum[t] - loc;
-    for (unsigned k = lo; k < hi; k++) {
-        Q.cstart[k] = (uint32_t)run;
-        run += (Q.cnt[k] + ch - 1) / ch;
-    }
-    if (t == T - 1) {
-        Q.ctr[1] = (unsigned)s_sum[T - 1];
-        Q.ctr[2] = (unsigned)ch;
-    }
-}
-
-// owner[c] = the queued pixel of chunk c (one thread per queued pixel).
-__global__ __launch_bounds__(256) void star_chunk_owner(StarQueue Q) {
-    const unsigned n = Q.ctr[0], nch = Q.ctr[1];
+// One lane per odd pixel (star_pixel_odd) of few candidates: all its
+// candidates, each tap checked for the reference's slow expression.
+template <int STEP>
+__global__ __launch_bounds__(256) void star_map_odd(const float* thphi, const uint8_t* bh, SceneK s, Cam7 cam,
+                                                    int W, int H, float offset, int lr, const float* area,
+                                                    const float* view, float* taps, StarQueue Q) {
+    constexpr int FW = 2 * STEP + 1;
+    __shared__ int s_cand[256][kLightStars];
+    const unsigned n = Q.ctr[2];
+    const size_t NP = (size_t)W * H;
     for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const unsigned c1 = k + 1 < n ? Q.cstart[k + 1] : nch;
-        for (unsigned c = Q.cstart[k]; c < c1; c++) Q.owner[c] = k;
+        const uint32_t ij = Q.odd[k];
+        float acc[FW * FW][3];
+#pragma unroll
+        for (int u = 0; u < FW * FW; u++) acc[u][0] = acc[u][1] = acc[u][2] = 0.f;
+        StarPix P;
+        if (star_pixel(thphi, bh, W, (int)(ij / (uint32_t)W), (int)(ij % (uint32_t)W), offset, lr, area, view,
+                       cam, P)) {
+            int count = 0;
+            for_ranges(P, s, [&](int b, int e) { count += e - b; });
+            star_seq<STEP, true>(P, s, count, s_cand[threadIdx.x], acc);
+        }
+        store_taps<STEP>(taps, NP, ij, acc);
+    }
+}
+
+// One lane per queued tree pixel: walks the star tree once more and stores
+// the listed node ranges, in list order, at the place star_map_light took
+// for them.
+__global__ __launch_bounds__(256) void star_tree_ranges(const float* thphi, int W, float offset, SceneK s,
+                                                        Cam7 cam, StarQueue Q) {
+    const unsigned n = Q.ctr[0];
+    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        if (!(Q.cnt[k] & kTreePixel)) continue;
+        const uint32_t ij = Q.pix[k];
+        StarPix P;
+        star_pixel_geom(thphi, W, (int)(ij / (uint32_t)W), (int)(ij % (uint32_t)W), offset, cam, P);
+        unsigned r = Q.rbase[k];
+        for_ranges(P, s, [&](int b, int e) { Q.ranges[r++] = make_int2(b, e); });
     }
 }
 
 // One wavefront per chunk (grid-stride): lane l takes the chunk's
 // candidates l, l + 64, ...; the taps are reduced over the wave by a fixed
-// xor butterfly into the chunk's partial.  For a binned pixel the wave
-// walks the footprint's bin rows 64 at a time: each lane loads one row's
-// range, a wave scan gives the rows' first candidate numbers (kept in LDS),
-// and each lane finds the row of each of its candidates by bisection.
-template <int STEP>
+// xor butterfly into the chunk's partial, summed per pixel by
+// star_chunk_reduce.  For a binned pixel
+// the wave walks the footprint's bin rows 64 at a time: each lane loads one
+// row's range, a wave scan gives the rows' first candidate numbers (kept in
+// LDS), and each lane finds the row of each of its candidates.
+template <int STEP, bool ODD>
 __global__ __launch_bounds__(256) void star_map_chunk(const float* thphi, const uint8_t* bh, SceneK s, Cam7 cam,
-                                                      int W, float offset, int lr, const float* area,
-                                                      const float* view, StarQueue Q) {
+                                                      int W, int H, float offset, int lr, const float* area,
+                                                      const float* view, float* taps, StarQueue Q) {
     constexpr int FW = 2 * STEP + 1;
     constexpr int NT = FW * FW * 3;
+    constexpr int PER = (NT + 63) / 64;   // tap values per lane
     const unsigned lane = threadIdx.x & 63;
     const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const unsigned nwaves = (gridDim.x * blockDim.x) >> 6;
-    const unsigned n = Q.ctr[0], nch = Q.ctr[1], CH = Q.ctr[2];
+    const unsigned nch = min(Q.ctr[1], Q.cap);
+    const unsigned CH = kChunk;
+    const size_t NP = (size_t)W * H;
     __shared__ int s_beg[4][64];
     __shared__ unsigned s_pre[4][65];
+    // stars found inside, staged until 64 can be splatted by a full wave
+    __shared__ int s_iq[4][128];
+    __shared__ float s_ist[4][128], s_isp[4][128];
     const unsigned wslot = threadIdx.x >> 6;
-    (void)n;
     for (unsigned c = wave; c < nch; c += nwaves) {
         const unsigned lo = Q.owner[c];
+        if (lo == kNoOwner) continue;
+        const uint32_t qc = Q.cnt[lo];
+        if (((qc & kOddPixel) != 0) != ODD) continue;   // the other instance's chunk
         const uint32_t ij = Q.pix[lo];
         const unsigned g0 = (c - Q.cstart[lo]) * CH;
-        const unsigned g1 = min(g0 + CH, Q.cnt[lo]);
+        const unsigned g1 = min(g0 + CH, queued_count(qc));
         const int i = (int)(ij / (uint32_t)W), j = (int)(ij % (uint32_t)W);
         StarPix P;
         star_pixel_geom(thphi, W, i, j, offset, cam, P);
@@ -781,53 +946,133 @@ __global__ __launch_bounds__(256) void star_map_chunk(const float* thphi, const 
         float acc[FW * FW][3];
 #pragma unroll
         for (int u = 0; u < FW * FW; u++) acc[u][0] = acc[u][1] = acc[u][2] = 0.f;
-        if (P.picheck) {
-            unsigned base = 0;   // candidates before the current range
-            for_ranges(P, s, [&](int b, int e) {
-                const unsigned len = (unsigned)(e - b);
-                const unsigned a0 = max(base, g0), a1 = min(base + len, g1);
-                for (unsigned g = a0 + lane; g < a1; g += 64)
-                    star_accumulate<STEP>(P, pos, mag, b + (int)(g - base), acc);
-                base += len;
-            });
-        } else {
+        const bool tree = (qc & kTreePixel) != 0;
+        const uint32_t rbase = tree ? Q.rbase[lo] : 0u;
+        {
+            // The pixel's candidate ranges, 64 at a time: the stored tree
+            // node ranges, or the bin rows r0..r1 cut to their phi spans.
+            // Candidates are tested one per lane; the stars inside are
+            // staged in LDS and splatted 64 at a time, so that the long
+            // splat runs on full waves however few candidates pass.  One
+            // loop alternates filling (up to 64 staged) and splatting, so
+            // that the splat has one call site.
             const int r0 = bin_of(P.mn[0], kBinDT, kBinT), r1 = bin_of(P.mx[0], kBinDT, kBinT);
             const int p0 = bin_of(P.mn[1], kBinDP, kBinP), p1 = bin_of(P.mx[1], kBinDP, kBinP);
-            unsigned base = 0;
-            for (int rb = r0; rb <= r1 && base < g1; rb += 64) {
-                const int r = rb + (int)lane;
-                int b = 0, e = 0;
-                if (r <= r1) {
-                    b = s.bin_start[r * kBinP + p0];
-                    e = s.bin_start[r * kBinP + p1 + 1];
-                }
-                unsigned len = (unsigned)(e - b), inc = len;
+            const int nrange = tree ? (int)Q.nnode[lo] : r1 - r0 + 1;
+            // wave-uniform state: current block, its candidate window
+            // [gb, a1), the candidates before it, staged stars
+            int kb = 0;
+            unsigned base = 0, tot = 0, gb = 0, a1 = 0, nin = 0;
+            bool loaded = false, done = false;
+            int row = 0;   // per lane: range of the block holding its next candidate
+            while (true) {
+                while (!done && nin < 64) {
+                    if (!loaded) {
+                        if (kb >= nrange || base >= g1) {
+                            done = true;
+                            break;
+                        }
+                        const int k = kb + (int)lane;
+                        int b = 0, e = 0;
+                        if (k < nrange) {
+                            if (tree) {
+                                const int2 R = Q.ranges[rbase + (unsigned)k];
+                                b = R.x;
+                                e = R.y;
+                            } else {
+                                const int r = r0 + k;
+                                int q0, q1;
+                                row_span(P, r, p0, p1, q0, q1);
+                                if (q0 <= q1) {
+                                    b = s.bin_start[r * kBinP + q0];
+                                    e = s.bin_start[r * kBinP + q1 + 1];
+                                }
+                            }
+                        }
+                        unsigned len = (unsigned)(e - b), inc = len;
 #pragma unroll
-                for (int off = 1; off < 64; off <<= 1) {
-                    unsigned o = __shfl_up(inc, off);
-                    if ((int)lane >= off) inc += o;
-                }
-                s_beg[wslot][lane] = b;
-                s_pre[wslot][lane + 1] = inc;
-                if (lane == 0) s_pre[wslot][0] = 0;
-                const unsigned tot = __shfl(inc, 63);
-                __builtin_amdgcn_wave_barrier();
-                const unsigned a0 = max(base, g0), a1 = min(base + tot, g1);
-                for (unsigned g = a0 + lane; g < a1; g += 64) {
-                    const unsigned x = g - base;   // candidate number within this row block
-                    int lo2 = 0, hi2 = 63;         // row k with s_pre[k] <= x < s_pre[k+1]
-                    while (lo2 < hi2) {
-                        int mid = (lo2 + hi2 + 1) >> 1;
-                        if (s_pre[wslot][mid] <= x) lo2 = mid;
-                        else hi2 = mid - 1;
+                        for (int off = 1; off < 64; off <<= 1) {
+                            unsigned o = __shfl_up(inc, off);
+                            if ((int)lane >= off) inc += o;
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        s_beg[wslot][lane] = b;
+                        s_pre[wslot][lane + 1] = inc;
+                        if (lane == 0) s_pre[wslot][0] = 0;
+                        tot = __shfl(inc, 63);
+                        __builtin_amdgcn_wave_barrier();
+                        gb = max(base, g0);
+                        a1 = min(base + tot, g1);
+                        // range k with s_pre[k] <= x < s_pre[k+1] for the
+                        // lane's first candidate, by bisection; later ones
+                        // walk on from it (x grows)
+                        row = 0;
+                        if (gb + lane < a1) {
+                            const unsigned x = gb + lane - base;
+                            int hi2 = 63;
+                            while (row < hi2) {
+                                int mid = (row + hi2 + 1) >> 1;
+                                if (s_pre[wslot][mid] <= x) row = mid;
+                                else hi2 = mid - 1;
+                            }
+                        }
+                        loaded = true;
                     }
-                    star_accumulate<STEP>(P, pos, mag, s_beg[wslot][lo2] + (int)(x - s_pre[wslot][lo2]), acc);
+                    if (gb >= a1) {
+                        base += tot;
+                        kb += 64;
+                        loaded = false;
+                        continue;
+                    }
+                    const unsigned g = gb + lane;
+                    bool in = false;
+                    int q = 0;
+                    float st = 0.f, sp = 0.f;
+                    if (g < a1) {
+                        const unsigned x = g - base;   // candidate number within this block
+                        while (s_pre[wslot][row + 1] <= x) row++;
+                        q = s_beg[wslot][row] + (int)(x - s_pre[wslot][row]);
+                        in = star_test(P, pos, q, st, sp);
+                    }
+                    const unsigned long long m = __ballot(in);
+                    if (in) {
+                        const unsigned k = nin + (unsigned)__popcll(m & ((1ull << lane) - 1));
+                        s_iq[wslot][k] = q;
+                        s_ist[wslot][k] = st;
+                        s_isp[wslot][k] = sp;
+                    }
+                    nin += (unsigned)__popcll(m);
+                    gb += 64;
+                    __builtin_amdgcn_wave_barrier();
+                }
+                if (nin == 0) break;
+                const unsigned take = nin < 64 ? nin : 64u;
+                if (lane < take)
+                    star_splat<STEP, ODD>(P, mag, s_iq[wslot][lane], s_ist[wslot][lane], s_isp[wslot][lane],
+                                          acc);
+                const unsigned rest = nin - take;
+                int q2 = 0;
+                float st2 = 0.f, sp2 = 0.f;
+                if (lane < rest) {
+                    q2 = s_iq[wslot][64 + lane];
+                    st2 = s_ist[wslot][64 + lane];
+                    sp2 = s_isp[wslot][64 + lane];
                 }
                 __builtin_amdgcn_wave_barrier();
-                base += tot;
+                if (lane < rest) {
+                    s_iq[wslot][lane] = q2;
+                    s_ist[wslot][lane] = st2;
+                    s_isp[wslot][lane] = sp2;
+                }
+                nin = rest;
+                __builtin_amdgcn_wave_barrier();
+                if (done && nin == 0) break;
             }
         }
-        float* out = Q.partial + (size_t)c * NT;
+        // tap value t (tap t / 3, colour t % 3) goes to lane t % 64, slot t / 64
+        float mine[PER];
+#pragma unroll
+        for (int k = 0; k < PER; k++) mine[k] = 0.f;
 #pragma unroll
         for (int u = 0; u < FW * FW; u++)
 #pragma unroll
@@ -835,25 +1080,34 @@ __global__ __launch_bounds__(256) void star_map_chunk(const float* thphi, const 
                 float v = acc[u][cc];
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                if (lane == 0) out[3 * u + cc] = v;
+                if (lane == (unsigned)((3 * u + cc) & 63)) mine[(3 * u + cc) >> 6] = v;
             }
+        // contiguous per chunk (one coalesced store); the per-tap planes
+        // are written by star_chunk_reduce
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const unsigned t = (unsigned)k * 64 + lane;
+            if (t < (unsigned)NT) Q.partial[(size_t)c * NT + t] = mine[k];
+        }
     }
 }
 
-// One thread per queued pixel: its chunk partials summed in chunk order.
+// One thread per (queued pixel, tap value): the pixel's chunk partials
+// summed in chunk order.
 template <int STEP>
 __global__ __launch_bounds__(256) void star_chunk_reduce(int W, int H, float* taps, StarQueue Q) {
     constexpr int NT = (2 * STEP + 1) * (2 * STEP + 1) * 3;
-    const unsigned n = Q.ctr[0], nch = Q.ctr[1];
+    const unsigned n = Q.ctr[0];
     const size_t NP = (size_t)W * H;
-    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const unsigned c0 = Q.cstart[k], c1 = k + 1 < n ? Q.cstart[k + 1] : nch;
-        const uint32_t ij = Q.pix[k];
-        for (int tix = 0; tix < NT; tix++) {
-            float v = 0.f;
-            for (unsigned c = c0; c < c1; c++) v += Q.partial[(size_t)c * NT + tix];
-            taps[(size_t)tix * NP + ij] = v;
-        }
+    const size_t total = (size_t)n * NT;
+    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (size_t)gridDim.x * blockDim.x) {
+        const unsigned k = (unsigned)(x / NT), t = (unsigned)(x % NT);
+        const unsigned nchunk = queued_chunks(Q.cnt[k]);
+        if (nchunk == 0) continue;   // traced by its own lane
+        const unsigned c0 = Q.cstart[k];
+        float v = 0.f;
+        for (unsigned ch = 0; ch < nchunk; ch++) v += Q.partial[(size_t)(c0 + ch) * NT + t];
+        taps[(size_t)t * NP + Q.pix[k]] = v;
     }
 }
 
@@ -1280,9 +1534,10 @@ size_t frame_bytes(int w, int h, int step) {
     const size_t NP = (size_t)w * h, NC = (size_t)(w + 1) * (h + 1);
     const size_t fw = 2 * (size_t)step + 1;
     // thphi, gap | bh, area x2, star and map images, per-tap star light,
-    // star chunk partials, star queues (4) and lensing (3), env queues (2),
-    // BGRA output
-    return NC * (8 + 4) + NP * (1 + 4 + 4 + 4 + 4 + 12 * fw * fw + 12 * fw * fw + 16 + 12 + 8 + 4) + 4096;
+    // star chunk partials, star queues (6), odd pixels and lensing (3), env queues (2),
+    // tree ranges (NP / 4 + 65536 of 8 B), BGRA output
+    return NC * (8 + 4) + NP * (1 + 4 + 4 + 4 + 4 + 12 * fw * fw + 12 * fw * fw + 28 + 12 + 8 + 2 + 4) + 4096 +
+           65536 * 8;
 }
 
 int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, const float* cam7,
@@ -1346,20 +1601,28 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         StarQueue Q;
         if ((rc = c->buffer("r_star_lens", sizeof(float) * 3 * (NP + 64), &p))) return rc;
         Q.lens = (float*)p;
-        if ((rc = c->buffer("r_star_q", sizeof(uint32_t) * 4 * (NP + 64), &p))) return rc;
+        if ((rc = c->buffer("r_star_odd", sizeof(uint32_t) * (NP + 64), &p))) return rc;
+        Q.odd = (uint32_t*)p;
+        if ((rc = c->buffer("r_star_q", sizeof(uint32_t) * 6 * (NP + 64), &p))) return rc;
         Q.pix = (uint32_t*)p;
         Q.cnt = Q.pix + NP + 64;
         Q.cstart = Q.cnt + NP + 64;
         Q.owner = Q.cstart + NP + 64;
+        Q.nnode = Q.owner + NP + 64;
+        Q.rbase = Q.nnode + NP + 64;
+        // tree pixels are the few whose quad crosses phi = 2 pi
+        Q.rcap = (unsigned)(NP / 4 + 65536);
+        if ((rc = c->buffer("r_star_ranges", sizeof(int2) * (size_t)Q.rcap, &p))) return rc;
+        Q.ranges = (int2*)p;
         if ((rc = c->buffer("r_star_qn", 64, &p))) return rc;
         Q.ctr = (unsigned*)p;
         const int NT = fw * fw * 3;
-        // at least one chunk per queued pixel fits (queued <= NP), so the
-        // chunk-size search in star_chunk_scan ends
+        // chunk indices beyond cap are never used: a pixel whose chunks do
+        // not fit is traced by its own lane in star_map_light
         Q.cap = (unsigned)(NP + 64);
         if ((rc = c->buffer("r_star_part", sizeof(float) * NT * (size_t)Q.cap, &p))) return rc;
         Q.partial = (float*)p;
-        zero_u32_r<<<1, 64, 0, c->stream>>>(Q.ctr, 3);
+        zero_u32_r<<<1, 64, 0, c->stream>>>(Q.ctr, 4);
         const unsigned nwave = (unsigned)c->num_cus * 16, nred = (unsigned)c->num_cus * 4;
 #define BT_STAR_LAUNCH(S)                                                                                    \
     {                                                                                                       \
@@ -1368,11 +1631,25 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
             star_map_light<S><<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr,  \
                                                                   area, v.view, taps, Q);                   \
         }                                                                                                   \
+        {                                                                                                   \
+            KTimer t0(c, "star_odd");                                                                       \
+            star_map_odd<S><<<nred, 256, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area,     \
+                                                         v.view, taps, Q);                                  \
+        }                                                                                                   \
         KTimer t(c, "star_heavy");                                                                          \
-        star_chunk_scan<<<1, 1024, 0, c->stream>>>(Q);                                                      \
-        star_chunk_owner<<<nred, 256, 0, c->stream>>>(Q);                                                   \
-        star_map_chunk<S><<<nwave, 256, 0, c->stream>>>(thphi, bh, s, cam, W, f.offset, f.lr, area, v.view, Q); \
-        star_chunk_reduce<S><<<nred, 256, 0, c->stream>>>(W, H, taps, Q);                                   \
+        {                                                                                                   \
+            KTimer t1(c, "star_tree");                                                                      \
+            star_tree_ranges<<<nred, 256, 0, c->stream>>>(thphi, W, f.offset, s, cam, Q);                   \
+        }                                                                                                   \
+        {                                                                                                   \
+            KTimer t2(c, "star_chunk");                                                                     \
+            star_map_chunk<S, false><<<nwave, 256, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr,  \
+                                                                   area, v.view, taps, Q);                  \
+            star_map_chunk<S, true><<<nred, 256, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr,    \
+                                                                 area, v.view, taps, Q);                    \
+        }                                                                                                   \
+        KTimer t3(c, "star_reduce");                                                                        \
+        star_chunk_reduce<S><<<nwave, 256, 0, c->stream>>>(W, H, taps, Q);                                  \
     }
         if (step == 1)
             BT_STAR_LAUNCH(1)

@@ -186,6 +186,9 @@ static int scene_upload(Context& c, bt_scene* s, const int32_t* tree, int tree_l
     BT_HIP(hipMalloc((void**)&s->bin_pos, sizeof(float) * 2 * (S > 0 ? S : 1)));
     BT_HIP(hipMalloc((void**)&s->bin_mag, sizeof(float) * 2 * (S > 0 ? S : 1)));
     s->bytes = sizeof(int32_t) * (tsz + nb) + sizeof(float) * 8 * (size_t)S + sizeof(float) * 4 * (size_t)rows * cols;
+    s->mag_tame = 1;
+    for (int q = 0; q < S; q++)
+        if (!(mag[2 * q] >= -10.f && mag[2 * q] <= 60.f)) s->mag_tame = 0;
     std::vector<int32_t> bstart(nb);
     std::vector<float> bpos(2 * (size_t)(S > 0 ? S : 1)), bmag(2 * (size_t)(S > 0 ? S : 1));
     star_bins(pos, mag, S, bstart.data(), bpos.data(), bmag.data());
@@ -402,6 +405,11 @@ int bt_render_frame_stages(bt_context* h, const bt_grid* g, const bt_scene* s, c
     if (star) BT_HIP(hipMemcpyAsync(star, dbg.star, sizeof(float) * 3 * NP, hipMemcpyDeviceToHost, c.stream));
     if (out_bgra || want) BT_HIP(hipStreamSynchronize(c.stream));
     return BT_OK;
+}
+
+size_t bt_frame_bytes(int width, int height, int step) {
+    if (width <= 0 || height <= 0 || step < 1) return 0;
+    return bt::frame_bytes(width, height, step);
 }
 
 int bt_render_frame(bt_context* h, const bt_grid* g, const bt_scene* s, const bt_view* v, const float* cam7,

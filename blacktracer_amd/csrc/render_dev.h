@@ -331,6 +331,7 @@ __device__ inline void interpolate_star(float t0, float t1, float t2, float t3, 
     float starInPixY = 0.5f;
     float starInPixX = 0.5f;
     float perc = 0.5f;
+#pragma unroll 1
     for (int q = 0; q < 10; q++) {
         if ((fabsf(start - midT) < error) && (fabsf(starp - midP) < error)) break;
         float half01T = (t0 + t1) * .5f;

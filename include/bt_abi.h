@@ -239,6 +239,10 @@ void bt_view_free(bt_view* v);
 int  bt_render_frame(bt_context* ctx, const bt_grid* g, const bt_scene* s, const bt_view* v,
                      const float* cam7, const bt_render_flags* flags, int step,
                      uint8_t* out_bgra, void* d_out_bgra);
+/* Device bytes a frame of this size works in (intermediates, queues and
+ * output; not the grid, scene or view): what the server's admission
+ * reserves per frame.  No reference counterpart. */
+size_t bt_frame_bytes(int width, int height, int step);
 /* Same, also copying the frame's intermediates to host buffers (any may be
  * null): thphi (h+1)(w+1)*2, gap (h+1)(w+1), bh w*h, smoothed area w*h,
  * summed star light w*h*3.  For parity tests. */

@@ -63,10 +63,10 @@ def test_alone_equals_in_mix_equals_direct(ctx, scene):
 
 
 def test_budget_admission_and_eviction(ctx, scene):
-    # room for one 640x360 frame (about 62 MiB of working set), its view
-    # (2 MiB) and a few level-7 grids (about 1 MiB each): frames run one at
-    # a time and the three poses evict each other
-    budget = 66 << 20
+    # room for one 640x360 frame, its view (641 x 361 corners of 8 B) and
+    # about two level-7 grids (about 1 MiB each): frames run one at a time
+    # and the three poses evict each other
+    budget = bt.frame_bytes(640, 360) + 641 * 361 * 8 + (5 << 19)
     srv = bt.Server(scene, hbm_budget=budget, slots=4, max_level=LEVEL, ctx=ctx)
     reqs = [(POSES[k % 3], 0.1 * k, (640, 360)) for k in range(9)]
     outs, comps = _run(srv, reqs)
@@ -93,7 +93,9 @@ def test_budget_admission_and_eviction(ctx, scene):
 def test_request_that_can_never_fit_fails(ctx, scene):
     """Frame + view + grid above the budget with nothing in flight: the
     request completes with BT_ERR_BUDGET instead of waiting forever."""
-    srv = bt.Server(scene, hbm_budget=63 << 20, slots=2, max_level=LEVEL, ctx=ctx)
+    # the frame alone fits; frame + view does not
+    srv = bt.Server(scene, hbm_budget=bt.frame_bytes(640, 360) + 641 * 361 * 4, slots=2, max_level=LEVEL,
+                    ctx=ctx)
     out = np.zeros((360, 640, 4), dtype=np.uint8)
     srv.submit(*POSES[0], 0.0, out, 7)
     srv.drain(60)
