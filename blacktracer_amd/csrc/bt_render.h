@@ -44,6 +44,7 @@ void star_bins(const float* pos, const float* mag, int S, int32_t* bin_start, fl
 struct ViewK {
     const float* view;
     int W, H;
+    const float* pix_area;   // per-pixel view solid angle, or null (made per frame)
 };
 
 // Optional device copies of intermediates (for parity tests).
@@ -90,7 +91,8 @@ struct bt_scene {
 
 struct bt_view {
     bt::Context* ctx = nullptr;
-    float* view = nullptr;
+    float* view = nullptr;       // (H+1)(W+1) corners, then W*H pixel solid angles
+    float* pix_area = nullptr;   // inside the view block
     int W = 0, H = 0;
     size_t bytes = 0;
 };
@@ -101,4 +103,6 @@ struct GridF;
 int grid_f32_of(Context& c, const bt_grid* g, GridF* out);
 // Device bytes a frame of w x h pixels uses at star-splat step `step`.
 size_t frame_bytes(int w, int h, int step);
+// Fills out (W*H floats) with each pixel's view solid angle, on c's stream.
+int view_pixel_areas(Context* c, const float* view, int W, int H, float* out);
 }  // namespace bt

@@ -35,7 +35,8 @@ __constant__ int kTempRGB[1173] = {
 
 struct Cam7 {
     float v[7];
-    const float* colours;   // star_colours table, 391 x 6 floats
+    const float* colours;    // star_colours table, 391 x 6 floats
+    const float* pix_area;   // per pixel: camera-sky solid angle of its view quad
 };
 
 __device__ __forceinline__ F2 gload(const float* grid, int GN, int GM, int i, int j) {
@@ -353,11 +354,27 @@ __device__ void lensing_redshift(const float* t, const float* p, int W, int ind,
     float2 v0 = v[ind], v1 = v[ind + 1], v2 = v[ind + M1 + 1], v3 = v[ind + M1];
     float ver4[4] = {v0.x, v1.x, v2.x, v3.x};
     float hor4[4] = {v0.y, v1.y, v2.y, v3.y};
-    float pixArea = calc_area4(ver4, hor4);
+    // calc_area4 of these corners, made once per view (view_pixel_areas)
+    const int pi = ind / M1, pj = ind - pi * M1;
+    const float pixArea = cam.pix_area[(size_t)pi * W + pj];
     frac = pixArea / solidAngle;
     float thetaCam = (ver4[0] + ver4[1] + ver4[2] + ver4[3]) * .25f;
     float phiCam = (hor4[0] + hor4[1] + hor4[2] + hor4[3]) * .25f;
     redshft = redshift(thetaCam, phiCam, cam.v);
+}
+
+// The camera-sky solid angle of every pixel's view quad (the pixArea of
+// kernel.cuh:679-680): it depends on the view alone, so a view makes it once.
+__global__ __launch_bounds__(256) void view_pixel_areas(const float* view, int W, int H, float* out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= H || j >= W) return;
+    const int M1 = W + 1, ind = i * M1 + j;
+    const float2* v = reinterpret_cast<const float2*>(view);
+    float2 v0 = v[ind], v1 = v[ind + 1], v2 = v[ind + M1 + 1], v3 = v[ind + M1];
+    float ver4[4] = {v0.x, v1.x, v2.x, v3.x};
+    float hor4[4] = {v0.y, v1.y, v2.y, v3.y};
+    out[(size_t)i * W + j] = calc_area4(ver4, hor4);
 }
 
 // ---- star pass ---------------------------------------------------------
@@ -471,7 +488,7 @@ __device__ bool star_pixel(const float* thphi, const uint8_t* bh, int W, int i, 
     P.redshft = 1.f;
     P.frac = 1.f;
     if (lr) lensing_redshift(t, p, W, ind, cam, view, P.frac, P.redshft, area[ij]);
-    P.red = 4.f * f_log10(P.redshft);
+    P.red = 4.f * log10f(P.redshft);   // device library FP32 (a few ulps)
     return true;
 }
 
@@ -1225,16 +1242,18 @@ __device__ __forceinline__ float4 env_column(const float4* table, int rows, int 
 // Mean colour -> gamma -> HSP -> lensing/redshift -> BGRA (MakeImageKernel.cu:733-750).
 __device__ __forceinline__ uchar4 env_finish(const EnvPix& E, float cx, float cy, float cz, float cw, int W,
                                              int lr, const Cam7& cam, const float* view, float area_ij) {
-    cx = fminf(255.f, f_pow(cx / cw, 1.f / 2.2f));
-    cy = fminf(255.f, f_pow(cy / cw, 1.f / 2.2f));
-    cz = fminf(255.f, f_pow(cz / cw, 1.f / 2.2f));
+    // FP32 pow of the device library (a few ulps, as the reference's own
+    // powf): these values only reach the 8-bit output
+    cx = fminf(255.f, powf(cx / cw, 1.f / 2.2f));
+    cy = fminf(255.f, powf(cy / cw, 1.f / 2.2f));
+    cz = fminf(255.f, powf(cz / cw, 1.f / 2.2f));
     float Hh, Ss, Pp;
     rgb_to_hsp(cz / 255.f, cy / 255.f, cx / 255.f, Hh, Ss, Pp);
     if (lr) {
         float redshft, frac;
         lensing_redshift(E.t, E.p, W, E.ind, cam, view, frac, redshft, area_ij);
         Pp *= frac;
-        Pp = f_pow(Pp, redshft);
+        Pp = powf(Pp, redshft);
     }
     hsp_to_rgb(Hh, Ss, fminf(1.f, Pp), cz, cy, cx);
     uchar4 o;
@@ -1530,6 +1549,12 @@ int render_grid_from_cel(Context* c, const double* d_cel, int GN, int GM, float*
     return BT_OK;
 }
 
+int view_pixel_areas(Context* c, const float* view, int W, int H, float* out) {
+    view_pixel_areas<<<grid2d(W, H), dim3(16, 16), 0, c->stream>>>(view, W, H, out);
+    BT_HIP(hipGetLastError());
+    return BT_OK;
+}
+
 size_t frame_bytes(int w, int h, int step) {
     const size_t NP = (size_t)w * h, NC = (size_t)(w + 1) * (h + 1);
     const size_t fw = 2 * (size_t)step + 1;
@@ -1580,6 +1605,13 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
     }
     GridV gv{g.grid, g.GN, g.GM};
     const dim3 blk(16, 16);
+    if (v.pix_area) {
+        cam.pix_area = v.pix_area;
+    } else {
+        if ((rc = c->buffer("r_pix_area", sizeof(float) * NP, &p))) return rc;
+        cam.pix_area = (const float*)p;
+        view_pixel_areas<<<grid2d(W, H), blk, 0, c->stream>>>(v.view, W, H, (float*)p);
+    }
     {
         KTimer t(c, "pix_interp");
         pix_interp<<<grid2d(W + 1, H + 1), blk, 0, c->stream>>>(gv, g.gridlvl, v.view, W, H, f.hor, f.ver, thphi, gap);

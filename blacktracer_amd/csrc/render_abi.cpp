@@ -297,16 +297,21 @@ int bt_view_create(bt_context* h, double view_angle, double off_up, double off_r
     v->ctx = &c;
     v->W = width;
     v->H = height;
-    v->bytes = sizeof(float) * 2 * n;
+    v->bytes = sizeof(float) * (2 * n + (size_t)width * height);
     if (hipMalloc((void**)&v->view, v->bytes) != hipSuccess) {
         delete v;
         return fail(BT_ERR_NOMEM, "bt_view_create: hipMalloc failed");
     }
-    hipError_t e = hipMemcpy(v->view, host.data(), v->bytes, hipMemcpyHostToDevice);
+    v->pix_area = v->view + 2 * n;
+    hipError_t e = hipMemcpyAsync(v->view, host.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, c.stream);
+    if (e == hipSuccess) {
+        rc = view_pixel_areas(&c, v->view, width, height, v->pix_area);
+        e = rc ? hipErrorUnknown : hipStreamSynchronize(c.stream);
+    }
     if (e != hipSuccess) {
         hipFree(v->view);
         delete v;
-        return fail(BT_ERR_HIP, std::string("bt_view_create: ") + hipGetErrorString(e));
+        return rc ? rc : fail(BT_ERR_HIP, std::string("bt_view_create: ") + hipGetErrorString(e));
     }
     *out = v;
     return BT_OK;
@@ -394,7 +399,7 @@ int bt_render_frame_stages(bt_context* h, const bt_grid* g, const bt_scene* s, c
         off += sizeof(float) * 3 * NP;
         dbg.bh = (uint8_t*)(base + off);
     }
-    if ((rc = render_frame(&c, gf, s->k(), ViewK{v->view, v->W, v->H}, cam, f, step, (uint8_t*)dout,
+    if ((rc = render_frame(&c, gf, s->k(), ViewK{v->view, v->W, v->H, v->pix_area}, cam, f, step, (uint8_t*)dout,
                            want ? &dbg : nullptr)))
         return rc;
     if (out_bgra) BT_HIP(hipMemcpyAsync(out_bgra, dout, 4 * NP, hipMemcpyDeviceToHost, c.stream));
@@ -478,7 +483,7 @@ int bt_make_image(bt_context* h, const float* stars, const int32_t* star_tree_in
         bt_render_flags f;
         bt_render_flags_default(&f);
         if (flags) f = *flags;
-        rc = render_frame(&c, GridF{(const float*)dgrid, gn1, GM, gridlvl}, scene.k(), ViewK{(const float*)dview, M, N},
+        rc = render_frame(&c, GridF{(const float*)dgrid, gn1, GM, gridlvl}, scene.k(), ViewK{(const float*)dview, M, N, nullptr},
                           cam_param, f, step, (uint8_t*)dout, nullptr);
         if (rc) goto done;
     }

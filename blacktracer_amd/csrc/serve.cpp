@@ -266,7 +266,7 @@ int launch(bt_server* s, Slot& sl, CachedGrid& cg, const bt_view* v) {
     f.offset = r.offset;
     float cam[7];
     cam7_of(cg.g->pose, cam);
-    if ((rc = render_frame(&c, gf, s->scene->k(), ViewK{v->view, v->W, v->H}, cam, f, s->cfg.step, (uint8_t*)dout,
+    if ((rc = render_frame(&c, gf, s->scene->k(), ViewK{v->view, v->W, v->H, v->pix_area}, cam, f, s->cfg.step, (uint8_t*)dout,
                            nullptr)))
         return rc;
     BT_HIP(hipMemcpyAsync(sl.staging, dout, nbytes, hipMemcpyDeviceToHost, c.stream));

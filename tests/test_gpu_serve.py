@@ -63,10 +63,10 @@ def test_alone_equals_in_mix_equals_direct(ctx, scene):
 
 
 def test_budget_admission_and_eviction(ctx, scene):
-    # room for one 640x360 frame, its view (641 x 361 corners of 8 B) and
-    # about two level-7 grids (about 1 MiB each): frames run one at a time
-    # and the three poses evict each other
-    budget = bt.frame_bytes(640, 360) + 641 * 361 * 8 + (5 << 19)
+    # room for one 640x360 frame, its view (641 x 361 corners of 8 B and
+    # 640 x 360 pixel solid angles) and about two level-7 grids (about 1 MiB
+    # each): frames run one at a time and the three poses evict each other
+    budget = bt.frame_bytes(640, 360) + 641 * 361 * 8 + 640 * 360 * 4 + (5 << 19)
     srv = bt.Server(scene, hbm_budget=budget, slots=4, max_level=LEVEL, ctx=ctx)
     reqs = [(POSES[k % 3], 0.1 * k, (640, 360)) for k in range(9)]
     outs, comps = _run(srv, reqs)
