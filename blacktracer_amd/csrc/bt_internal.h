@@ -226,6 +226,9 @@ int trace_f64(Context* c, const PoseK& pose, const ParamsK& prm, int64_t n, cons
 // ray leaves after its first attempt with errmax > ERRCON and its state
 // goes to cont (cont.nstp = -1 for a ray that ended during the warm-up);
 // prio is the waves' base issue priority (0..3).
+// Zeroes n (<= 1024) 32-bit words with a kernel (no memset node, so the
+// call can sit in a captured graph).
+int zero_words(unsigned* p, int n, hipStream_t st);
 int queue_alloc(Context* c, int64_t n, QueueK* Q);
 template <class Src>
 int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const QueueK& Q,

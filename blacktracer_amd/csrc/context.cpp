@@ -380,7 +380,7 @@ static int trace_common(bt_context* h, const bt_pose* pose, const bt_integ_param
     void* p;
     if ((rc = c.buffer("trace_stats", sizeof(StatsK), &p))) return rc;
     StatsK* d_stats = (StatsK*)p;
-    BT_HIP(hipMemsetAsync(d_stats, 0, sizeof(StatsK), c.stream));
+    if (int zr = zero_words((unsigned*)d_stats, (int)(sizeof(StatsK) / 4), c.stream)) return zr;
     RayIO io;
     std::memset(&io, 0, sizeof(io));
     io.in_th = d_th;

@@ -433,7 +433,7 @@ int grid_build(Context* c, bt_grid* g, const bt_grid_params* gp) {
     unsigned* ctr = (unsigned*)p;
     if ((rc = c->buffer("grid_stats", sizeof(StatsK), &p))) return rc;
     StatsK* d_stats = (StatsK*)p;
-    BT_HIP(hipMemsetAsync(d_stats, 0, sizeof(StatsK), c->stream));
+    if (int zr = zero_words((unsigned*)d_stats, (int)(sizeof(StatsK) / 4), c->stream)) return zr;
 
     const int64_t nwords = (NM + 31) / 32;
     init_grid<<<(unsigned)std::min<int64_t>(blocks_for(NM, 256), 4096), 256, 0, c->stream>>>(G, NM, nwords);
@@ -547,7 +547,7 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
     unsigned* ctr = (unsigned*)p;
     if ((rc = c->buffer("gbatch_stats", sizeof(StatsK), &p))) return rc;
     StatsK* d_stats = (StatsK*)p;
-    BT_HIP(hipMemsetAsync(d_stats, 0, sizeof(StatsK), c->stream));
+    if (int zr = zero_words((unsigned*)d_stats, (int)(sizeof(StatsK) / 4), c->stream)) return zr;
     const ParamsK& prm = gs[0]->prm;
 
     for (int k = 0; k < G; k++) {

@@ -212,6 +212,14 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
 
 }  // namespace
 
+int zero_words(unsigned* p, int n, hipStream_t st) {
+    if (n <= 0) return BT_OK;
+    if (n > 1024) return fail(BT_ERR_INVALID, "zero_words: at most 1024 words");
+    zero_counters<<<1, n <= 64 ? 64 : (n + 63) / 64 * 64, 0, st>>>(p, n);
+    BT_HIP(hipGetLastError());
+    return BT_OK;
+}
+
 int queue_alloc(Context* c, int64_t n, QueueK* Q) {
     void* p;
     int rc;
