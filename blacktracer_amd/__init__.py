@@ -302,6 +302,10 @@ class Grid:
             self._export()
         return self._levels
 
+    def write(self, path):
+        """A .grid file (Grid::serialize, Grid.h:33-39) of this grid."""
+        check(_lib.load().bt_grid_file_write(str(path).encode(), self.handle))
+
     def psh(self):
         """saveAsGpuHash (Grid.h:642-654): (hash_table, offset_table, hash_pos_tag, hw, ow)."""
         L = _lib.load()
@@ -380,6 +384,24 @@ class Scene:
         t = np.zeros((self.rows, self.cols, 4), dtype=np.float32)
         check(_lib.load().bt_scene_export_table(self.handle, t.ctypes.data_as(_lib.FP)))
         return t
+
+    @classmethod
+    def from_tree(cls, tree, tree_level, star_pos, star_mag, sky_bgr, ctx=None):
+        """A scene from a star tree already built (the arrays of a .star file)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx or default_context()
+        tr = np.ascontiguousarray(tree, dtype=np.int32)
+        pos, mag = _f32(star_pos), _f32(star_mag)
+        sky = np.ascontiguousarray(sky_bgr, dtype=np.uint8)
+        self.rows, self.cols = sky.shape[:2]
+        self.tree_level = tree_level
+        h = C.c_void_p()
+        check(_lib.load().bt_scene_create_tree(self.ctx.handle, tr.ctypes.data_as(_lib.I32P), tree_level,
+                                               pos.ctypes.data_as(_lib.FP), mag.ctypes.data_as(_lib.FP),
+                                               pos.size // 2, sky.ctypes.data_as(_lib.U8P), self.rows,
+                                               self.cols, C.byref(h)))
+        self.handle = h
+        return self
 
     def close(self):
         if getattr(self, "handle", None):
@@ -480,6 +502,63 @@ def make_image(stars, star_tree_arr, cam_param, magnitude, tree_level, M, N, ste
         ts.ctypes.data_as(_lib.I32P), ot.size // 2, ht.size // 2, C.byref(flags),
         out.ctypes.data_as(_lib.U8P)))
     return out
+
+
+# ------------------------------------------- on-disk caches and catalogue
+
+
+def read_star_catalog(text, magnitude_cut):
+    """readStars2 + magnitude cut (StarProcessor.h:34-46,71-96): stars4
+    (S, 4) = theta, phi, magnitude, B-V of the stars at or below the cut."""
+    raw = text.encode() if isinstance(text, str) else bytes(text)
+    L = _lib.load()
+    n = C.c_int()
+    check(L.bt_star_catalog_parse(raw, len(raw), magnitude_cut, None, 0, C.byref(n)))
+    out = np.zeros((n.value, 4), dtype=np.float32)
+    if n.value:
+        check(L.bt_star_catalog_parse(raw, len(raw), magnitude_cut, out.ctypes.data_as(_lib.FP), n.value,
+                                      C.byref(n)))
+    return out
+
+
+def write_star_file(path, tree, tree_level, star_pos, star_mag):
+    """A .star file (StarProcessor::serialize) of star_tree()'s arrays."""
+    tr = np.ascontiguousarray(tree, dtype=np.int32)
+    pos, mag = _f32(star_pos), _f32(star_mag)
+    check(_lib.load().bt_star_file_write(str(path).encode(), tr.ctypes.data_as(_lib.I32P), tree_level,
+                                         pos.ctypes.data_as(_lib.FP), mag.ctypes.data_as(_lib.FP),
+                                         pos.size // 2))
+
+
+def read_star_file(path):
+    """(tree, tree_level, star_pos (S,2), star_mag (S,2)) of a .star file."""
+    L = _lib.load()
+    S, lv = C.c_int(), C.c_int()
+    p = str(path).encode()
+    check(L.bt_star_file_read(p, None, None, None, C.byref(S), C.byref(lv)))
+    tree = np.zeros((1 << (lv.value + 1)) - 1, dtype=np.int32)
+    pos = np.zeros((S.value, 2), dtype=np.float32)
+    mag = np.zeros((S.value, 2), dtype=np.float32)
+    check(L.bt_star_file_read(p, tree.ctypes.data_as(_lib.I32P), pos.ctypes.data_as(_lib.FP),
+                              mag.ctypes.data_as(_lib.FP), C.byref(S), C.byref(lv)))
+    return tree, lv.value, pos, mag
+
+
+def read_grid_file(path):
+    """A .grid file (Grid::serialize): dict of max_level, N, M and the PSH
+    tables (hash_table, offset_table, hash_pos_tag, hw, ow) that make_image takes."""
+    L = _lib.load()
+    v = [C.c_int() for _ in range(5)]
+    p = str(path).encode()
+    check(L.bt_grid_file_read(p, *[C.byref(x) for x in v], None, None, None))
+    ml, N, M, hw, ow = (x.value for x in v)
+    ht = np.zeros(2 * hw * hw, dtype=np.float32)
+    ot = np.zeros(2 * ow * ow, dtype=np.int32)
+    tag = np.zeros(2 * hw * hw, dtype=np.int32)
+    check(L.bt_grid_file_read(p, *[C.byref(x) for x in v], ht.ctypes.data_as(_lib.FP),
+                              ot.ctypes.data_as(_lib.I32P), tag.ctypes.data_as(_lib.I32P)))
+    return {"max_level": ml, "N": N, "M": M, "hash_table": ht, "offset_table": ot, "hash_pos_tag": tag,
+            "hw": hw, "ow": ow}
 
 
 # ---------------------------------------------------------------- serving

@@ -91,6 +91,13 @@ class BtTune(C.Structure):
 # name -> (restype, argtypes); every symbol here is declared in include/bt_abi.h
 SIGNATURES = {
     "bt_frame_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
+    "bt_scene_create_tree": (C.c_int, [C.c_void_p, I32P, C.c_int, FP, FP, C.c_int, U8P, C.c_int, C.c_int,
+                                       C.POINTER(C.c_void_p)]),
+    "bt_star_catalog_parse": (C.c_int, [C.c_char_p, C.c_size_t, C.c_float, FP, C.c_int, C.POINTER(C.c_int)]),
+    "bt_star_file_write": (C.c_int, [C.c_char_p, I32P, C.c_int, FP, FP, C.c_int]),
+    "bt_star_file_read": (C.c_int, [C.c_char_p, I32P, FP, FP, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "bt_grid_file_write": (C.c_int, [C.c_char_p, C.c_void_p]),
+    "bt_grid_file_read": (C.c_int, [C.c_char_p] + [C.POINTER(C.c_int)] * 5 + [FP, I32P, I32P]),
     "bt_tune_get": (C.c_int, [C.c_void_p, C.POINTER(BtTune)]),
     "bt_tune_set": (C.c_int, [C.c_void_p, C.POINTER(BtTune)]),
     "bt_autotune": (C.c_int, [C.c_void_p, C.POINTER(BtPose), C.POINTER(BtIntegParams), C.c_int64,

@@ -258,6 +258,32 @@ int bt_scene_create(bt_context* h, const float* stars4, int star_count, int tree
     return BT_OK;
 }
 
+int bt_scene_create_tree(bt_context* h, const int32_t* tree, int tree_level, const float* pos, const float* mag,
+                         int star_count, const uint8_t* sky_bgr, int sky_rows, int sky_cols, bt_scene** out) {
+    if (!h || !out || !tree || (star_count > 0 && (!pos || !mag)))
+        return fail(BT_ERR_INVALID, "bt_scene_create_tree: null argument");
+    *out = nullptr;
+    if (tree_level < 1 || tree_level > 10 || star_count < 0)
+        return fail(BT_ERR_INVALID, "bt_scene_create_tree: tree_level in [1, 10], star_count >= 0");
+    // the tree's last node holds the star count (cumulative end indices)
+    if (tree[(1 << (tree_level + 1)) - 2] != star_count)
+        return fail(BT_ERR_INVALID, "bt_scene_create_tree: tree does not index star_count stars");
+    Context& c = h->impl;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    BT_HIP(hipSetDevice(c.device));
+    bt_scene* s = new bt_scene();
+    s->ctx = &c;
+    int rc = scene_upload(c, s, tree, tree_level, pos, mag, star_count, sky_bgr, sky_rows, sky_cols);
+    if (rc) {
+        hipStreamSynchronize(c.stream);
+        scene_release(s);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return BT_OK;
+}
+
 int bt_scene_export_table(const bt_scene* s, float* table) {
     if (!s || !table) return fail(BT_ERR_INVALID, "null argument");
     Context& c = *s->ctx;

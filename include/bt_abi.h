@@ -227,6 +227,11 @@ typedef struct bt_view bt_view;     /* pixel-corner directions of one resolution
 
 int  bt_scene_create(bt_context* ctx, const float* stars4, int star_count, int tree_level,
                      const uint8_t* sky_bgr, int sky_rows, int sky_cols, bt_scene** out);
+/* Same from a star tree already built (the arrays of a .star file,
+ * StarProcessor.h:19-30): tree 2^(L+1)-1 ints, pos and mag 2 floats per star. */
+int  bt_scene_create_tree(bt_context* ctx, const int32_t* tree, int tree_level, const float* pos,
+                          const float* mag, int star_count, const uint8_t* sky_bgr, int sky_rows,
+                          int sky_cols, bt_scene** out);
 /* Host copy of the summed-area table, sky_rows*sky_cols*4 floats. */
 int  bt_scene_export_table(const bt_scene* s, float* table);
 void bt_scene_free(bt_scene* s);
@@ -250,6 +255,32 @@ int  bt_render_frame_stages(bt_context* ctx, const bt_grid* g, const bt_scene* s
                             const bt_view* v, const float* cam7, const bt_render_flags* flags,
                             int step, uint8_t* out_bgra, void* d_out_bgra, float* thphi,
                             int32_t* gap, uint8_t* bh, float* area, float* star);
+
+/* ---- the reference's on-disk caches and catalogue (host only) ----
+ * cereal binary archives as the reference writes them (BlackTracer.cpp:20-47):
+ * 32-bit ints raw, vectors as a 64-bit count then the elements, little
+ * endian; float2/int2 elements as x then y. */
+/* readStars2 + the magnitude cut (StarProcessor.h:34-46,71-96): records of
+ * five numbers (magnitude, B-V, ra, dec, ignored; radians) ->
+ * stars4 = {theta = pi/2 - dec, ra (wrapped), magnitude, B-V} of every star
+ * with magnitude <= cut, in file order.  capacity 0: count only. */
+int  bt_star_catalog_parse(const char* text, size_t len, float magnitude_cut, float* stars4,
+                           int capacity, int* count);
+/* .star: StarProcessor::serialize (StarProcessor.h:19-23). */
+int  bt_star_file_write(const char* path, const int32_t* tree, int tree_level, const float* pos,
+                        const float* mag, int star_count);
+/* Null arrays: star_count and tree_level only. */
+int  bt_star_file_read(const char* path, int32_t* tree, float* pos, float* mag, int* star_count,
+                       int* tree_level);
+/* .grid: Grid::serialize (Grid.h:33-39) with its PSHOffsetTable
+ * (PSHOffsetTable.h:44-48), built by bt_grid_export_psh; a uniform grid
+ * (start level == max level) writes empty tables, as the reference does
+ * not build one (Grid.h:623). */
+int  bt_grid_file_write(const char* path, const bt_grid* g);
+/* Null arrays: header only.  hash 2*hw*hw floats, offsets 2*ow*ow ints,
+ * tags 2*hw*hw ints: the bt_make_image table arguments. */
+int  bt_grid_file_read(const char* path, int* max_level, int* N, int* M, int* hash_width,
+                       int* offset_width, float* hash, int32_t* offsets, int32_t* tags);
 
 /* ---- serving: continuous batching of render requests (no counterpart in
  * the reference, whose frame loop renders one frame at a time,
