@@ -338,15 +338,15 @@ def precision_sweep(bt, ctx):
     def timed(level, cam, bh, prm):
         bt.Grid(level, level, True, cam, bh, params=prm, ctx=ctx).close()
         ts = []
-        for _ in range(3):
+        for _ in range(5):
             ctx.sync()
             t0 = time.perf_counter()
             g = bt.Grid(level, level, True, cam, bh, params=prm, ctx=ctx)
             ctx.sync()
             ts.append(time.perf_counter() - t0)
-            if len(ts) < 3:
+            if len(ts) < 5:
                 g.close()
-        return g, sorted(ts)[1]
+        return g, sorted(ts)[2]
 
     cases = [(a, r, LEVEL, "reference") for a in (0.5, 0.9, 0.999) for r in (3.0, 5.0, 20.0)]
     cases.append((0.999, 5.0, 8, "radius-stop"))
@@ -354,9 +354,9 @@ def precision_sweep(bt, ctx):
         bh = bt.BlackHole(a)
         cam = bt.Camera(math.pi / 2, 0.0, r, black_hole=bh)
         mk = (lambda prec: bt.radius_stop_params(prec)) if kind == "radius-stop" else bt.IntegParams
-        res = {}
+        res, shapes = {}, {}
         for name, prec in (("fp64", bt.BT_PREC_FP64), ("mixed", bt.BT_PREC_MIXED)):
-            ctx.autotune(cam, params=mk(prec), n_rays=1 << 20)
+            shapes[name] = ctx.autotune(cam, params=mk(prec), n_rays=1 << 21)
             g, t = timed(level, cam, bh, mk(prec))
             res[name] = (g, t)
         g64, t64 = res["fp64"]
@@ -369,6 +369,7 @@ def precision_sweep(bt, ctx):
         row["fp64_rays_per_s"] = g64.rays_traced / t64
         row["mixed_rays_per_s"] = gm.rays_traced / tm
         row["mixed_over_fp64"] = t64 / tm
+        row["launch_shape"] = shapes
         g64.close()
         gm.close()
         print(json.dumps(row), flush=True)

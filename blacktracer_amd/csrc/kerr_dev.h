@@ -249,6 +249,147 @@ __device__ __forceinline__ double attempt(double a, double asq, double* var, dou
     return errmax;
 }
 
+// ---- the mixed path's FP64 warm-up attempt ------------------------------
+// The warm-up reproduces the FP64 step-size growth of a ray (its decisions
+// errmax <= ERRCON hold with orders of magnitude to spare) and hands its
+// state to the FP32 bulk, which parts from the FP64 path at the 1e-7 level
+// anyway; it needs FP64 accuracy, not the reference's exact roundings.  So
+// it evaluates the regrouped right-hand side of the FP32 kernel in FP64,
+// with reciprocals from the hardware estimate refined by two Newton steps
+// and sin/cos from their Taylor series after a two-constant range reduction,
+// at about a third of the exact attempt's instructions.
+
+// 1/x to about one ulp: the hardware estimate and two Newton-Raphson steps.
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// sin and cos of th: th = n pi/2 + v with pi/2 in two parts, |v| <= pi/4,
+// Taylor series to v^17 / v^18 (truncation below 1e-19 on |v| <= pi/4).
+__device__ __forceinline__ void sincos64(double th, double& s, double& c) {
+    constexpr double PIO2_HI = 1.5707963267948966192;          // pi/2 rounded
+    constexpr double PIO2_LO = 6.123233995736766036e-17;       // pi/2 - PIO2_HI
+    constexpr double TWO_OVER_PI = 0.63661977236758134308;
+    const double n = rint(th * TWO_OVER_PI);
+    double v = fma(-n, PIO2_HI, th);
+    v = fma(-n, PIO2_LO, v);
+    const double v2 = v * v;
+    double ps = 1.0 / 355687428096000.0;                        // 1/17!
+    ps = fma(ps, v2, -1.0 / 1307674368000.0);                  // 1/15!
+    ps = fma(ps, v2, 1.0 / 6227020800.0);                      // 1/13!
+    ps = fma(ps, v2, -1.0 / 39916800.0);                       // 1/11!
+    ps = fma(ps, v2, 1.0 / 362880.0);                          // 1/9!
+    ps = fma(ps, v2, -1.0 / 5040.0);
+    ps = fma(ps, v2, 1.0 / 120.0);
+    ps = fma(ps, v2, -1.0 / 6.0);
+    const double sv = fma(v * v2, ps, v);
+    double pc = 1.0 / 6402373705728000.0;                       // 1/18!
+    pc = fma(pc, v2, -1.0 / 20922789888000.0);                 // 1/16!
+    pc = fma(pc, v2, 1.0 / 87178291200.0);                     // 1/14!
+    pc = fma(pc, v2, -1.0 / 479001600.0);                      // 1/12!
+    pc = fma(pc, v2, 1.0 / 3628800.0);                         // 1/10!
+    pc = fma(pc, v2, -1.0 / 40320.0);
+    pc = fma(pc, v2, 1.0 / 720.0);
+    pc = fma(pc, v2, -1.0 / 24.0);
+    pc = fma(pc, v2, 0.5);
+    const double cv = fma(-v2, pc, 1.0);
+    const int qd = (int)n & 3;
+    const double s0 = (qd & 1) ? cv : sv;
+    const double c0 = (qd & 1) ? sv : cv;
+    s = (qd & 2) ? -s0 : s0;
+    c = ((qd + 1) & 2) ? -c0 : c0;
+}
+
+// Per-ray constants of the warm-up right-hand side.
+struct RayW {
+    double a, asq, b, bsq, q, partR, e, Poff;   // e = 1 - a^2, Poff = a^2 - a b
+    __device__ __forceinline__ void set(double a_, double asq_, double b_, double q_) {
+        a = a_;
+        asq = asq_;
+        b = b_;
+        q = q_;
+        bsq = b_ * b_;
+        partR = q_ + (a_ - b_) * (a_ - b_);
+        e = 1.0 - asq_;
+        Poff = asq_ - a_ * b_;
+    }
+};
+
+// The right-hand side of Metric.h:198-226 regrouped (as the FP32 kernel's):
+// iR = 1/rho^2, iD = 1/Delta, Ph = P/Delta and the null-constraint term X.
+__device__ __forceinline__ void derivs_w(const RayW& C, const double* v, double* o) {
+    const double r = v[0], pr = v[3], pt = v[4];
+    double s, c;
+    sincos64(v[1], s, c);
+    const double rm1 = r - 1.0;
+    const double D = fma(rm1, rm1, -C.e);
+    const double P = fma(r, r, C.Poff);
+    const double c2 = c * c;
+    const double rosq = fma(C.asq, c2, r * r);
+    const double iR = rcp64(rosq), iD = rcp64(D), is = rcp64(s);
+    const double is2 = is * is;
+    const double Ph = P * iD;
+    const double Theta = C.q - c2 * (C.bsq * is2 - C.asq);
+    const double prsq = pr * pr;
+    const double X = pt * pt - Theta + C.partR + prsq * D - P * Ph;
+    const double iR2X = iR * iR * X;
+    o[0] = D * iR * pr;
+    o[1] = iR * pt;
+    o[2] = iR * (C.a * Ph - C.a + C.b * is2);
+    o[3] = iR * (2.0 * r * Ph - rm1 * (Ph * Ph + prsq)) + r * iR2X;
+    o[4] = iR * c * (C.bsq * is2 * is - C.asq * s) - C.asq * c * s * iR2X;
+}
+
+// attempt() with derivs_w; the same tableau and step control.
+__device__ __forceinline__ double attempt_w(const RayW& C, double* var, double& z, double& h, double eps) {
+    double dvdz[5], scal[5], k2[5], k3[5], k4[5], k5[5], k6[5], t[5];
+    derivs_w(C, var, dvdz);
+#pragma unroll
+    for (int i = 0; i < 5; i++) scal[i] = fabs(var[i]) + fabs(dvdz[i] * h) + TINY;
+    const double hh = h;
+#pragma unroll
+    for (int i = 0; i < 5; i++) t[i] = var[i] + b21 * hh * dvdz[i];
+    derivs_w(C, t, k2);
+#pragma unroll
+    for (int i = 0; i < 5; i++) t[i] = var[i] + hh * (b31 * dvdz[i] + b32 * k2[i]);
+    derivs_w(C, t, k3);
+#pragma unroll
+    for (int i = 0; i < 5; i++) t[i] = var[i] + hh * (b41 * dvdz[i] + b42 * k2[i] + b43 * k3[i]);
+    derivs_w(C, t, k4);
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        t[i] = var[i] + hh * (b51 * dvdz[i] + b52 * k2[i] + b53 * k3[i] + b54 * k4[i]);
+    derivs_w(C, t, k5);
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        t[i] = var[i] + hh * (b61 * dvdz[i] + b62 * k2[i] + b63 * k3[i] + b64 * k4[i] + b65 * k5[i]);
+    derivs_w(C, t, k6);
+    double errmax = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        t[i] = var[i] + hh * (c1 * dvdz[i] + c3 * k3[i] + c4 * k4[i] + c6 * k6[i]);
+        double err = hh * (dc1 * dvdz[i] + dc3 * k3[i] + dc4 * k4[i] + dc5 * k5[i] + dc6 * k6[i]);
+        errmax = fmax(errmax, fabs(err) * rcp64(scal[i]));
+    }
+    errmax /= eps;
+    if (errmax <= 1.0) {
+        z += h;
+#pragma unroll
+        for (int i = 0; i < 5; i++) var[i] = t[i];
+        if (errmax > ERRCON)
+            h = SAFETY * h * pow(errmax, PGROW);
+        else
+            h = ADAPTIVE * h;
+    } else {
+        h = fmin(SAFETY * h * pow(errmax, PSHRNK), 0.1 * h);
+    }
+    return errmax;
+}
+
 // Metric.h:176-185 (the while loops are kept: repeated addition of 2*PI
 // rounds differently from a single fmod).
 __device__ __forceinline__ void wrap_to_pi(double& thetaW, double& phiW) {

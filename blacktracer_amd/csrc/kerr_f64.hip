@@ -117,9 +117,9 @@ __global__ __launch_bounds__(256) void ic_capture(Src S, ParamsK prm, int64_t n,
 }
 
 // WARM = false: every ray runs to its end (the plain FP64 path).  WARM =
-// true (mixed precision): a ray runs in FP64 through the initial step-size
-// growth, where the FP64 error estimate is far below ERRCON and an FP32
-// estimate would only see rounding noise; after its first attempt with
+// true (mixed precision): a ray runs in FP64 (attempt_w) through the initial
+// step-size growth, where the FP64 error estimate is far below ERRCON and an
+// FP32 estimate would only see rounding noise; after its first attempt with
 // errmax > ERRCON its state goes to cont.  cont.nstp[ray] = -1 marks a ray
 // that ended here.
 template <bool WARM, class Src>
@@ -172,7 +172,14 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
         if (__ballot(slot >= 0) == 0) break;
         update_wave_prio(prio, slot >= 0, nstp, cur_prio);
         if (slot >= 0) {
-            double em = attempt(lp.a, lp.asq, var, z, h, prm.eps, b, q);
+            double em;
+            if (WARM) {
+                RayW C;
+                C.set(lp.a, lp.asq, b, q);
+                em = attempt_w(C, var, z, h, prm.eps);
+            } else {
+                em = attempt(lp.a, lp.asq, var, z, h, prm.eps, b, q);
+            }
             attempts++;
             const int step = nstp;
             nstp++;
