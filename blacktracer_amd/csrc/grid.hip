@@ -450,29 +450,26 @@ int grid_build(Context* c, bt_grid* g, const bt_grid_params* gp) {
     io.out_step = G.steps;
     g->rays = 0;
 
-    // raytrace(): the pole rays (Grid.h:295-301)
+    // raytrace(): the pole rays (Grid.h:295-301) and integrateFirst(gap)
+    // (Grid.h:325-338) are independent, so they are traced in one batch:
+    // keys [poles | start lattice]; the pole rows are filled afterwards.
     pole_keys<<<1, 64, 0, c->stream>>>(trace_keys, (uint32_t)(N - 1) * (uint32_t)M);
     BT_HIP(hipGetLastError());
     io.keys = trace_keys;
-    if ((rc = trace(c, g->pose, g->prm, 1 + eq, io, d_stats))) return rc;
-    g->rays += 1 + eq;
-    // (stream order: the pole keys are consumed before the buffer is reused)
     const int gap0 = 1 << (max_level - start_level);
     const unsigned npole = (unsigned)(M / gap0);
-    pole_rows<<<blocks_for(npole, 256), 256, 0, c->stream>>>(G, gap0, blocks);
-    BT_HIP(hipGetLastError());
-
-    // integrateFirst(gap)
     unsigned nrows = 0;
     for (int i = gap0; i < N - eq; i += gap0) nrows++;
     const unsigned ncols = (unsigned)(M / gap0);
     const int64_t nlat = (int64_t)nrows * ncols;
     const bool uniform = (max_level == start_level);
     start_lattice<<<blocks_for(nlat, 256), 256, 0, c->stream>>>(
-        G, gap0, nrows, ncols, trace_keys, blocks + npole, uniform ? start_level : -1);
+        G, gap0, nrows, ncols, trace_keys + 1 + eq, blocks + npole, uniform ? start_level : -1);
     BT_HIP(hipGetLastError());
-    if ((rc = trace(c, g->pose, g->prm, nlat, io, d_stats))) return rc;
-    g->rays += nlat;
+    if ((rc = trace(c, g->pose, g->prm, 1 + eq + nlat, io, d_stats))) return rc;
+    g->rays += 1 + eq + nlat;
+    pole_rows<<<blocks_for(npole, 256), 256, 0, c->stream>>>(G, gap0, blocks);
+    BT_HIP(hipGetLastError());
     unsigned nb = npole;
     if (!uniform) {
         unsigned lat_blocks = nrows * ncols;
@@ -589,27 +586,24 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
         }
         return trace_batch(c, b, prm, d_stats);
     };
-    // raytrace(): the pole rays of every grid (Grid.h:295-301)
+    // raytrace() and integrateFirst(gap) of every grid (Grid.h:295-301,
+    // 325-338), in one batch: per grid the keys [poles | start lattice]; the
+    // pole rows are filled once the pole rays are done.
     std::vector<int64_t> counts(G);
-    for (int k = 0; k < G; k++) counts[k] = 1 + gs[k]->eq;
-    if ((rc = run_batch(counts))) return rc;
-    // integrateFirst(gap) of every grid
     for (int k = 0; k < G; k++) {
         bt_grid* g = gs[k];
         const int N = g->N, M = g->M, eq = g->eq;
         const int gap0 = 1 << (g->max_level - g->start_level);
         const unsigned npole = (unsigned)(M / gap0);
-        pole_rows<<<blocks_for(npole, 256), 256, 0, c->stream>>>(g->k, gap0, per[k].blocks);
-        BT_HIP(hipGetLastError());
         unsigned nrows = 0;
         for (int i = gap0; i < N - eq; i += gap0) nrows++;
         const unsigned ncols = (unsigned)(M / gap0);
         const int64_t nlat = (int64_t)nrows * ncols;
         const bool uniform = (g->max_level == g->start_level);
         start_lattice<<<blocks_for(nlat, 256), 256, 0, c->stream>>>(
-            g->k, gap0, nrows, ncols, per[k].keys, per[k].blocks + npole, uniform ? g->start_level : -1);
+            g->k, gap0, nrows, ncols, per[k].keys + 1 + eq, per[k].blocks + npole, uniform ? g->start_level : -1);
         BT_HIP(hipGetLastError());
-        counts[k] = nlat;
+        counts[k] = 1 + eq + nlat;
         per[k].nb = npole;
         if (!uniform) {
             unsigned lat_blocks = nrows * ncols;
@@ -620,6 +614,12 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
         per[k].active = true;
     }
     if ((rc = run_batch(counts))) return rc;
+    for (int k = 0; k < G; k++) {
+        const int gap0 = 1 << (gs[k]->max_level - gs[k]->start_level);
+        const unsigned npole = (unsigned)(gs[k]->M / gap0);
+        pole_rows<<<blocks_for(npole, 256), 256, 0, c->stream>>>(gs[k]->k, gap0, per[k].blocks);
+        BT_HIP(hipGetLastError());
+    }
     // adaptiveBlockIntegration(startLevel), all grids one level per round
     std::vector<unsigned> h_ctr(2 * G);
     while (true) {
