@@ -236,7 +236,7 @@ int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const Q
 template <class Src>
 int trace_f64_queue(Context* c, const Src& S, const ParamsK& prm, int64_t n_max, const QueueK& Q,
                     StatsK* d_stats, const char* timer, const ContK* cont,
-                    hipStream_t stream = nullptr, int prio = 0);
+                    hipStream_t stream = nullptr, int prio = 0, bool fast = false);
 template <class Src>
 int trace_f64_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, StatsK* d_stats);
 // kerr_mixed.hip: FP32 bulk with FP64 set-up, warm-up, fallback and accumulation.

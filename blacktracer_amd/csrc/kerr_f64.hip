@@ -19,6 +19,11 @@ namespace {
 
 using namespace kerr;
 
+// rk_f64 modes: the exact attempt run to the end (the plain FP64 path);
+// attempt_w through the step-size growth only (the mixed path's warm-up);
+// attempt_w run to the end.
+constexpr int kExact = 0, kWarm = 1, kFast = 2;
+
 __global__ void zero_counters(unsigned* c, int n) {
     int i = threadIdx.x;
     if (i < n) c[i] = 0;
@@ -122,9 +127,10 @@ __global__ __launch_bounds__(256) void ic_capture(Src S, ParamsK prm, int64_t n,
 // FP32 estimate would only see rounding noise; after its first attempt with
 // errmax > ERRCON its state goes to cont.  cont.nstp[ray] = -1 marks a ray
 // that ended here.
-template <bool WARM, class Src>
+template <int MODE, class Src>
 __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, StatsK* st,
                                               unsigned chunk, ContK cont, int prio) {
+    constexpr bool WARM = MODE == kWarm;
     const unsigned lane = lane_id();
     int cur_prio = 0;
     update_wave_prio(prio, false, 0, cur_prio);
@@ -173,7 +179,7 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
         update_wave_prio(prio, slot >= 0, nstp, cur_prio);
         if (slot >= 0) {
             double em;
-            if (WARM) {
+            if (MODE != kExact) {
                 RayW C;
                 C.set(lp.a, lp.asq, b, q);
                 em = attempt_w(C, var, z, h, prm.eps);
@@ -262,17 +268,22 @@ int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const Q
 
 template <class Src>
 int trace_f64_queue(Context* c, const Src& S, const ParamsK& prm, int64_t n_max, const QueueK& Q,
-                    StatsK* d_stats, const char* timer, const ContK* cont, hipStream_t stream, int prio) {
+                    StatsK* d_stats, const char* timer, const ContK* cont, hipStream_t stream, int prio,
+                    bool fast) {
     if (!stream) stream = c->stream;
     KTimer t(c, timer, stream);
     Launch L;
     if (cont) {
-        if (int rc = persistent_launch(c, rk_f64<true, Src>, n_max, &L)) return rc;
-        rk_f64<true, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, *cont, prio);
-    } else {
-        if (int rc = persistent_launch(c, rk_f64<false, Src>, n_max, &L)) return rc;
-        rk_f64<false, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, ContK{},
+        if (int rc = persistent_launch(c, rk_f64<kWarm, Src>, n_max, &L)) return rc;
+        rk_f64<kWarm, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, *cont, prio);
+    } else if (fast) {
+        if (int rc = persistent_launch(c, rk_f64<kFast, Src>, n_max, &L)) return rc;
+        rk_f64<kFast, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, ContK{},
                                                                prio);
+    } else {
+        if (int rc = persistent_launch(c, rk_f64<kExact, Src>, n_max, &L)) return rc;
+        rk_f64<kExact, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, ContK{},
+                                                                prio);
     }
     BT_HIP(hipGetLastError());
     return BT_OK;
@@ -283,9 +294,9 @@ template int trace_setup<SingleSrc>(Context*, const SingleSrc&, const ParamsK&, 
 template int trace_setup<MultiSrc>(Context*, const MultiSrc&, const ParamsK&, int64_t, const QueueK&,
                                    const QueueK*, StatsK*);
 template int trace_f64_queue<SingleSrc>(Context*, const SingleSrc&, const ParamsK&, int64_t, const QueueK&,
-                                        StatsK*, const char*, const ContK*, hipStream_t, int);
+                                        StatsK*, const char*, const ContK*, hipStream_t, int, bool);
 template int trace_f64_queue<MultiSrc>(Context*, const MultiSrc&, const ParamsK&, int64_t, const QueueK&,
-                                       StatsK*, const char*, const ContK*, hipStream_t, int);
+                                       StatsK*, const char*, const ContK*, hipStream_t, int, bool);
 
 template <class Src>
 int trace_f64_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, StatsK* d_stats) {

@@ -326,7 +326,7 @@ int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, Sta
     const int64_t crit_lanes = (int64_t)c->num_cus * c->tune.block;
     // the critical rays' waves issue ahead of the bulk's (base priority 1)
     rc = trace_f64_queue(c, S, prm, n < crit_lanes ? n : crit_lanes, Qc, d_stats, "rk_f64_critical",
-                         nullptr, c->aux, 1);
+                         nullptr, c->aux, 1, true);
     // join (also on error, so that nothing is left running on the aux stream)
     auto join = [&]() {
         hipEventRecord(c->ev_join, c->aux);
