@@ -62,6 +62,15 @@ __global__ void zero_u32(unsigned* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
+__global__ void zero_span(unsigned* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Batched builds keep each grid's refinement counters in a 128-byte line of
+// their own: the counters take an atomic per new point, and the grids of a
+// batch refine in one launch.
+constexpr int kCtrStride = 32;
+
 __global__ void pole_keys(uint32_t* keys, uint32_t south) {
     if (threadIdx.x == 0) {
         keys[0] = 0u;
@@ -118,14 +127,55 @@ __device__ __forceinline__ void claim(const GridK& G, uint32_t i, uint32_t j, ui
 }
 
 // adaptiveBlockIntegration's loop body (Grid.h:476-497) with refineCheck
-// (Grid.h:410-445) for one level.
+// (Grid.h:410-445) for one block of one level.
+__device__ __forceinline__ void refine_block(const GridK& G, uint32_t key, int level, double thresh,
+                                             uint32_t* next_blocks, uint32_t* trace, unsigned* counters);
+
+__device__ __forceinline__ void refine_block_wave(const GridK& G, bool live, uint32_t key, int level,
+                                                  double thresh, uint32_t* next_blocks, uint32_t* trace,
+                                                  unsigned* counters, bool uniform);
+
 __global__ void refine_level(GridK G, const uint32_t* blocks, unsigned nb, int level,
                              double thresh, uint32_t* next_blocks, uint32_t* trace,
                              unsigned* counters) {
     unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nb) return;
+    const bool live = q < nb;
+    refine_block_wave(G, live, live ? blocks[q] : 0u, level, thresh, next_blocks, trace, counters, true);
+}
+
+// One level of every grid of a batch in one launch: job g covers threads
+// [item0, item0 + nb) of the grid.
+struct RefineJob {
+    GridK G;
+    const uint32_t* blocks;
+    uint32_t* next;
+    uint32_t* keys;
+    unsigned* ctr;
+    unsigned nb, item0;
+    int level;
+};
+
+__global__ __launch_bounds__(256) void refine_level_multi(const RefineJob* jobs, int njobs, unsigned items,
+                                                          double thresh) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned tc = t < items ? t : items - 1;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].item0 <= tc) lo = mid;
+        else hi = mid - 1;
+    }
+    const RefineJob J = jobs[lo];   // by value: the job's pointers stay in registers
+    const unsigned q = tc - J.item0;
+    const bool live = t < items && q < J.nb;
+    // a wave inside one grid's job shares that grid's counters
+    const bool uniform = __ballot(lo != __shfl(lo, 0)) == 0;
+    refine_block_wave(J.G, live, live ? J.blocks[q] : 0u, J.level, thresh, J.next, J.keys, J.ctr, uniform);
+}
+
+__device__ __forceinline__ void refine_block(const GridK& G, uint32_t key, int level, double thresh,
+                                             uint32_t* next_blocks, uint32_t* trace, unsigned* counters) {
     const uint32_t M = (uint32_t)G.M;
-    uint32_t key = blocks[q];
     uint32_t gap = 1u << (G.max_level - level);
     uint32_t i = key / M;
     uint32_t j = key % M;
@@ -158,6 +208,83 @@ __global__ void refine_level(GridK G, const uint32_t* blocks, unsigned nb, int l
     next_blocks[base + 1] = k * M + j;
     next_blocks[base + 2] = k * M + l;
     next_blocks[base + 3] = i * M + l;
+}
+
+// refine_block for a whole wave (every lane calls it; `live` lanes hold a
+// block).  With `uniform` (all lanes share the counters) the new points and
+// the next level's blocks are appended with one atomic per wave each;
+// otherwise lane by lane.  Which slot a point gets does not matter: the
+// next level traces every point and refines every block.
+__device__ __forceinline__ void refine_block_wave(const GridK& G, bool live, uint32_t key, int level,
+                                                  double thresh, uint32_t* next_blocks, uint32_t* trace,
+                                                  unsigned* counters, bool uniform) {
+    if (!uniform) {
+        if (live) refine_block(G, key, level, thresh, next_blocks, trace, counters);
+        return;
+    }
+    const unsigned lane = __lane_id();
+    const uint32_t M = (uint32_t)G.M;
+    bool refine = false;
+    uint32_t pts[5];
+    unsigned newmask = 0;
+    uint32_t i = 0, j = 0, k = 0, l = 0;
+    if (live) {
+        const uint32_t gap = 1u << (G.max_level - level);
+        i = key / M;
+        j = key % M;
+        k = i + gap / 2;
+        l = j + gap / 2;
+        const uint32_t kk = i + gap, ll = (j + gap) % M;
+        const double* c = G.cel;
+        double th1 = c[2 * (i * M + j)], th2 = c[2 * (kk * M + j)];
+        double th3 = c[2 * (i * M + ll)], th4 = c[2 * (kk * M + ll)];
+        double ph1 = c[2 * (i * M + j) + 1], ph2 = c[2 * (kk * M + j) + 1];
+        double ph3 = c[2 * (i * M + ll) + 1], ph4 = c[2 * (kk * M + ll) + 1];
+        double diag = (th1 - th4) * (th1 - th4) + (ph1 - ph4) * (ph1 - ph4);
+        double diag2 = (th2 - th3) * (th2 - th3) + (ph2 - ph3) * (ph2 - ph3);
+        double mx = (diag < diag2) ? diag2 : diag;   // std::max
+        refine = (level < 6 && mx > 1E-10) || (mx > thresh);
+        if (!refine) {
+            G.level[i * M + j] = (int8_t)level;
+        } else {
+            pts[0] = k * M + j;
+            pts[1] = k * M + l;
+            pts[2] = i * M + l;
+            pts[3] = (i + gap) * M + l;
+            pts[4] = k * M + (j + gap) % M;
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                const uint32_t bit = 1u << (pts[m] & 31);
+                if (!(atomicOr(&G.present[pts[m] >> 5], bit) & bit)) newmask |= 1u << m;
+            }
+        }
+    }
+    // new points: one reservation for the wave
+    const unsigned nnew = (unsigned)__popc(newmask);
+    unsigned inc = nnew;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        unsigned o = __shfl_up(inc, off);
+        if ((int)lane >= off) inc += o;
+    }
+    const unsigned tot = __shfl(inc, 63);
+    unsigned base = 0;
+    if (lane == 0 && tot) base = atomicAdd(&counters[0], tot);
+    base = __shfl(base, 0) + inc - nnew;
+#pragma unroll
+    for (int m = 0; m < 5; m++)
+        if ((newmask >> m) & 1u) trace[base++] = pts[m];
+    // the refined blocks' four children
+    const unsigned long long rm = __ballot(refine);
+    unsigned nbase = 0;
+    if (lane == 0 && rm) nbase = atomicAdd(&counters[1], 4u * (unsigned)__popcll(rm));
+    nbase = __shfl(nbase, 0) + 4u * (unsigned)__popcll(rm & ((1ull << lane) - 1));
+    if (refine) {
+        next_blocks[nbase + 0] = i * M + j;
+        next_blocks[nbase + 1] = k * M + j;
+        next_blocks[nbase + 2] = k * M + l;
+        next_blocks[nbase + 3] = i * M + l;
+    }
 }
 
 // Grid.h:503-504
@@ -540,7 +667,7 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
     std::vector<Per> per(G);
     int rc;
     void* p;
-    if ((rc = c->buffer("gbatch_ctr", sizeof(unsigned) * 2 * 128, &p))) return rc;
+    if ((rc = c->buffer("gbatch_ctr", sizeof(unsigned) * kCtrStride * 128, &p))) return rc;
     unsigned* ctr = (unsigned*)p;
     if ((rc = c->buffer("gbatch_stats", sizeof(StatsK), &p))) return rc;
     StatsK* d_stats = (StatsK*)p;
@@ -621,7 +748,10 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
         BT_HIP(hipGetLastError());
     }
     // adaptiveBlockIntegration(startLevel), all grids one level per round
-    std::vector<unsigned> h_ctr(2 * G);
+    std::vector<unsigned> h_ctr((size_t)kCtrStride * G);
+    std::vector<RefineJob> rjobs;
+    if ((rc = c->buffer("gbatch_rjobs", sizeof(RefineJob) * 128, &p))) return rc;
+    RefineJob* d_rjobs = (RefineJob*)p;
     while (true) {
         bool any = false;
         for (int k = 0; k < G; k++) {
@@ -639,26 +769,44 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
             any = true;
         }
         if (!any) break;
-        zero_u32<<<1, 256, 0, c->stream>>>(ctr, 2 * G);
+        zero_span<<<(kCtrStride * G + 255) / 256, 256, 0, c->stream>>>(ctr, kCtrStride * G);
         BT_HIP(hipGetLastError());
+        rjobs.clear();
+        unsigned items = 0;
         for (int k = 0; k < G; k++) {
             Per& q = per[k];
             if (!q.active) continue;
+            RefineJob j;
+            j.G = gs[k]->k;
+            j.blocks = q.blocks;
+            j.next = q.next;
+            j.keys = q.keys;
+            j.ctr = ctr + kCtrStride * k;
+            j.nb = q.nb;
+            j.item0 = items;
+            j.level = q.level;
+            items += q.nb;
+            rjobs.push_back(j);
+        }
+        // the previous level's launch has finished (the stream was synchronised)
+        BT_HIP(hipMemcpyAsync(d_rjobs, rjobs.data(), sizeof(RefineJob) * rjobs.size(), hipMemcpyHostToDevice,
+                              c->stream));
+        {
             KTimer t(c, "refine_level");
-            refine_level<<<blocks_for(q.nb, 256), 256, 0, c->stream>>>(gs[k]->k, q.blocks, q.nb, q.level, thresh,
-                                                                        q.next, q.keys, ctr + 2 * k);
+            refine_level_multi<<<blocks_for(items, 256), 256, 0, c->stream>>>(d_rjobs, (int)rjobs.size(), items,
+                                                                              thresh);
             BT_HIP(hipGetLastError());
         }
-        BT_HIP(hipMemcpyAsync(h_ctr.data(), ctr, sizeof(unsigned) * 2 * G, hipMemcpyDeviceToHost, c->stream));
+        BT_HIP(hipMemcpyAsync(h_ctr.data(), ctr, sizeof(unsigned) * h_ctr.size(), hipMemcpyDeviceToHost, c->stream));
         BT_HIP(hipStreamSynchronize(c->stream));
-        for (int k = 0; k < G; k++) counts[k] = per[k].active ? (int64_t)h_ctr[2 * k] : 0;
+        for (int k = 0; k < G; k++) counts[k] = per[k].active ? (int64_t)h_ctr[(size_t)kCtrStride * k] : 0;
         if ((rc = run_batch(counts))) return rc;
         for (int k = 0; k < G; k++) {
             Per& q = per[k];
             if (!q.active) continue;
             q.level++;
             std::swap(q.blocks, q.next);
-            q.nb = h_ctr[2 * k + 1];
+            q.nb = h_ctr[(size_t)kCtrStride * k + 1];
         }
     }
     if (fix && (rc = grid_fix_tvertices_multi(c, gs))) return rc;
