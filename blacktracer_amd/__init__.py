@@ -478,6 +478,29 @@ def render_frame(grid, scene, view, flags=None, cam7=None, step=1, stages=False,
     return out, dict(thphi=thphi, gap=gap, bh=bh, area=area, star=star)
 
 
+def render_frame_blend(grid0, grid1, alpha, scene, view, flags=None, cam14=None, step=1,
+                       stages=False, ctx=None):
+    """A frame between the poses of two grids at fraction alpha (BGRA uint8,
+    (h, w, 4)): the reference's G > 1 frame (MakeImageKernel.cu:386-510).
+    With stages, also the pixel-corner celestial angles and the shadow
+    border (2 + 2000 float2)."""
+    ctx = ctx or grid0.ctx
+    W, H = view.width, view.height
+    out = np.zeros((H, W, 4), dtype=np.uint8)
+    flags = flags or RenderFlags()
+    c14 = None if cam14 is None else _f32(cam14).ctypes.data_as(_lib.FP)
+    thphi = np.zeros((H + 1, W + 1, 2), dtype=np.float32) if stages else None
+    border = np.zeros((2 + 2000, 2), dtype=np.float32) if stages else None
+    check(_lib.load().bt_render_frame_blend(
+        ctx.handle, grid0.handle, grid1.handle, float(alpha), scene.handle, view.handle, c14,
+        C.byref(flags), step, out.ctypes.data_as(_lib.U8P), None,
+        None if thphi is None else thphi.ctypes.data_as(_lib.FP),
+        None if border is None else border.ctypes.data_as(_lib.FP)))
+    if stages:
+        return out, dict(thphi=thphi, border=border)
+    return out
+
+
 def make_image(stars, star_tree_arr, cam_param, magnitude, tree_level, M, N, step, sky_bgr,
                viewthing, GM, GN, gridlvl, offset_table, hash_table, hash_pos_tag, hw, ow,
                flags=None, ctx=None):

@@ -159,6 +159,9 @@ SIGNATURES = {
     "bt_render_frame_stages": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, FP,
                                          C.POINTER(BtRenderFlags), C.c_int, U8P, C.c_void_p, FP,
                                          I32P, U8P, FP, FP]),
+    "bt_render_frame_blend": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p,
+                                        C.c_void_p, FP, C.POINTER(BtRenderFlags), C.c_int, U8P,
+                                        C.c_void_p, FP, FP]),
 }
 
 _lib = None

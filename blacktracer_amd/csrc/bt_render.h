@@ -60,8 +60,19 @@ int render_sum_table(Context* c, const uint8_t* d_sky, int rows, int cols, float
 int render_grid_from_psh(Context* c, const float* d_ht, const int32_t* d_ot, const int32_t* d_tag, int hw,
                          int ow, int GN, int GM, float* d_grid, int* GN1_out);
 int render_grid_from_cel(Context* c, const double* d_cel, int GN, int GM, float* d_grid, int* GN1_out);
+// Shadow-border directions of multi-grid blending (MakeImageKernel.cu:1257).
+constexpr int kAngleNum = 1000;
+// The second grid of a blended (G > 1) frame; render_frame fills border, a
+// device array of 2 + 2 * kAngleNum float2, from the two grids.
+struct Blend {
+    GridF b;
+    float alpha;
+    float* border;
+};
+
 int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, const float* cam7,
-                 const bt_render_flags& f, int step, uint8_t* d_out, FrameDebug* dbg);
+                 const bt_render_flags& f, int step, uint8_t* d_out, FrameDebug* dbg,
+                 const Blend* blend = nullptr);
 
 // Host-side inputs (Viewer.h, StarProcessor.h).
 int viewer_matrix(double view_angle, double off_up, double off_right, int w, int h, int sphere, float* out);

@@ -261,6 +261,163 @@ __global__ __launch_bounds__(256) void pix_interp(GridV g, int gridlvl, const fl
     gap[c] = gs;
 }
 
+// ---- multi-grid blending (G > 1): a frame between the poses of two grids
+// A and B at fraction alpha (MakeImageKernel.cu:386-456,466-510).
+//
+// border (float2): [0] = (min, max) of pi/GN * i over the points captured in
+// either grid, [1] the same of pi/GN * j (findBhCenter), both starting at
+// (100, 0) (MakeImageKernel.cu:1308-1310); then for direction s, [2+2s] the
+// shadow border of A and [3+2s] that of B (findBhBorders), box-smoothed
+// four times (smoothBorder).  GN is the rows of the whole-sky grid.
+
+// min i, max i, min j, max j of the captured points (ints: pi/GN * i is
+// non-decreasing in i, so its extremes are those of i)
+__global__ void bh_extent_init(int* ext) {
+    if (threadIdx.x == 0) {
+        ext[0] = 0x7fffffff;
+        ext[1] = -1;
+        ext[2] = 0x7fffffff;
+        ext[3] = -1;
+    }
+}
+
+__global__ __launch_bounds__(256) void bh_extent(GridV a, GridV b, int* ext) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const bool cap = i < a.GN && j < a.GM && (a.at(i, j).x == -1.f || b.at(i, j).x == -1.f);
+    int v[4] = {cap ? i : 0x7fffffff, cap ? i : -1, cap ? j : 0x7fffffff, cap ? j : -1};
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        v[0] = min(v[0], __shfl_xor(v[0], off));
+        v[1] = max(v[1], __shfl_xor(v[1], off));
+        v[2] = min(v[2], __shfl_xor(v[2], off));
+        v[3] = max(v[3], __shfl_xor(v[3], off));
+    }
+    if ((threadIdx.y * blockDim.x + threadIdx.x) % 64 == 0 && v[1] >= 0) {
+        atomicMin(&ext[0], v[0]);
+        atomicMax(&ext[1], v[1]);
+        atomicMin(&ext[2], v[2]);
+        atomicMax(&ext[3], v[3]);
+    }
+}
+
+// findBhCenter's result from the extents, then findBhBorders: from the
+// centre, step one grid unit at a time along direction s until a captured
+// point is followed by a visible one.  The reference walks without a bound
+// (and reads outside its grid when the walk leaves it); here the walk stops
+// at the grid's edge.
+__global__ __launch_bounds__(256) void bh_borders(GridV a, GridV b, const int* ext, float2* border) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * kAngleNum) return;
+    const float gridsize = BT_FPI / (1.f * a.GN);
+    float2 b0 = make_float2(100.f, 0.f), b1 = make_float2(100.f, 0.f);
+    if (ext[1] >= 0) {
+        b0 = make_float2(fminf(100.f, gridsize * (float)ext[0]), fmaxf(0.f, gridsize * (float)ext[1]));
+        b1 = make_float2(fminf(100.f, gridsize * (float)ext[2]), fmaxf(0.f, gridsize * (float)ext[3]));
+    }
+    if (i == 0) {
+        border[0] = b0;
+        border[1] = b1;
+    }
+    const int ii = i / 2;
+    const float angle = BT_FPI2 / (1.f * kAngleNum) * 1.f * ii;
+    const float thetaChange = -f_sin(angle);
+    const float phiChange = f_cos(angle);
+    float ptx = .5f * b0.x + .5f * b0.y, pty = .5f * b1.x + .5f * b1.y;
+    ptx = ptx / BT_FPI2 * (float)a.GM;
+    pty = pty / BT_FPI2 * (float)a.GM;
+    const GridV& g = (i % 2) ? b : a;
+    float gA = -2.f, gB = -2.f;
+    while (!(gA > 0 && gB == -1.f)) {
+        gB = gA;
+        ptx += thetaChange;
+        pty += phiChange;
+        const int gx = cvt_i32(ptx), gy = cvt_i32(pty);
+        if (gx < 0 || gx >= g.GN || gy < 0 || gy >= g.GM) break;
+        gA = g.at(gx, gy).x;
+    }
+    border[2 + i] = make_float2((ptx - thetaChange) * BT_FPI2 / (1.f * a.GM), (pty - phiChange) * BT_FPI2 / (1.f * a.GM));
+}
+
+// smoothBorder (MakeImageKernel.cu:444-456): mean of each border point and
+// its two neighbours of the same grid
+__global__ __launch_bounds__(256) void smooth_border(const float2* src, float2* dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * kAngleNum) return;
+    if (i == 0) {
+        dst[0] = src[0];
+        dst[1] = src[1];
+    }
+    const int prev = (i - 2 + 2 * kAngleNum) % (2 * kAngleNum);
+    const int next = (i + 2) % (2 * kAngleNum);
+    dst[i + 2] = make_float2(1.f / 3.f * (src[prev + 2].x + src[i + 2].x + src[next + 2].x),
+                             1.f / 3.f * (src[prev + 2].y + src[i + 2].y + src[next + 2].y));
+}
+
+// pixInterpolation with Gr > 1 (MakeImageKernel.cu:466-510).  Inside the
+// stretch circle around the shadow the two grids are sampled at positions
+// warped so that their shadow borders meet the blended border.  The
+// reference's direction slot can reach angleNum when the angle rounds to
+// 2 pi (one entry past its border array); here it is clamped.
+__global__ __launch_bounds__(256) void pix_interp_blend(GridV ga, GridV gb, int gridlvl, const float2* border,
+                                                        float alpha, const float* view, int W, int H, float hor,
+                                                        float ver, float* thphi, int* gap) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > H || j > W) return;
+    size_t c = (size_t)i * (W + 1) + j;
+    float theta = view[2 * c] + ver;
+    float phi = fmodf(view[2 * c + 1] + hor + BT_FPI2, BT_FPI2);
+    const float2 b0 = border[0], b1 = border[1];
+    const float cx = .5f * b0.x + .5f * b0.y, cy = .5f * b1.x + .5f * b1.y;
+    const float stretchRad = fmaxf(b0.y - b0.x, b1.x - b1.y) * 0.75f;
+    const float centerdist = (theta - cx) * (theta - cx) + (phi - cy) * (phi - cy);
+    int gs = 0;
+    F2 A, B;
+    if (centerdist < stretchRad * stretchRad) {
+        float angle = f_atan2(cx - theta, phi - cy);
+        angle = fmodf(angle + BT_FPI2, BT_FPI2);
+        int slot = cvt_i32(angle / BT_FPI2 * (float)kAngleNum);
+        slot = slot < 0 ? 0 : (slot >= kAngleNum ? kAngleNum - 1 : slot);
+        const float2 bA = border[2 * slot + 2], bB = border[2 * slot + 3];
+        const float nx = (1.f - alpha) * bA.x + alpha * bB.x;
+        const float ny = (1.f - alpha) * bA.y + alpha * bB.y;
+        if (centerdist <= (nx - cx) * (nx - cx) + (ny - cy) * (ny - cy)) {
+            // inside the blended shadow; the reference leaves gap unwritten
+            // here, and only shadow pixels touch this corner
+            thphi[2 * c] = -1.f;
+            thphi[2 * c + 1] = -1.f;
+            gap[c] = 0;
+            return;
+        }
+        const float tStoB = (cx - stretchRad * f_sin(angle) - nx);
+        const float pStoB = (cy + stretchRad * f_cos(angle) - ny);
+        // 1E-5 is a double in the reference: the comparison is in double
+        const float thetaPerc = (double)fabsf(tStoB) < 1E-5 ? 0.f : 1.f - (theta - nx) / tStoB;
+        const float phiPerc = (double)fabsf(pStoB) < 1E-5 ? 0.f : 1.f - (phi - ny) / pStoB;
+        const float thetaA = theta - thetaPerc * (nx - bA.x);
+        const float phiA = phi - phiPerc * (ny - bA.y);
+        const float thetaB = theta - thetaPerc * (nx - bB.x);
+        const float phiB = phi - phiPerc * (ny - bB.y);
+        A = interpolate_pix(ga, thetaA, phiA, gridlvl, gs);
+        B = interpolate_pix(gb, thetaB, phiB, gridlvl, gs);
+    } else {
+        A = interpolate_pix(ga, theta, phi, gridlvl, gs);
+        B = interpolate_pix(gb, theta, phi, gridlvl, gs);
+    }
+    F2 r;
+    if (A.x == -1.f || B.x == -1.f) {
+        r = F2{-1.f, -1.f};
+    } else {
+        if (A.y < .2f * BT_FPI2 && B.y > .8f * BT_FPI2) A.y += BT_FPI2;
+        if (B.y < .2f * BT_FPI2 && A.y > .8f * BT_FPI2) B.y += BT_FPI2;
+        r = F2{(1.f - alpha) * A.x + alpha * B.x, fmodf((1.f - alpha) * A.y + alpha * B.y, BT_FPI2)};
+    }
+    thphi[2 * c] = r.x;
+    thphi[2 * c + 1] = r.y;
+    gap[c] = gs;   // the second lookup's, as the reference's writes land in that order
+}
+
 // kernel.cuh:652-672; false when a corner is negative (shadow).
 __device__ __forceinline__ bool corners(const float* thphi, int W, int ind, float* t, float* p,
                                         bool& picheck, float offset) {
@@ -1566,7 +1723,7 @@ size_t frame_bytes(int w, int h, int step) {
 }
 
 int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, const float* cam7,
-                 const bt_render_flags& f, int step, uint8_t* d_out, FrameDebug* dbg) {
+                 const bt_render_flags& f, int step, uint8_t* d_out, FrameDebug* dbg, const Blend* blend) {
     const int W = v.W, H = v.H;
     const size_t NP = (size_t)W * H, NC = (size_t)(W + 1) * (H + 1);
     if (step != 1 && step != 2 && step != 3)
@@ -1577,6 +1734,9 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         return fail(BT_ERR_INVALID, "render: offset must be finite and >= 0; hor/ver finite");
     if (W <= 0 || H <= 0 || g.GN < 2 || g.GM < 2 || s.rows <= 0 || s.cols <= 0)
         return fail(BT_ERR_INVALID, "render: empty frame, grid or sky");
+    if (blend && (blend->b.GN != g.GN || blend->b.GM != g.GM || blend->b.gridlvl != g.gridlvl || !blend->border ||
+                  !(blend->alpha >= 0.f && blend->alpha <= 1.f)))
+        return fail(BT_ERR_INVALID, "render: blended grids must have equal sizes and levels; alpha in [0, 1]");
     void* p;
     int rc;
     if ((rc = c->buffer("r_thphi", sizeof(float) * 2 * NC, &p))) return rc;
@@ -1612,9 +1772,30 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         cam.pix_area = (const float*)p;
         view_pixel_areas<<<grid2d(W, H), blk, 0, c->stream>>>(v.view, W, H, (float*)p);
     }
-    {
+    if (!blend) {
         KTimer t(c, "pix_interp");
         pix_interp<<<grid2d(W + 1, H + 1), blk, 0, c->stream>>>(gv, g.gridlvl, v.view, W, H, f.hor, f.ver, thphi, gap);
+    } else {
+        GridV gb{blend->b.grid, blend->b.GN, blend->b.GM};
+        float2* border = reinterpret_cast<float2*>(blend->border);
+        {
+            // findBhCenter, findBhBorders, smoothBorder x 4 (MakeImageKernel.cu:1429-1442)
+            KTimer t(c, "blend_border");
+            if ((rc = c->buffer("r_blend_tmp", sizeof(float2) * (2 + 2 * kAngleNum) + 64, &p))) return rc;
+            float2* tmp = (float2*)p;
+            int* ext = reinterpret_cast<int*>(tmp + 2 + 2 * kAngleNum);
+            bh_extent_init<<<1, 64, 0, c->stream>>>(ext);
+            bh_extent<<<grid2d(g.GM, g.GN), blk, 0, c->stream>>>(gv, gb, ext);
+            const unsigned nb = (2 * kAngleNum + 255) / 256;
+            bh_borders<<<nb, 256, 0, c->stream>>>(gv, gb, ext, border);
+            smooth_border<<<nb, 256, 0, c->stream>>>(border, tmp);
+            smooth_border<<<nb, 256, 0, c->stream>>>(tmp, border);
+            smooth_border<<<nb, 256, 0, c->stream>>>(border, tmp);
+            smooth_border<<<nb, 256, 0, c->stream>>>(tmp, border);
+        }
+        KTimer t(c, "pix_interp");
+        pix_interp_blend<<<grid2d(W + 1, H + 1), blk, 0, c->stream>>>(gv, gb, g.gridlvl, border, blend->alpha, v.view,
+                                                                      W, H, f.hor, f.ver, thphi, gap);
     }
     {
         KTimer t(c, "bh_area");

@@ -385,9 +385,16 @@ int grid_f32_of(Context& c, const bt_grid* g, GridF* out) {
 
 extern "C" {
 
-int bt_render_frame_stages(bt_context* h, const bt_grid* g, const bt_scene* s, const bt_view* v, const float* cam7,
-                           const bt_render_flags* flags, int step, uint8_t* out_bgra, void* d_out_bgra,
-                           float* thphi, int32_t* gap, uint8_t* bh, float* area, float* star) {
+}  // extern "C"
+
+namespace {
+
+// One frame from a device-resident grid, or (g1 non-null) blended between
+// grids g and g1 at alpha with cam = 14 floats (the two poses' parameter
+// arrays), as the G > 1 frame of the reference.
+int frame_impl(bt_context* h, const bt_grid* g, const bt_grid* g1, float alpha, const bt_scene* s, const bt_view* v,
+               const float* cam_in, const bt_render_flags* flags, int step, uint8_t* out_bgra, void* d_out_bgra,
+               float* thphi, int32_t* gap, uint8_t* bh, float* area, float* star, float* border_out) {
     if (!h || !g || !s || !v) return fail(BT_ERR_INVALID, "bt_render_frame: null handle");
     if (!out_bgra && !d_out_bgra) return fail(BT_ERR_INVALID, "bt_render_frame: no output buffer");
     Context& c = h->impl;
@@ -397,13 +404,36 @@ int bt_render_frame_stages(bt_context* h, const bt_grid* g, const bt_scene* s, c
     bt_render_flags_default(&f);
     if (flags) f = *flags;
     float cam[7];
-    if (cam7)
-        std::memcpy(cam, cam7, sizeof(cam));
-    else
-        cam7_of(g->pose, cam);
+    if (!g1) {
+        if (cam_in)
+            std::memcpy(cam, cam_in, sizeof(cam));
+        else
+            cam7_of(g->pose, cam);
+    } else {
+        // camUpdate (MakeImageKernel.cu:386-389)
+        float c0[7], c1[7];
+        if (cam_in) {
+            std::memcpy(c0, cam_in, sizeof(c0));
+            std::memcpy(c1, cam_in + 7, sizeof(c1));
+        } else {
+            cam7_of(g->pose, c0);
+            cam7_of(g1->pose, c1);
+        }
+        for (int q = 0; q < 7; q++) cam[q] = (1.f - alpha) * c0[q] + alpha * c1[q];
+    }
     GridF gf;
     int rc;
     if ((rc = grid_f32_of(c, g, &gf))) return rc;
+    Blend blend{};
+    if (g1) {
+        if (g1->N != g->N || g1->M != g->M || g1->max_level != g->max_level)
+            return fail(BT_ERR_INVALID, "bt_render_frame_blend: the grids differ in size or level");
+        if ((rc = grid_f32_of(c, g1, &blend.b))) return rc;
+        void* p;
+        if ((rc = c.buffer("r_blend_border", sizeof(float) * 2 * (2 + 2 * kAngleNum), &p))) return rc;
+        blend.border = (float*)p;
+        blend.alpha = alpha;
+    }
     const size_t NP = (size_t)v->W * v->H, NC = (size_t)(v->W + 1) * (v->H + 1);
     void* dout = d_out_bgra;
     if (!dout && (rc = c.buffer("r_out", 4 * NP, &dout))) return rc;
@@ -426,16 +456,39 @@ int bt_render_frame_stages(bt_context* h, const bt_grid* g, const bt_scene* s, c
         dbg.bh = (uint8_t*)(base + off);
     }
     if ((rc = render_frame(&c, gf, s->k(), ViewK{v->view, v->W, v->H, v->pix_area}, cam, f, step, (uint8_t*)dout,
-                           want ? &dbg : nullptr)))
+                           want ? &dbg : nullptr, g1 ? &blend : nullptr)))
         return rc;
+    if (border_out && g1)
+        BT_HIP(hipMemcpyAsync(border_out, blend.border, sizeof(float) * 2 * (2 + 2 * kAngleNum), hipMemcpyDeviceToHost,
+                              c.stream));
     if (out_bgra) BT_HIP(hipMemcpyAsync(out_bgra, dout, 4 * NP, hipMemcpyDeviceToHost, c.stream));
     if (thphi) BT_HIP(hipMemcpyAsync(thphi, dbg.thphi, sizeof(float) * 2 * NC, hipMemcpyDeviceToHost, c.stream));
     if (gap) BT_HIP(hipMemcpyAsync(gap, dbg.gap, sizeof(int) * NC, hipMemcpyDeviceToHost, c.stream));
     if (bh) BT_HIP(hipMemcpyAsync(bh, dbg.bh, NP, hipMemcpyDeviceToHost, c.stream));
     if (area) BT_HIP(hipMemcpyAsync(area, dbg.area, sizeof(float) * NP, hipMemcpyDeviceToHost, c.stream));
     if (star) BT_HIP(hipMemcpyAsync(star, dbg.star, sizeof(float) * 3 * NP, hipMemcpyDeviceToHost, c.stream));
-    if (out_bgra || want) BT_HIP(hipStreamSynchronize(c.stream));
+    if (out_bgra || want || border_out) BT_HIP(hipStreamSynchronize(c.stream));
     return BT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bt_render_frame_stages(bt_context* h, const bt_grid* g, const bt_scene* s, const bt_view* v, const float* cam7,
+                           const bt_render_flags* flags, int step, uint8_t* out_bgra, void* d_out_bgra,
+                           float* thphi, int32_t* gap, uint8_t* bh, float* area, float* star) {
+    return frame_impl(h, g, nullptr, 0.f, s, v, cam7, flags, step, out_bgra, d_out_bgra, thphi, gap, bh, area, star,
+                      nullptr);
+}
+
+int bt_render_frame_blend(bt_context* h, const bt_grid* g0, const bt_grid* g1, float alpha, const bt_scene* s,
+                          const bt_view* v, const float* cam14, const bt_render_flags* flags, int step,
+                          uint8_t* out_bgra, void* d_out_bgra, float* thphi, float* border) {
+    if (!g1) return fail(BT_ERR_INVALID, "bt_render_frame_blend: second grid is null");
+    if (!(alpha >= 0.f && alpha <= 1.f)) return fail(BT_ERR_INVALID, "bt_render_frame_blend: alpha must be in [0, 1]");
+    return frame_impl(h, g0, g1, alpha, s, v, cam14, flags, step, out_bgra, d_out_bgra, thphi, nullptr, nullptr,
+                      nullptr, nullptr, border);
 }
 
 size_t bt_frame_bytes(int width, int height, int step) {

@@ -196,7 +196,9 @@ void bt_render_flags_default(bt_render_flags* f);
 /* makeImage (MakeImageKernel.cu:1206) with cv::Mat csImage replaced by a
  * BGR8 buffer and the PNG written by the reference replaced by out_bgra
  * (N*M*4 bytes, B,G,R,255 per pixel).  Arrays of int2/float2 are passed as
- * interleaved pairs.  G must be 1 (multi-grid blending is not built). */
+ * interleaved pairs.  G must be 1: the reference renders its G > 1 frames
+ * only in loops that do not run from makeImage (movielength = 0,
+ * MakeImageKernel.cu:1410); bt_render_frame_blend renders them. */
 int bt_make_image(bt_context* ctx, const float* stars, const int32_t* star_tree,
                   int star_size, const float* cam_param, const float* magnitude,
                   int tree_level, int M, int N, int step, const uint8_t* sky_bgr,
@@ -244,6 +246,20 @@ void bt_view_free(bt_view* v);
 int  bt_render_frame(bt_context* ctx, const bt_grid* g, const bt_scene* s, const bt_view* v,
                      const float* cam7, const bt_render_flags* flags, int step,
                      uint8_t* out_bgra, void* d_out_bgra);
+/* Multi-grid camera-motion blending: the G > 1 frame of the reference
+ * (pixInterpolation with Gr > 1, camUpdate, findBhCenter, findBhBorders,
+ * smoothBorder; MakeImageKernel.cu:386-510,1415-1451), which its display
+ * loop renders between two cached grids (MakeImageKernel.cu:986-1013).  The
+ * frame lies between the poses of g0 and g1 at fraction alpha in [0, 1]:
+ * the camera parameters are blended linearly (cam14: the two poses' 7-float
+ * arrays; null: the grids' own poses), and near the shadow both grids are
+ * sampled at positions warped so that their shadow borders (1000
+ * directions) meet.  The grids must have equal sizes and levels.  Optional
+ * outputs: thphi (h+1)(w+1)*2, border (2 + 2*1000)*2 floats. */
+int  bt_render_frame_blend(bt_context* ctx, const bt_grid* g0, const bt_grid* g1, float alpha,
+                           const bt_scene* s, const bt_view* v, const float* cam14,
+                           const bt_render_flags* flags, int step, uint8_t* out_bgra,
+                           void* d_out_bgra, float* thphi, float* border);
 /* Device bytes a frame of this size works in (intermediates, queues and
  * output; not the grid, scene or view): what the server's admission
  * reserves per frame.  No reference counterpart. */

@@ -145,6 +145,18 @@ int    btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* v
                   int sky_rows, int sky_cols, int star, int map, int lr, float hor, float ver,
                   float offset, int step, uint8_t* out_bgra, float* o_thphi, int32_t* o_gap,
                   uint8_t* o_bh, float* o_area, float* o_star);
+/* Multi-grid blending (the G > 1 frame, MakeImageKernel.cu:386-510):
+ * border = (2 + 2*1000) float2 from grids A and B (GN x GM, GN the rows of
+ * the whole-sky grid); the frame between them at alpha, cam14 = the two
+ * poses' parameter arrays.  o_thphi (h+1)(w+1)*2 and o_border optional. */
+void   btor_blend_border(const float* grid_a, const float* grid_b, int GN, int GM, float* border);
+int    btor_frame_blend(const float* grid_a, const float* grid_b, int GN, int GM, int gridlvl,
+                        float alpha, const float* view, int w, int h, const float* cam14,
+                        const int32_t* tree, int tree_level, const float* star_pos,
+                        const float* star_mag, int S, const float* sum_table, int sky_rows,
+                        int sky_cols, int star, int map, int lr, float hor, float ver,
+                        float offset, int step, uint8_t* out_bgra, float* o_thphi,
+                        float* o_border);
 /* Helpers exposed for pinning tests. */
 void   btor_rgb_to_hsp(float r, float g, float b, float* hsp);
 void   btor_hsp_to_rgb(float h, float s, float p, float* rgb);

@@ -34,6 +34,16 @@
  * Trails and diffraction are not part of this path (both are identities on
  * a first frame with no diffraction sprite).
  *
+ * Multi-grid blending (the G > 1 frame: camUpdate, findBhCenter,
+ * findBhBorders, smoothBorder and pixInterpolation with Gr > 1,
+ * MakeImageKernel.cu:386-510) is restated in btor_blend_border and
+ * btor_frame_blend, with two more divergences on undefined behaviour: the
+ * border walk stops at the grid's edge (the reference walks on, reading
+ * outside its grid), and the direction slot is clamped to angleNum - 1
+ * (the reference reads one entry past its border array when the angle
+ * rounds to 2 pi).  findBhCenter's extremes start from (100, 0) on every
+ * call; the reference's accumulate over the grid pairs of a session.
+ *
  * Pinning: stage outputs are checked by tests against closed forms
  * (Hermite identity on grid points, spherical-triangle areas summing to
  * 4 pi, summed-table identity, HSP round trip, star-tree leaf membership by
@@ -65,6 +75,7 @@ static const int kTempRGB[1173] = {
 static float f_sin(float x) { return (float)sin((double)x); }
 static float f_cos(float x) { return (float)cos((double)x); }
 static float f_atan(float x) { return (float)atan((double)x); }
+static float f_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
 static float f_pow(float x, float y) { return (float)pow((double)x, (double)y); }
 static float f_exp(float x) { return (float)exp((double)x); }
 static float f_log10(float x) { return (float)log10((double)x); }
@@ -927,15 +938,164 @@ void btor_make_grid(const float* hash_table, const int32_t* offset_table, const 
 
 /* ------------------------------------------------------------ the frame */
 
+/* ------------------------------------------------------------ multi-grid blending */
+
+#define ANGLE_NUM 1000 /* MakeImageKernel.cu:1257 */
+
+/* findBhCenter (MakeImageKernel.cu:391-404), findBhBorders (:406-431) and
+ * smoothBorder x 4 (:444-456, called at :1439-1442) for grids A and B of
+ * GN x GM (GN: rows of the whole-sky grid).  border: (2 + 2*ANGLE_NUM) float2. */
+void btor_blend_border(const float* grid_a, const float* grid_b, int GN, int GM, float* border) {
+    gridv ga = {grid_a, GN, GM}, gb = {grid_b, GN, GM};
+    float b0x = 100.f, b0y = 0.f, b1x = 100.f, b1y = 0.f; /* :1308-1310 */
+    for (int i = 0; i < GN; i++)
+        for (int j = 0; j < GM; j++)
+            if (grid_at(&ga, i, j).x == -1 || grid_at(&gb, i, j).x == -1) {
+                float gridsize = FPI / (1.f * GN);
+                float ti = gridsize * (float)i, tj = gridsize * (float)j;
+                if (ti < b0x) b0x = ti;
+                if (ti > b0y) b0y = ti;
+                if (tj < b1x) b1x = tj;
+                if (tj > b1y) b1y = tj;
+            }
+    float* cur = border;
+    float* tmp = (float*)malloc(sizeof(float) * 2 * (2 + 2 * ANGLE_NUM));
+    cur[0] = b0x;
+    cur[1] = b0y;
+    cur[2] = b1x;
+    cur[3] = b1y;
+    for (int i = 0; i < 2 * ANGLE_NUM; i++) {
+        int ii = i / 2;
+        float angle = FPI2 / (1.f * ANGLE_NUM) * 1.f * ii;
+        float thetaChange = -f_sin(angle);
+        float phiChange = f_cos(angle);
+        float ptx = .5f * b0x + .5f * b0y, pty = .5f * b1x + .5f * b1y;
+        ptx = ptx / FPI2 * GM;
+        pty = pty / FPI2 * GM;
+        const gridv* g = (i % 2) ? &gb : &ga;
+        float gA = -2.f, gB = -2.f;
+        while (!(gA > 0 && gB == -1)) {
+            gB = gA;
+            ptx += thetaChange;
+            pty += phiChange;
+            int gx = cvt_i32(ptx), gy = cvt_i32(pty);
+            if (gx < 0 || gx >= GN || gy < 0 || gy >= GM) break; /* the walk left the grid */
+            gA = grid_at(g, gx, gy).x;
+        }
+        cur[2 * (2 + i)] = (ptx - thetaChange) * FPI2 / (1.f * GM);
+        cur[2 * (2 + i) + 1] = (pty - phiChange) * FPI2 / (1.f * GM);
+    }
+    for (int pass = 0; pass < 4; pass++) {
+        const float* src = (pass % 2 == 0) ? border : tmp;
+        float* dst = (pass % 2 == 0) ? tmp : border;
+        dst[0] = src[0];
+        dst[1] = src[1];
+        dst[2] = src[2];
+        dst[3] = src[3];
+        for (int i = 0; i < 2 * ANGLE_NUM; i++) {
+            int prev = (i - 2 + 2 * ANGLE_NUM) % (2 * ANGLE_NUM);
+            int next = (i + 2) % (2 * ANGLE_NUM);
+            for (int d = 0; d < 2; d++)
+                dst[2 * (i + 2) + d] =
+                    1.f / 3.f * (src[2 * (prev + 2) + d] + src[2 * (i + 2) + d] + src[2 * (next + 2) + d]);
+        }
+    }
+    free(tmp);
+}
+
+/* pixInterpolation, Gr > 1 (MakeImageKernel.cu:466-505), one corner */
+static f2 blend_corner(const gridv* ga, const gridv* gb, int gridlvl, const float* border, float alpha,
+                       float theta, float phi, int* gs) {
+    float cx = .5f * border[0] + .5f * border[1], cy = .5f * border[2] + .5f * border[3];
+    float stretchRad = fmaxf(border[1] - border[0], border[2] - border[3]) * 0.75f;
+    float centerdist = (theta - cx) * (theta - cx) + (phi - cy) * (phi - cy);
+    f2 A, B, r;
+    *gs = 0;
+    if (centerdist < stretchRad * stretchRad) {
+        float angle = f_atan2(cx - theta, phi - cy);
+        angle = fmodf(angle + FPI2, FPI2);
+        int slot = cvt_i32(angle / FPI2 * ANGLE_NUM);
+        if (slot < 0) slot = 0;
+        if (slot > ANGLE_NUM - 1) slot = ANGLE_NUM - 1;
+        float ax = border[2 * (2 * slot + 2)], ay = border[2 * (2 * slot + 2) + 1];
+        float bx = border[2 * (2 * slot + 3)], by = border[2 * (2 * slot + 3) + 1];
+        float nx = (1.f - alpha) * ax + alpha * bx;
+        float ny = (1.f - alpha) * ay + alpha * by;
+        if (centerdist <= (nx - cx) * (nx - cx) + (ny - cy) * (ny - cy)) {
+            r.x = -1.f;
+            r.y = -1.f;
+            return r;
+        }
+        float tStoB = (cx - stretchRad * f_sin(angle) - nx);
+        float pStoB = (cy + stretchRad * f_cos(angle) - ny);
+        float thetaPerc = fabsf(tStoB) < 1E-5 ? 0 : 1.f - (theta - nx) / tStoB;
+        float phiPerc = fabsf(pStoB) < 1E-5 ? 0 : 1.f - (phi - ny) / pStoB;
+        float thetaA = theta - thetaPerc * (nx - ax);
+        float phiA = phi - phiPerc * (ny - ay);
+        float thetaB = theta - thetaPerc * (nx - bx);
+        float phiB = phi - phiPerc * (ny - by);
+        A = interpolate_pix(ga, thetaA, phiA, gridlvl, gs);
+        B = interpolate_pix(gb, thetaB, phiB, gridlvl, gs);
+    } else {
+        A = interpolate_pix(ga, theta, phi, gridlvl, gs);
+        B = interpolate_pix(gb, theta, phi, gridlvl, gs);
+    }
+    if (A.x == -1 || B.x == -1) {
+        r.x = -1.f;
+        r.y = -1.f;
+        return r;
+    }
+    if (A.y < .2f * FPI2 && B.y > .8f * FPI2) A.y += FPI2;
+    if (B.y < .2f * FPI2 && A.y > .8f * FPI2) B.y += FPI2;
+    r.x = (1.f - alpha) * A.x + alpha * B.x;
+    r.y = fmodf((1.f - alpha) * A.y + alpha * B.y, FPI2);
+    return r;
+}
+
+static int frame_impl(const float* grid, const float* grid_b, float alpha, const float* border, int GN, int GM,
+                      int gridlvl, const float* view, int W, int H, const float* cam, const int32_t* tree,
+                      int tree_level, const float* star_pos, const float* star_mag, int S,
+                      const float* sum_table, int sky_rows, int sky_cols, int star, int map, int lr, float hor,
+                      float ver, float offset, int step, uint8_t* out_bgra, float* o_thphi, int32_t* o_gap,
+                      uint8_t* o_bh, float* o_area, float* o_star);
+
 int btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* view, int W, int H,
                const float* cam, const int32_t* tree, int tree_level, const float* star_pos,
                const float* star_mag, int S, const float* sum_table, int sky_rows, int sky_cols,
                int star, int map, int lr, float hor, float ver, float offset, int step,
                uint8_t* out_bgra, float* o_thphi, int32_t* o_gap, uint8_t* o_bh, float* o_area,
                float* o_star) {
+    return frame_impl(grid, NULL, 0.f, NULL, GN, GM, gridlvl, view, W, H, cam, tree, tree_level, star_pos,
+                      star_mag, S, sum_table, sky_rows, sky_cols, star, map, lr, hor, ver, offset, step, out_bgra,
+                      o_thphi, o_gap, o_bh, o_area, o_star);
+}
+
+int btor_frame_blend(const float* grid_a, const float* grid_b, int GN, int GM, int gridlvl, float alpha,
+                     const float* view, int W, int H, const float* cam14, const int32_t* tree, int tree_level,
+                     const float* star_pos, const float* star_mag, int S, const float* sum_table, int sky_rows,
+                     int sky_cols, int star, int map, int lr, float hor, float ver, float offset, int step,
+                     uint8_t* out_bgra, float* o_thphi, float* o_border) {
+    float* border = o_border ? o_border : (float*)malloc(sizeof(float) * 2 * (2 + 2 * ANGLE_NUM));
+    btor_blend_border(grid_a, grid_b, GN, GM, border);
+    float cam[7];
+    for (int q = 0; q < 7; q++) cam[q] = (1.f - alpha) * cam14[q] + alpha * cam14[7 + q]; /* camUpdate */
+    int rc = frame_impl(grid_a, grid_b, alpha, border, GN, GM, gridlvl, view, W, H, cam, tree, tree_level,
+                        star_pos, star_mag, S, sum_table, sky_rows, sky_cols, star, map, lr, hor, ver, offset,
+                        step, out_bgra, o_thphi, NULL, NULL, NULL, NULL);
+    if (!o_border) free(border);
+    return rc;
+}
+
+static int frame_impl(const float* grid, const float* grid_b, float alpha, const float* border, int GN, int GM,
+                      int gridlvl, const float* view, int W, int H, const float* cam, const int32_t* tree,
+                      int tree_level, const float* star_pos, const float* star_mag, int S,
+                      const float* sum_table, int sky_rows, int sky_cols, int star, int map, int lr, float hor,
+                      float ver, float offset, int step, uint8_t* out_bgra, float* o_thphi, int32_t* o_gap,
+                      uint8_t* o_bh, float* o_area, float* o_star) {
     const int M = W, N = H, M1 = W + 1, N1 = H + 1;
     const size_t NP = (size_t)N * M, NC = (size_t)N1 * M1;
     gridv g = {grid, GN, GM};
+    gridv g2 = {grid_b, GN, GM};
     float* thphi = o_thphi ? o_thphi : (float*)malloc(sizeof(float) * 2 * NC);
     int32_t* gap = o_gap ? o_gap : (int32_t*)malloc(sizeof(int32_t) * NC);
     uint8_t* bh = o_bh ? o_bh : (uint8_t*)malloc(NP);
@@ -943,7 +1103,7 @@ int btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* view
     float* areaSmooth = (float*)malloc(sizeof(float) * NP);
     (void)S;
 
-    /* pixInterpolation (MakeImageKernel.cu:458-515), G == 1 */
+    /* pixInterpolation (MakeImageKernel.cu:458-515), G == 1 or (grid_b) Gr > 1 */
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 8)
 #endif
@@ -953,7 +1113,8 @@ int btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* view
             float theta = view[2 * c] + ver;
             float phi = fmodf(view[2 * c + 1] + hor + FPI2, FPI2);
             int gs = 0;
-            f2 r = interpolate_pix(&g, theta, phi, gridlvl, &gs);
+            f2 r = grid_b ? blend_corner(&g, &g2, gridlvl, border, alpha, theta, phi, &gs)
+                          : interpolate_pix(&g, theta, phi, gridlvl, &gs);
             thphi[2 * c] = r.x;
             thphi[2 * c + 1] = r.y;
             gap[c] = gs;

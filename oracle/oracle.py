@@ -92,6 +92,11 @@ def lib():
                                  C.c_int, FP, FP, C.c_int, FP, C.c_int, C.c_int, C.c_int, C.c_int,
                                  C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, U8P, FP, IP,
                                  U8P, FP, FP]
+        L.btor_blend_border.argtypes = [FP, FP, C.c_int, C.c_int, FP]
+        L.btor_frame_blend.argtypes = [FP, FP, C.c_int, C.c_int, C.c_int, C.c_float, FP, C.c_int,
+                                       C.c_int, FP, IP, C.c_int, FP, FP, C.c_int, FP, C.c_int,
+                                       C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                       C.c_float, C.c_int, U8P, FP, FP]
         L.btor_rgb_to_hsp.argtypes = [C.c_float] * 3 + [FP]
         L.btor_hsp_to_rgb.argtypes = [C.c_float] * 3 + [FP]
         for n in ("btor_calc_area4", "btor_calc_area3"):
@@ -279,4 +284,40 @@ def frame(grid, gridlvl, view, cam7, tree, tree_level, star_pos, star_mag, table
                      _ptr(thphi, FP), _ptr(gap, IP), _ptr(bh, U8P), _ptr(area, FP), _ptr(starl, FP))
     if stages:
         return out, dict(thphi=thphi, gap=gap, bh=bh, area=area, star=starl)
+    return out
+
+
+def blend_border(grid_a, grid_b):
+    """findBhCenter + findBhBorders + smoothBorder x 4 of two dense grids:
+    (2 + 2000, 2) float32 (MakeImageKernel.cu:391-456)."""
+    ga, gb = f32(grid_a), f32(grid_b)
+    assert ga.shape == gb.shape
+    GN, GM = ga.shape[:2]
+    border = np.zeros((2 + 2000, 2), dtype=np.float32)
+    lib().btor_blend_border(_ptr(ga, FP), _ptr(gb, FP), GN, GM, _ptr(border, FP))
+    return border
+
+
+def frame_blend(grid_a, grid_b, alpha, gridlvl, view, cam14, tree, tree_level, star_pos, star_mag,
+                table, star=True, map=True, lr=True, hor=0.0, ver=0.0, offset=0.0, step=1,
+                stages=False):
+    """One G > 1 frame between grids A and B at alpha (MakeImageKernel.cu:386-510)."""
+    ga, gb = f32(grid_a), f32(grid_b)
+    assert ga.shape == gb.shape
+    view = f32(view)
+    h, w = view.shape[0] - 1, view.shape[1] - 1
+    GN, GM = ga.shape[:2]
+    out = np.zeros((h, w, 4), dtype=np.uint8)
+    thphi = np.zeros((h + 1, w + 1, 2), dtype=np.float32)
+    border = np.zeros((2 + 2000, 2), dtype=np.float32)
+    pos, mag = f32(star_pos), f32(star_mag)
+    table = f32(table)
+    lib().btor_frame_blend(_ptr(ga, FP), _ptr(gb, FP), GN, GM, gridlvl, float(alpha), _ptr(view, FP),
+                           w, h, _ptr(f32(cam14), FP),
+                           _ptr(np.ascontiguousarray(tree, np.int32), IP), tree_level, _ptr(pos, FP),
+                           _ptr(mag, FP), pos.shape[0], _ptr(table, FP), table.shape[0],
+                           table.shape[1], int(star), int(map), int(lr), hor, ver, offset, step,
+                           _ptr(out, U8P), _ptr(thphi, FP), _ptr(border, FP))
+    if stages:
+        return out, dict(thphi=thphi, border=border)
     return out
