@@ -169,17 +169,57 @@ def frames(bt, S, ctx, nframes):
     kern = {n: ctx.timer(n) for n in RENDER_KERNELS}
     ctx.timing(False)
     nf = len(ms)
+    extra = display_effects(bt, ctx, g, sc, view, buf, cam, bh, speed)
     kern_ms = sum(v[1] for v in kern.values()) / nf
     b_frame = 8 * g.N * g.M + 16 * stars.shape[0] + 32 * W * H + 4 * W * H
     ach = b_frame / (kern_ms * 1e-3) / 1e9
     out = {"workload": "adaptive Grid(10,1) 1025x2048 (FP64) + 1920x1080 frame, 100k stars, 4096x2048 sky",
            "grid_build_ms": grid_ms, "grid_rays": g.rays_traced,
-           "frame_ms_p50": pct(ms, 0.5), "frame_ms_p99": pct(ms, 0.99), "frames_per_s": 1e3 * nf / sum(ms)}
+           "frame_ms_p50": pct(ms, 0.5), "frame_ms_p99": pct(ms, 0.99), "frames_per_s": 1e3 * nf / sum(ms),
+           "display_effects": extra}
     roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
             "bytes_per_frame": b_frame, "kernel_ms_per_frame": kern_ms,
             "per_kernel_ms": {n: v[1] / max(1, v[0]) for n, v in kern.items()}}
     g.close()
     return out, roof
+
+
+def display_effects(bt, ctx, g, sc, view, buf, cam, bh, speed, nframes=20):
+    """Frame times of the display loop's extras at the frame config: star
+    trails, trails with a diffraction sprite, and a frame blended between
+    this grid and a second pose's (r + 0.5M)."""
+    def timed(render):
+        ms = []
+        for q in range(nframes + 3):
+            s0 = time.perf_counter()
+            render(q)
+            ctx.sync()
+            if q >= 3:
+                ms.append(1e3 * (time.perf_counter() - s0))
+        return {"frame_ms_p50": pct(ms, 0.5), "frame_ms_p99": pct(ms, 0.99)}
+
+    def off(q):
+        return bt.RenderFlags(offset=2 * math.pi * q / (0.25 * speed * W))
+
+    out = {}
+    T = bt.Trails(sc, W, H, ctx=ctx)
+    out["trails"] = timed(lambda q: T.render(g, sc, view, flags=off(q), out_buffer=buf))
+    T.close()
+    fs = W // 16
+    y, x = np.mgrid[0:fs, 0:fs] - fs // 2
+    sprite = np.zeros((fs, fs, 3), np.uint8)
+    sprite[(x == 0) | (y == 0)] = (255, 220, 180)
+    T = bt.Trails(sc, W, H, diffraction=sprite, ctx=ctx)
+    out["trails_diffraction"] = timed(lambda q: T.render(g, sc, view, flags=off(q), out_buffer=buf))
+    T.close()
+    cam2 = bt.Camera(POSE["theta"], POSE["phi"], POSE["r"] + 0.5, black_hole=bh)
+    g2 = bt.Grid(10, 1, True, cam2, bh, ctx=ctx)
+    out["blend"] = timed(lambda q: bt.render_frame_blend(g, g2, (q % 20) / 20.0, sc, view, flags=off(q),
+                                                         out_buffer=buf))
+    g2.close()
+    out["note"] = ("1920x1080 frames as above: trails (per-star image caches over consecutive frames), "
+                   "trails + a 120x120 diffraction sprite, and two-grid blending (r=5M and 5.5M)")
+    return out
 
 
 def batch(bt, ctx):

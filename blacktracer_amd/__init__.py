@@ -479,16 +479,22 @@ def render_frame(grid, scene, view, flags=None, cam7=None, step=1, stages=False,
 
 
 def render_frame_blend(grid0, grid1, alpha, scene, view, flags=None, cam14=None, step=1,
-                       stages=False, ctx=None):
+                       stages=False, ctx=None, out_buffer=None):
     """A frame between the poses of two grids at fraction alpha (BGRA uint8,
     (h, w, 4)): the reference's G > 1 frame (MakeImageKernel.cu:386-510).
     With stages, also the pixel-corner celestial angles and the shadow
-    border (2 + 2000 float2)."""
+    border (2 + 2000 float2).  With out_buffer (a device buffer) the frame
+    goes there, asynchronously."""
     ctx = ctx or grid0.ctx
     W, H = view.width, view.height
-    out = np.zeros((H, W, 4), dtype=np.uint8)
     flags = flags or RenderFlags()
     c14 = None if cam14 is None else _f32(cam14).ctypes.data_as(_lib.FP)
+    if out_buffer is not None and not stages:
+        check(_lib.load().bt_render_frame_blend(
+            ctx.handle, grid0.handle, grid1.handle, float(alpha), scene.handle, view.handle, c14,
+            C.byref(flags), step, None, out_buffer.ptr, None, None))
+        return None
+    out = np.zeros((H, W, 4), dtype=np.uint8)
     thphi = np.zeros((H + 1, W + 1, 2), dtype=np.float32) if stages else None
     border = np.zeros((2 + 2000, 2), dtype=np.float32) if stages else None
     check(_lib.load().bt_render_frame_blend(
@@ -499,6 +505,68 @@ def render_frame_blend(grid0, grid1, alpha, scene, view, flags=None, cam14=None,
     if stages:
         return out, dict(thphi=thphi, border=border)
     return out
+
+
+class Trails:
+    """State of a frame sequence with star trails (and diffraction when a
+    sprite of (width // 16, width // 16, 3) uint8 is given): each
+    render() is the next frame (MakeImageKernel.cu:98-174,775-800)."""
+
+    def __init__(self, scene, width, height, diffraction=None, ctx=None):
+        self.ctx = ctx or scene.ctx
+        self.width, self.height = width, height
+        self._sprite = None
+        sp = None
+        if diffraction is not None:
+            fs = width // 16
+            self._sprite = np.ascontiguousarray(diffraction, dtype=np.uint8).reshape(fs, fs, 3)
+            sp = self._sprite.ctypes.data_as(_lib.U8P)
+        h = C.c_void_p()
+        check(_lib.load().bt_trails_create(self.ctx.handle, scene.handle, width, height, sp, C.byref(h)))
+        self.handle = h
+
+    def render(self, grid, scene, view, flags=None, cam7=None, step=1, stages=False, out_buffer=None):
+        """The next frame (BGRA uint8, (h, w, 4)); with stages also the summed
+        star light and this frame's trails ((h, w, 3) float32 each).  With
+        out_buffer (a device buffer) the frame goes there, asynchronously."""
+        W, H = view.width, view.height
+        flags = flags or RenderFlags()
+        c7 = None if cam7 is None else _f32(cam7).ctypes.data_as(_lib.FP)
+        if out_buffer is not None and not stages:
+            check(_lib.load().bt_render_frame_trails(
+                self.ctx.handle, grid.handle, scene.handle, view.handle, c7, C.byref(flags), step,
+                self.handle, None, out_buffer.ptr, None, None))
+            return None
+        out = np.zeros((H, W, 4), dtype=np.uint8)
+        star = np.zeros((H, W, 3), dtype=np.float32) if stages else None
+        trail = np.zeros((H, W, 3), dtype=np.float32) if stages else None
+        check(_lib.load().bt_render_frame_trails(
+            self.ctx.handle, grid.handle, scene.handle, view.handle, c7, C.byref(flags), step,
+            self.handle, out.ctypes.data_as(_lib.U8P), None,
+            None if star is None else star.ctypes.data_as(_lib.FP),
+            None if trail is None else trail.ctypes.data_as(_lib.FP)))
+        if stages:
+            return out, dict(star=star, trail=trail)
+        return out
+
+    def reset(self):
+        _lib.load().bt_trails_reset(self.handle)
+
+    def stats(self):
+        f, d = C.c_int64(), C.c_int64()
+        check(_lib.load().bt_trails_stats(self.handle, C.byref(f), C.byref(d)))
+        return dict(frames=f.value, dropped=d.value)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().bt_trails_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def make_image(stars, star_tree_arr, cam_param, magnitude, tree_level, M, N, step, sky_bgr,

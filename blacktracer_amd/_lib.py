@@ -159,6 +159,13 @@ SIGNATURES = {
     "bt_render_frame_stages": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, FP,
                                          C.POINTER(BtRenderFlags), C.c_int, U8P, C.c_void_p, FP,
                                          I32P, U8P, FP, FP]),
+    "bt_trails_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, U8P, C.POINTER(C.c_void_p)]),
+    "bt_trails_reset": (None, [C.c_void_p]),
+    "bt_trails_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "bt_trails_free": (None, [C.c_void_p]),
+    "bt_render_frame_trails": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, FP,
+                                         C.POINTER(BtRenderFlags), C.c_int, C.c_void_p, U8P,
+                                         C.c_void_p, FP, FP]),
     "bt_render_frame_blend": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p,
                                         C.c_void_p, FP, C.POINTER(BtRenderFlags), C.c_int, U8P,
                                         C.c_void_p, FP, FP]),

@@ -29,6 +29,7 @@ struct SceneK {
     // lensing in range, no tap of any star needs the reference's slow
     // expression
     int mag_tame;
+    const int32_t* bin_idx;   // tree-order index of each binned star
 };
 
 constexpr int kBinT = 256;
@@ -37,8 +38,9 @@ constexpr float kBinDT = 3.141592653589793238f / kBinT;
 constexpr float kBinDP = 6.283185307179586476f / kBinP;
 
 // Host: the binned star arrays for SceneK (bin_start kBinT*kBinP+1 ints,
-// pos/mag 2*S floats each), from the tree-ordered arrays.
-void star_bins(const float* pos, const float* mag, int S, int32_t* bin_start, float* bin_pos, float* bin_mag);
+// pos/mag 2*S floats each, bin_idx S ints), from the tree-ordered arrays.
+void star_bins(const float* pos, const float* mag, int S, int32_t* bin_start, float* bin_pos, float* bin_mag,
+               int32_t* bin_idx);
 
 // Pixel-corner camera-sky directions ((H+1) x (W+1) float2).
 struct ViewK {
@@ -70,9 +72,34 @@ struct Blend {
     float* border;
 };
 
+// Star trails of a frame sequence (MakeImageKernel.cu:98-174,618-629):
+// per star, the pixels of its images in the last two frames.
+constexpr int kTrailNum = 30;    // images kept per star and frame (MakeImageKernel.cu:1255)
+constexpr int kTrailHits = 32;   // images recorded per star and frame
+struct TrailK {
+    int S;              // stars
+    int cur;            // cache half this frame writes (the other is the previous frame's)
+    int2* cache;        // [2][S][kTrailNum] (row, column)
+    int* count;         // [2][S]
+    unsigned* nhits;    // [S] images this frame
+    uint32_t* hit_pix;  // [S][kTrailHits] pixel index row * W + column
+    float* hit_data;    // [S][kTrailHits][3] part, frac, blackbody index
+    float* grad;        // (H+1)(W+1) float2
+    float* trail;       // W*H float3
+    unsigned* stats;    // [1]: images beyond kTrailHits this frame (dropped)
+};
+// Per-frame trail and diffraction inputs of render_frame.
+struct TrailFrame {
+    TrailK k;
+    bool first;              // frame 0: no previous images
+    int W, H;                // the frame size the buffers were made for
+    const uint8_t* sprite;   // diffraction sprite, fs x fs RGB8, or null
+    int fs;
+};
+
 int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, const float* cam7,
                  const bt_render_flags& f, int step, uint8_t* d_out, FrameDebug* dbg,
-                 const Blend* blend = nullptr);
+                 const Blend* blend = nullptr, const TrailFrame* tf = nullptr);
 
 // Host-side inputs (Viewer.h, StarProcessor.h).
 int viewer_matrix(double view_angle, double off_up, double off_right, int w, int h, int sphere, float* out);
@@ -92,11 +119,13 @@ struct bt_scene {
     int32_t* bin_start = nullptr;
     float* bin_pos = nullptr;
     float* bin_mag = nullptr;
+    int32_t* bin_idx = nullptr;
     int tree_level = 0, S = 0, rows = 0, cols = 0;
     int mag_tame = 1;
     size_t bytes = 0;
     bt::SceneK k() const {
-        return bt::SceneK{tree, tree_level, pos, mag, S, table, rows, cols, bin_start, bin_pos, bin_mag, mag_tame};
+        return bt::SceneK{tree, tree_level, pos, mag, S, table, rows, cols, bin_start, bin_pos, bin_mag, mag_tame,
+                          bin_idx};
     }
 };
 

@@ -1094,7 +1094,6 @@ __global__ __launch_bounds__(256) void star_map_chunk(const float* thphi, const 
     const unsigned nwaves = (gridDim.x * blockDim.x) >> 6;
     const unsigned nch = min(Q.ctr[1], Q.cap);
     const unsigned CH = kChunk;
-    const size_t NP = (size_t)W * H;
     __shared__ int s_beg[4][64];
     __shared__ unsigned s_pre[4][65];
     // stars found inside, staged until 64 can be splatted by a full wave
@@ -1286,8 +1285,32 @@ __global__ __launch_bounds__(256) void star_chunk_reduce(int W, int H, float* ta
 }
 
 // sumStarLight (MakeImageKernel.cu:753-773, bounds-checked) + makePix (:802-835)
+// makePix (MakeImageKernel.cu:802-835) of one pixel's summed star light.
+__device__ __forceinline__ void make_pix(float br, float bg, float bb, uint8_t* img, size_t ij) {
+    float r = sqrtf(br), g = sqrtf(bg), b = sqrtf(bb);
+    float mxc = fmaxf(fmaxf(r, g), b);
+    if (mxc > 255.f) {
+        float hh, ss, pp;
+        rgb_to_hsp(r / 255.f, g / 255.f, b / 255.f, hh, ss, pp);
+        pp = 1.0f;
+        hsp_to_rgb(hh, ss, pp, r, g, b);
+        r *= 255;
+        g *= 255;
+        b *= 255;
+    }
+    uchar4 o;
+    o.x = (uint8_t)min(255, cvt_i32(b + 0.f));
+    o.y = (uint8_t)min(255, cvt_i32(g + 0.f));
+    o.z = (uint8_t)min(255, cvt_i32(r + 0.f));
+    o.w = 255;
+    reinterpret_cast<uchar4*>(img)[ij] = o;
+}
+
+// sumStarLight (MakeImageKernel.cu:753-773) + makePix; trail (or null): the
+// frame's star trails, added after the taps (factor 1).
 template <int STEP>
-__global__ __launch_bounds__(256) void star_sum(const float* taps, int W, int H, uint8_t* img, float* o_star) {
+__global__ __launch_bounds__(256) void star_sum(const float* taps, int W, int H, uint8_t* img, float* o_star,
+                                                const float* trail) {
     constexpr int FW = 2 * STEP + 1;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y * blockDim.y + threadIdx.y;
@@ -1306,28 +1329,391 @@ __global__ __launch_bounds__(256) void star_sum(const float* taps, int W, int H,
             bb += factor * taps[(size_t)(tap * 3 + 2) * NP + src];
         }
     const size_t ij = (size_t)i * W + j;
+    if (trail) {
+        br += trail[3 * ij];
+        bg += trail[3 * ij + 1];
+        bb += trail[3 * ij + 2];
+    }
     if (o_star) {
         o_star[3 * ij] = br;
         o_star[3 * ij + 1] = bg;
         o_star[3 * ij + 2] = bb;
     }
-    float r = sqrtf(br), g = sqrtf(bg), b = sqrtf(bb);
-    float mxc = fmaxf(fmaxf(r, g), b);
-    if (mxc > 255.f) {
-        float hh, ss, pp;
-        rgb_to_hsp(r / 255.f, g / 255.f, b / 255.f, hh, ss, pp);
-        pp = 1.0f;
-        hsp_to_rgb(hh, ss, pp, r, g, b);
-        r *= 255;
-        g *= 255;
-        b *= 255;
+    make_pix(br, bg, bb, img, ij);
+}
+
+// ---- star trails and diffraction (MakeImageKernel.cu:98-174,517-537,
+// 618-629,775-800): effects of a sequence of frames.
+//
+// Each frame records, per star, the pixels its images land in (pixels with
+// fewer than S/100 candidate stars in their tree leaves), keeping the first
+// kTrailNum; the next frame draws, for each image, a fading line from the
+// previous frame's image of the same star that lies closest to the
+// direction of celestial motion (the gradient field rotated by 90 degrees),
+// within 45 degrees and M/25 pixels, unless the star moved by at most one
+// pixel.  The reference appends images in atomic order and its lines
+// overwrite each other in whatever order threads run; here a star's images
+// are kept in pixel order and overlapping trails keep the per-channel
+// maximum, so a frame does not depend on scheduling.
+
+__global__ __launch_bounds__(256) void trail_clear(TrailK T, int frame0, size_t n3) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = t; q < (size_t)T.S; q += stride) {
+        T.nhits[q] = 0;
+        T.count[(size_t)T.cur * T.S + q] = 0;
+        if (frame0) T.count[(size_t)(1 - T.cur) * T.S + q] = 0;
     }
-    uchar4 o;
-    o.x = (uint8_t)min(255, cvt_i32(b + 0.f));
-    o.y = (uint8_t)min(255, cvt_i32(g + 0.f));
-    o.z = (uint8_t)min(255, cvt_i32(r + 0.f));
-    o.w = 255;
-    reinterpret_cast<uchar4*>(img)[ij] = o;
+    for (size_t p = t; p < n3; p += stride) T.trail[p] = 0.f;
+    if (t == 0) T.stats[1] = 0;
+}
+
+// makeGradField (MakeImageKernel.cu:517-537) per pixel corner.  Corners in
+// the shadow get (0, 0) (the reference leaves them unwritten), and the
+// right-hand neighbour of the last corner of the last row, which the
+// reference reads one element past its array, counts as invalid.
+__global__ __launch_bounds__(256) void trail_grad(const float* thphi, int W, int H, float* grad) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > H || j > W) return;
+    const int M1 = W + 1;
+    const size_t NC = (size_t)M1 * (H + 1);
+    auto th = [&](size_t c) { return c < NC ? thphi[2 * c] : -1.f; };
+    const float up = th((size_t)min(H, i + 1) * M1 + j);
+    const float down = th((size_t)max(0, i - 1) * M1 + j);
+    const float left = th((size_t)i * M1 + max(0, j - 1));
+    const float right = th((size_t)i * M1 + min(M1, j + 1));
+    const float mid = th((size_t)i * M1 + j);
+    float gx = 0.f, gy = 0.f;
+    if (mid > 0) {
+        float xdir = 0;
+        if (up > 0 && down > 0) xdir = .5f * (up - mid) - .5f * (down - mid);
+        float ydir = 0;
+        if (left > 0 && right > 0) ydir = .5f * (right - mid) - .5f * (left - mid);
+        gx = -ydir;
+        gy = xdir;
+    }
+    grad[2 * ((size_t)i * M1 + j)] = gx;
+    grad[2 * ((size_t)i * M1 + j) + 1] = gy;
+}
+
+// The images of this frame: per pixel with starsToCheck < S/100
+// (MakeImageKernel.cu:273-282,301), every star inside its quad, with the
+// quantities addTrails uses (part, frac, blackbody index).
+__global__ __launch_bounds__(256) void trail_hits(const float* thphi, const uint8_t* bh, SceneK s, Cam7 cam,
+                                                  int W, int H, float offset, int lr, const float* area,
+                                                  const float* view, TrailK T) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= H || j >= W) return;
+    StarPix P;
+    if (!star_pixel(thphi, bh, W, i, j, offset, lr, area, view, cam, P)) return;
+    int starsToCheck = 0;
+    search_tree(s.tree, s.tree_level, P.mn, P.mx, [&](int node) {
+        int b, e;
+        node_range(s.tree, node, b, e);
+        starsToCheck += e - b;
+    });
+    if (!(starsToCheck < s.S / 100)) return;
+    const uint32_t pix = (uint32_t)i * (uint32_t)W + (uint32_t)j;
+    // candidates as in the star pass: the binned stars of the quad's rows
+    // (tree order for quads across phi = 2 pi); the same stars test inside
+    const float* pos = P.picheck ? s.star_pos : s.bin_pos;
+    const float* magv = P.picheck ? s.star_mag : s.bin_mag;
+    for_ranges(P, s, [&](int b, int e) {
+        for (int k = b; k < e; k++) {
+            float st, sp;
+            if (!star_test(P, pos, k, st, sp)) continue;
+            const int q = P.picheck ? k : s.bin_idx[k];
+            const float2 mg = reinterpret_cast<const float2*>(magv)[k];
+            const float part = mg.x + P.red;
+            float temp = 46.f / P.redshft * ((1.f / ((0.92f * mg.y) + 1.7f)) + (1.f / ((0.92f * mg.y) + 0.62f))) - 10.f;
+            const int index = max(0, min(cvt_i32(floorf(temp)), 390));
+            const unsigned slot = atomicAdd(&T.nhits[q], 1u);
+            if (slot < (unsigned)kTrailHits) {
+                const size_t h = (size_t)q * kTrailHits + slot;
+                T.hit_pix[h] = pix;
+                T.hit_data[3 * h] = part;
+                T.hit_data[3 * h + 1] = P.frac;
+                T.hit_data[3 * h + 2] = (float)index;
+            } else {
+                atomicAdd(&T.stats[1], 1u);
+            }
+        }
+    });
+}
+
+// Per star: its images in pixel order; the first kTrailNum become this
+// frame's cache (MakeImageKernel.cu:102-104).
+__global__ __launch_bounds__(256) void trail_cache(TrailK T, int W) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= T.S) return;
+    const int n = (int)min(T.nhits[q], (unsigned)kTrailHits);
+    if (n == 0) return;
+    uint32_t* hp = T.hit_pix + (size_t)q * kTrailHits;
+    float* hd = T.hit_data + (size_t)q * kTrailHits * 3;
+    for (int a = 1; a < n; a++) {   // insertion sort by pixel (a star has a few images)
+        const uint32_t kp = hp[a];
+        const float d0 = hd[3 * a], d1 = hd[3 * a + 1], d2 = hd[3 * a + 2];
+        int b = a - 1;
+        while (b >= 0 && hp[b] > kp) {
+            hp[b + 1] = hp[b];
+            hd[3 * (b + 1)] = hd[3 * b];
+            hd[3 * (b + 1) + 1] = hd[3 * b + 1];
+            hd[3 * (b + 1) + 2] = hd[3 * b + 2];
+            b--;
+        }
+        hp[b + 1] = kp;
+        hd[3 * (b + 1)] = d0;
+        hd[3 * (b + 1) + 1] = d1;
+        hd[3 * (b + 1) + 2] = d2;
+    }
+    const int keep = min(n, kTrailNum);
+    int2* c = T.cache + ((size_t)T.cur * T.S + q) * kTrailNum;
+    for (int k = 0; k < keep; k++) c[k] = make_int2((int)(hp[k] / (uint32_t)W), (int)(hp[k] % (uint32_t)W));
+    T.count[(size_t)T.cur * T.S + q] = keep;
+}
+
+__device__ __forceinline__ void trail_put(float* trail, size_t ij, float v, int ch) {
+    if (v >= 0.f) atomicMax(reinterpret_cast<int*>(trail) + 3 * ij + ch, __float_as_int(v));
+}
+
+// addTrails (MakeImageKernel.cu:98-174) for one image of one star.
+__global__ __launch_bounds__(256) void trail_lines(TrailK T, Cam7 cam, const float* grad, int W, int H) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)T.S * kTrailHits) return;
+    const int q = (int)(t / kTrailHits), slot = (int)(t % kTrailHits);
+    if (slot >= (int)min(T.nhits[q], (unsigned)kTrailHits)) return;
+    const size_t h = (size_t)q * kTrailHits + slot;
+    const int M = W, M1 = W + 1;
+    const int i = (int)(T.hit_pix[h] / (uint32_t)W), j = (int)(T.hit_pix[h] % (uint32_t)W);
+    const float part = T.hit_data[3 * h], frac = T.hit_data[3 * h + 1];
+    const float* col = cam.colours + 6 * (int)T.hit_data[3 * h + 2];
+    const int prev_half = 1 - T.cur;
+    const int pc = T.count[(size_t)prev_half * T.S + q];
+    const int2* pcache = T.cache + ((size_t)prev_half * T.S + q) * kTrailNum;
+    float traildist = (float)M;
+    float angle = BT_FPI2;
+    int2 prev = make_int2(0, 0);
+    bool line = false;
+    for (int w = 0; w < pc; w++) {
+        const int2 pr = pcache[w];
+        const int dx = pr.x - i, dy = pr.y - j;
+        const int dxx = dx * dx, dyy = dy * dy;
+        if (dxx <= 1 && dyy <= 1) {
+            line = false;
+            break;
+        }
+        const float grx = grad[2 * ((size_t)pr.x * M1 + pr.y)], gry = grad[2 * ((size_t)pr.x * M1 + pr.y) + 1];
+        const float dist = sqrtf((float)(dxx + dyy));
+        if (dist > M / 25.f) continue;
+        const float div = (dist * sqrtf(grx * grx + gry * gry));
+        const float a1 = f_acos((1.f * dx * grx + 1.f * dy * gry) / div);
+        const float a2 = f_acos((1.f * dx * -grx + 1.f * dy * -gry) / div);
+        const float a = fminf(a1, a2);
+        if (a > angle || a > BT_FPI * .25f) continue;
+        if (a < angle) {
+            angle = a;
+            traildist = dist;
+            prev = pr;
+            line = true;
+        }
+    }
+    if (!line) return;
+    const int deltax = i - prev.x, deltay = j - prev.y;
+    const int sgnDeltaX = deltax < 0 ? -1 : 1, sgnDeltaY = deltay < 0 ? -1 : 1;
+    const float deltaerr = deltay == 0 ? fabsf((float)deltax) : fabsf(deltax / (1.f * deltay));
+    float error = 0.f;
+    int y = prev.y, x = prev.x;
+    const float appMag = part - 2.5f * f_log10(frac);
+    const float bright0 = 100 * f_exp10(-.4f * appMag);
+    // the reference's walk can miss (i, j) (a purely vertical move steps y
+    // off the line) and then never ends; here it stops after as many steps
+    // as the line is long or when it leaves the frame
+    const int maxit = 2 * (abs(deltax) + abs(deltay)) + 2;
+    for (int it = 0; it < maxit && (y != j || x != i); it++) {
+        if (error < 1.f) {
+            y += sgnDeltaY;
+            error += deltaerr;
+        }
+        if (error >= 0.5f) {
+            x += sgnDeltaX;
+            error -= 1.f;
+        }
+        if (x < 0 || x >= H || y < 0 || y >= W) break;
+        const float dist = (float)((x - i) * (x - i) + (y - j) * (y - j));
+        const float fade = (traildist - sqrtf(dist)) * (traildist - sqrtf(dist)) / (traildist * traildist);
+        const float brightness = bright0 * fade;
+        float r, g, b;
+        hsp_to_rgb(col[0], col[1], col[2] * brightness, r, g, b);
+        r *= 255.f;
+        g *= 255.f;
+        b *= 255.f;
+        const size_t ij = (size_t)x * M + y;
+        trail_put(T.trail, ij, r * r, 0);
+        trail_put(T.trail, ij, g * g, 1);
+        trail_put(T.trail, ij, b * b, 2);
+        if (dist <= 1.f) break;
+    }
+}
+
+// addDiffraction (MakeImageKernel.cu:775-800): after sumStarLight the
+// diffraction sprite is scattered around each pixel brighter than 255^2,
+// then makePix runs.  Sources and thresholds are the light before
+// diffraction (the reference reads them while other threads add to them).
+// Here each output pixel gathers the contributions of the bright pixels
+// whose sprite window covers it, in ascending pixel order, so the sums do
+// not depend on scheduling (the reference adds with float atomics).
+
+// Per frame row, the columns of its bright pixels in ascending order and
+// their light over the reference's divisor 10000 * max^0.6 (one wavefront
+// per row).
+__global__ __launch_bounds__(256) void diff_rows(const float* light, int W, int H, uint32_t* cols, float* scaled,
+                                                 int* nrow) {
+    const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (row >= H) return;
+    const unsigned lane = threadIdx.x & 63;
+    int n = 0;
+    for (int j0 = 0; j0 < W; j0 += 64) {
+        const int j = j0 + (int)lane;
+        bool hot = false;
+        float l0 = 0.f, l1 = 0.f, l2 = 0.f, mx = 0.f;
+        if (j < W) {
+            const float* l = light + 3 * ((size_t)row * W + j);
+            l0 = l[0];
+            l1 = l[1];
+            l2 = l[2];
+            mx = fmaxf(fmaxf(l0, l1), l2);
+            hot = mx > 65025.f;
+        }
+        const unsigned long long m = __ballot(hot);
+        if (hot) {
+            const size_t o = (size_t)row * W + n + __popcll(m & ((1ull << lane) - 1));
+            const float div = 10000 * f_pow(mx, 0.6f);
+            cols[o] = (uint32_t)j;
+            scaled[3 * o] = l0 / div;
+            scaled[3 * o + 1] = l1 / div;
+            scaled[3 * o + 2] = l2 / div;
+        }
+        n += __popcll(m);
+    }
+    if (lane == 0) nrow[row] = n;
+}
+
+constexpr int kDiffCand = 512;   // bright pixels staged in LDS per round
+
+// One 16 x 16 tile of output pixels per workgroup: lane 0 lists, in pixel
+// order, the bright pixels whose window can reach the tile; every thread
+// then adds the ones whose window covers its pixel.
+__global__ __launch_bounds__(256) void diff_gather(const float* light, float* acc, const uint32_t* cols,
+                                                   const float* scaled, const int* nrow, const uint8_t* sprite, int fs,
+                                                   int W, int H) {
+    __shared__ int c_r[kDiffCand], c_c[kDiffCand], c_x0[kDiffCand], c_x1[kDiffCand];
+    __shared__ float c_l[kDiffCand][3];
+    __shared__ int s_n, s_row, s_k, s_done;
+    const int oj0 = blockIdx.x * 16, oi0 = blockIdx.y * 16;
+    const int oj = oj0 + (int)threadIdx.x, oi = oi0 + (int)threadIdx.y;
+    const int tid = threadIdx.y * 16 + threadIdx.x;
+    const int fh = fs / 2;
+    const bool mine = oi < H && oj < W;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    if (mine) {
+        const float* l = light + 3 * ((size_t)oi * W + oj);
+        a0 = l[0];
+        a1 = l[1];
+        a2 = l[2];
+    }
+    // rows whose windows can reach the tile
+    const int r_lo = max(0, oi0 - (fs - 1 - fh)), r_hi = min(H - 1, oi0 + 15 + fh);
+    if (tid == 0) {
+        s_row = r_lo;
+        s_k = 0;
+        s_done = 0;
+    }
+    __syncthreads();
+    while (true) {
+        if (tid < 64) {
+            // wavefront 0 lists the next candidates in pixel order, 64 row
+            // entries at a time (a ballot keeps their order)
+            int n = 0, r = s_row, k = s_k;
+            const unsigned lane = (unsigned)tid;
+            while (r <= r_hi) {
+                const int nr = nrow[r];
+                if (k >= nr) {
+                    r++;
+                    k = 0;
+                    continue;
+                }
+                const int kk = k + (int)lane;
+                bool take = false;
+                int c = 0;
+                if (kk < nr) {
+                    c = (int)cols[(size_t)r * W + kk];
+                    // the window's columns c - fh .. c - fh + fs - 1 (mod W) against the tile's
+                    const int e = ((c - fh - oj0) % W + W) % W;
+                    take = e <= 15 || e >= W - fs + 1;
+                }
+                const unsigned long long m = __ballot(take);
+                if (n + __popcll(m) > kDiffCand) break;   // the rest in the next round
+                if (take) {
+                    const int d = n + __popcll(m & ((1ull << lane) - 1));
+                    int startx = 0, endx = fs;
+                    if (r < fh) startx = fh - r;
+                    if (r > (H - fh)) endx = H - r + fh;
+                    const float* l = scaled + 3 * ((size_t)r * W + kk);
+                    c_r[d] = r;
+                    c_c[d] = c;
+                    c_x0[d] = startx;
+                    c_x1[d] = endx;
+                    c_l[d][0] = l[0];
+                    c_l[d][1] = l[1];
+                    c_l[d][2] = l[2];
+                }
+                n += __popcll(m);
+                k += 64;
+            }
+            if (lane == 0) {
+                s_n = n;
+                s_row = r;
+                s_k = k;
+                s_done = r > r_hi;
+            }
+        }
+        __syncthreads();
+        const int n = s_n;
+        if (mine) {
+            for (int k = 0; k < n; k++) {
+                const int qq = oi - (c_r[k] - fh);
+                if (qq < c_x0[k] || qq >= c_x1[k]) continue;
+                const int pp = ((oj - c_c[k] + fh) % W + W) % W;
+                if (pp >= fs) continue;
+                const uint8_t* sp = sprite + 3 * ((size_t)qq * fs + pp);
+                const float dx = c_l[k][0] * (float)sp[0], dy = c_l[k][1] * (float)sp[1], dz = c_l[k][2] * (float)sp[2];
+                a0 += fminf(65025.f, dx * dx);
+                a1 += fminf(65025.f, dy * dy);
+                a2 += fminf(65025.f, dz * dz);
+            }
+        }
+        const int done = s_done;
+        __syncthreads();
+        if (done) break;
+    }
+    if (mine) {
+        float* o = acc + 3 * ((size_t)oi * W + oj);
+        o[0] = a0;
+        o[1] = a1;
+        o[2] = a2;
+    }
+}
+
+__global__ __launch_bounds__(256) void copy_f32(const float* src, float* dst, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t];
+}
+
+__global__ __launch_bounds__(256) void make_pix_all(const float* light, int64_t NP, uint8_t* img) {
+    const int64_t ij = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ij < NP) make_pix(light[3 * ij], light[3 * ij + 1], light[3 * ij + 2], img, (size_t)ij);
 }
 
 // distortEnvironmentMap (MakeImageKernel.cu:631-751)
@@ -1723,7 +2109,8 @@ size_t frame_bytes(int w, int h, int step) {
 }
 
 int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, const float* cam7,
-                 const bt_render_flags& f, int step, uint8_t* d_out, FrameDebug* dbg, const Blend* blend) {
+                 const bt_render_flags& f, int step, uint8_t* d_out, FrameDebug* dbg, const Blend* blend,
+                 const TrailFrame* tf) {
     const int W = v.W, H = v.H;
     const size_t NP = (size_t)W * H, NC = (size_t)(W + 1) * (H + 1);
     if (step != 1 && step != 2 && step != 3)
@@ -1734,6 +2121,9 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         return fail(BT_ERR_INVALID, "render: offset must be finite and >= 0; hor/ver finite");
     if (W <= 0 || H <= 0 || g.GN < 2 || g.GM < 2 || s.rows <= 0 || s.cols <= 0)
         return fail(BT_ERR_INVALID, "render: empty frame, grid or sky");
+    if (tf && (tf->W != W || tf->H != H))
+        return fail(BT_ERR_INVALID, "render: the trail state was made for another frame size");
+    if (tf && tf->k.S != s.S) return fail(BT_ERR_INVALID, "render: the trail state was made for another star set");
     if (blend && (blend->b.GN != g.GN || blend->b.GM != g.GM || blend->b.gridlvl != g.gridlvl || !blend->border ||
                   !(blend->alpha >= 0.f && blend->alpha <= 1.f)))
         return fail(BT_ERR_INVALID, "render: blended grids must have equal sizes and levels; alpha in [0, 1]");
@@ -1837,6 +2227,16 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         Q.partial = (float*)p;
         zero_u32_r<<<1, 64, 0, c->stream>>>(Q.ctr, 4);
         const unsigned nwave = (unsigned)c->num_cus * 16, nred = (unsigned)c->num_cus * 4;
+        if (tf) {
+            KTimer t(c, "trails");
+            const TrailK& T = tf->k;
+            trail_clear<<<(unsigned)c->num_cus * 4, 256, 0, c->stream>>>(T, tf->first ? 1 : 0, 3 * NP);
+            trail_grad<<<grid2d(W + 1, H + 1), blk, 0, c->stream>>>(thphi, W, H, T.grad);
+            trail_hits<<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view, T);
+            trail_cache<<<(unsigned)((T.S + 255) / 256), 256, 0, c->stream>>>(T, W);
+            trail_lines<<<(unsigned)(((size_t)T.S * kTrailHits + 255) / 256), 256, 0, c->stream>>>(T, cam, T.grad, W, H);
+            BT_HIP(hipGetLastError());
+        }
 #define BT_STAR_LAUNCH(S)                                                                                    \
     {                                                                                                       \
         {                                                                                                   \
@@ -1872,17 +2272,39 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
             BT_STAR_LAUNCH(3)
 #undef BT_STAR_LAUNCH
         BT_HIP(hipGetLastError());
+        const bool diffract = tf && tf->sprite;
+        float* light = nullptr;
+        if (diffract) {
+            if ((rc = c->buffer("r_diff_light", sizeof(float) * 3 * NP * 2, &p))) return rc;
+            light = (float*)p;
+        }
         {
             KTimer t(c, "star_sum");
-            float* ostar = dbg ? dbg->star : nullptr;
+            float* ostar = diffract ? light : (dbg ? dbg->star : nullptr);
+            const float* trail = tf ? tf->k.trail : nullptr;
             if (step == 1)
-                star_sum<1><<<grid2d(W, H), blk, 0, c->stream>>>(taps, W, H, img_star, ostar);
+                star_sum<1><<<grid2d(W, H), blk, 0, c->stream>>>(taps, W, H, img_star, ostar, trail);
             else if (step == 2)
-                star_sum<2><<<grid2d(W, H), blk, 0, c->stream>>>(taps, W, H, img_star, ostar);
+                star_sum<2><<<grid2d(W, H), blk, 0, c->stream>>>(taps, W, H, img_star, ostar, trail);
             else
-                star_sum<3><<<grid2d(W, H), blk, 0, c->stream>>>(taps, W, H, img_star, ostar);
+                star_sum<3><<<grid2d(W, H), blk, 0, c->stream>>>(taps, W, H, img_star, ostar, trail);
         }
         BT_HIP(hipGetLastError());
+        if (diffract) {
+            KTimer t(c, "diffraction");
+            float* acc = light + 3 * NP;
+            if ((rc = c->buffer("r_diff_rows", sizeof(uint32_t) * 4 * NP + sizeof(int) * (size_t)H, &p))) return rc;
+            uint32_t* cols = (uint32_t*)p;
+            float* scaled = reinterpret_cast<float*>(cols + NP);
+            int* nrow = reinterpret_cast<int*>(cols + 4 * NP);
+            const unsigned b3 = (unsigned)((3 * NP + 255) / 256), b1 = (unsigned)((NP + 255) / 256);
+            diff_rows<<<(unsigned)((H + 3) / 4), 256, 0, c->stream>>>(light, W, H, cols, scaled, nrow);
+            diff_gather<<<grid2d(W, H), blk, 0, c->stream>>>(light, acc, cols, scaled, nrow, tf->sprite, tf->fs, W,
+                                                             H);
+            make_pix_all<<<b1, 256, 0, c->stream>>>(acc, (int64_t)NP, img_star);
+            if (dbg && dbg->star) copy_f32<<<b3, 256, 0, c->stream>>>(acc, dbg->star, (int64_t)(3 * NP));
+            BT_HIP(hipGetLastError());
+        }
     }
     if (f.map) {
         if ((rc = c->buffer("r_env_q", sizeof(uint32_t) * 2 * (NP + 64), &p))) return rc;

@@ -120,7 +120,8 @@ static int bin_host(float v, float d, int n) {
 // Counting sort of the stars into kBinT x kBinP (theta, phi) cells; stable,
 // so stars keep their tree order inside a cell.  The cell of a value is
 // computed exactly as the device computes the cell range of a bounding box.
-void star_bins(const float* pos, const float* mag, int S, int32_t* bin_start, float* bin_pos, float* bin_mag) {
+void star_bins(const float* pos, const float* mag, int S, int32_t* bin_start, float* bin_pos, float* bin_mag,
+               int32_t* bin_idx) {
     const int NB = kBinT * kBinP;
     std::vector<int32_t> cell(S > 0 ? S : 1);
     for (int c = 0; c <= NB; c++) bin_start[c] = 0;
@@ -137,6 +138,7 @@ void star_bins(const float* pos, const float* mag, int S, int32_t* bin_start, fl
         bin_pos[2 * d + 1] = pos[2 * q + 1];
         bin_mag[2 * d] = mag[2 * q];
         bin_mag[2 * d + 1] = mag[2 * q + 1];
+        bin_idx[d] = q;
     }
 }
 
@@ -163,6 +165,8 @@ static void scene_release(bt_scene* s) {
     if (s->bin_start) hipFree(s->bin_start);
     if (s->bin_pos) hipFree(s->bin_pos);
     if (s->bin_mag) hipFree(s->bin_mag);
+    if (s->bin_idx) hipFree(s->bin_idx);
+    s->bin_idx = nullptr;
     s->tree = nullptr;
     s->bin_start = nullptr;
     s->pos = s->mag = s->table = s->bin_pos = s->bin_mag = nullptr;
@@ -185,13 +189,16 @@ static int scene_upload(Context& c, bt_scene* s, const int32_t* tree, int tree_l
     BT_HIP(hipMalloc((void**)&s->bin_start, sizeof(int32_t) * nb));
     BT_HIP(hipMalloc((void**)&s->bin_pos, sizeof(float) * 2 * (S > 0 ? S : 1)));
     BT_HIP(hipMalloc((void**)&s->bin_mag, sizeof(float) * 2 * (S > 0 ? S : 1)));
-    s->bytes = sizeof(int32_t) * (tsz + nb) + sizeof(float) * 8 * (size_t)S + sizeof(float) * 4 * (size_t)rows * cols;
+    BT_HIP(hipMalloc((void**)&s->bin_idx, sizeof(int32_t) * (S > 0 ? S : 1)));
+    s->bytes = sizeof(int32_t) * (tsz + nb + (size_t)S) + sizeof(float) * 8 * (size_t)S +
+               sizeof(float) * 4 * (size_t)rows * cols;
     s->mag_tame = 1;
     for (int q = 0; q < S; q++)
         if (!(mag[2 * q] >= -10.f && mag[2 * q] <= 60.f)) s->mag_tame = 0;
     std::vector<int32_t> bstart(nb);
     std::vector<float> bpos(2 * (size_t)(S > 0 ? S : 1)), bmag(2 * (size_t)(S > 0 ? S : 1));
-    star_bins(pos, mag, S, bstart.data(), bpos.data(), bmag.data());
+    std::vector<int32_t> bidx((size_t)(S > 0 ? S : 1));
+    star_bins(pos, mag, S, bstart.data(), bpos.data(), bmag.data(), bidx.data());
     BT_HIP(hipMemcpy(s->tree, tree, sizeof(int32_t) * tsz, hipMemcpyHostToDevice));
     BT_HIP(hipMemcpy(s->bin_start, bstart.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
     if (S > 0) {
@@ -199,6 +206,7 @@ static int scene_upload(Context& c, bt_scene* s, const int32_t* tree, int tree_l
         BT_HIP(hipMemcpy(s->mag, mag, sizeof(float) * 2 * S, hipMemcpyHostToDevice));
         BT_HIP(hipMemcpy(s->bin_pos, bpos.data(), sizeof(float) * 2 * S, hipMemcpyHostToDevice));
         BT_HIP(hipMemcpy(s->bin_mag, bmag.data(), sizeof(float) * 2 * S, hipMemcpyHostToDevice));
+        BT_HIP(hipMemcpy(s->bin_idx, bidx.data(), sizeof(int32_t) * S, hipMemcpyHostToDevice));
     }
     void* dsky;
     int rc;
@@ -387,6 +395,17 @@ extern "C" {
 
 }  // extern "C"
 
+// Trail state of a frame sequence: per star, the pixels of its images in the
+// last two frames, and the frame's trail, gradient and image buffers.
+struct bt_trails {
+    bt::Context* ctx = nullptr;
+    int S = 0, W = 0, H = 0, fs = 0;
+    int64_t frame = 0;
+    void* block = nullptr;
+    uint8_t* sprite = nullptr;
+    bt::TrailK k{};
+};
+
 namespace {
 
 // One frame from a device-resident grid, or (g1 non-null) blended between
@@ -394,8 +413,11 @@ namespace {
 // arrays), as the G > 1 frame of the reference.
 int frame_impl(bt_context* h, const bt_grid* g, const bt_grid* g1, float alpha, const bt_scene* s, const bt_view* v,
                const float* cam_in, const bt_render_flags* flags, int step, uint8_t* out_bgra, void* d_out_bgra,
-               float* thphi, int32_t* gap, uint8_t* bh, float* area, float* star, float* border_out) {
+               float* thphi, int32_t* gap, uint8_t* bh, float* area, float* star, float* border_out,
+               bt_trails* tr = nullptr, float* trail_out = nullptr) {
     if (!h || !g || !s || !v) return fail(BT_ERR_INVALID, "bt_render_frame: null handle");
+    if (tr && (tr->ctx != &h->impl || tr->W != v->W || tr->H != v->H || tr->S != s->S))
+        return fail(BT_ERR_INVALID, "bt_render_frame_trails: the trail state was made for another context, size or star set");
     if (!out_bgra && !d_out_bgra) return fail(BT_ERR_INVALID, "bt_render_frame: no output buffer");
     Context& c = h->impl;
     std::lock_guard<std::recursive_mutex> lk(c.mu);
@@ -455,9 +477,25 @@ int frame_impl(bt_context* h, const bt_grid* g, const bt_grid* g1, float alpha, 
         off += sizeof(float) * 3 * NP;
         dbg.bh = (uint8_t*)(base + off);
     }
+    TrailFrame tf{};
+    if (tr) {
+        tf.k = tr->k;
+        tf.k.cur = (int)(tr->frame % 2);
+        tf.first = tr->frame == 0;
+        tf.W = tr->W;
+        tf.H = tr->H;
+        tf.sprite = tr->sprite;
+        tf.fs = tr->fs;
+    }
     if ((rc = render_frame(&c, gf, s->k(), ViewK{v->view, v->W, v->H, v->pix_area}, cam, f, step, (uint8_t*)dout,
-                           want ? &dbg : nullptr, g1 ? &blend : nullptr)))
+                           want ? &dbg : nullptr, g1 ? &blend : nullptr, tr ? &tf : nullptr)))
         return rc;
+    if (tr) {
+        // the reference's frame counter advances with every displayed frame
+        tr->frame++;
+        if (trail_out)
+            BT_HIP(hipMemcpyAsync(trail_out, tr->k.trail, sizeof(float) * 3 * NP, hipMemcpyDeviceToHost, c.stream));
+    }
     if (border_out && g1)
         BT_HIP(hipMemcpyAsync(border_out, blend.border, sizeof(float) * 2 * (2 + 2 * kAngleNum), hipMemcpyDeviceToHost,
                               c.stream));
@@ -467,7 +505,7 @@ int frame_impl(bt_context* h, const bt_grid* g, const bt_grid* g1, float alpha, 
     if (bh) BT_HIP(hipMemcpyAsync(bh, dbg.bh, NP, hipMemcpyDeviceToHost, c.stream));
     if (area) BT_HIP(hipMemcpyAsync(area, dbg.area, sizeof(float) * NP, hipMemcpyDeviceToHost, c.stream));
     if (star) BT_HIP(hipMemcpyAsync(star, dbg.star, sizeof(float) * 3 * NP, hipMemcpyDeviceToHost, c.stream));
-    if (out_bgra || want || border_out) BT_HIP(hipStreamSynchronize(c.stream));
+    if (out_bgra || want || border_out || trail_out) BT_HIP(hipStreamSynchronize(c.stream));
     return BT_OK;
 }
 
@@ -500,6 +538,98 @@ int bt_render_frame(bt_context* h, const bt_grid* g, const bt_scene* s, const bt
                     const bt_render_flags* flags, int step, uint8_t* out_bgra, void* d_out_bgra) {
     return bt_render_frame_stages(h, g, s, v, cam7, flags, step, out_bgra, d_out_bgra, nullptr, nullptr, nullptr,
                                   nullptr, nullptr);
+}
+
+int bt_trails_create(bt_context* h, const bt_scene* s, int width, int height, const uint8_t* diffraction_rgb,
+                     bt_trails** out) {
+    if (!h || !s || !out || width <= 0 || height <= 0) return fail(BT_ERR_INVALID, "bt_trails_create: bad arguments");
+    const int fs = width / 16;   // the reference's sprite size (MakeImageKernel.cu:1284)
+    if (diffraction_rgb && fs < 1) return fail(BT_ERR_INVALID, "bt_trails_create: diffraction needs width >= 16");
+    Context& c = h->impl;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    BT_HIP(hipSetDevice(c.device));
+    const size_t S = (size_t)s->S, NP = (size_t)width * height, NC = (size_t)(width + 1) * (height + 1);
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b_cache = up(sizeof(int2) * 2 * S * kTrailNum), b_count = up(sizeof(int) * 2 * S),
+                 b_nhits = up(sizeof(unsigned) * S), b_pix = up(sizeof(uint32_t) * S * kTrailHits),
+                 b_data = up(sizeof(float) * 3 * S * kTrailHits), b_grad = up(sizeof(float) * 2 * NC),
+                 b_trail = up(sizeof(float) * 3 * NP), b_stats = 256,
+                 b_sprite = diffraction_rgb ? up((size_t)3 * fs * fs) : 0;
+    const size_t total = b_cache + b_count + b_nhits + b_pix + b_data + b_grad + b_trail + b_stats + b_sprite;
+    char* base = nullptr;
+    if (hipMalloc((void**)&base, total) != hipSuccess) return fail(BT_ERR_NOMEM, "bt_trails_create: hipMalloc failed");
+    bt_trails* t = new bt_trails;
+    t->ctx = &c;
+    t->S = (int)S;
+    t->W = width;
+    t->H = height;
+    t->fs = fs;
+    t->block = base;
+    size_t off = 0;
+    t->k.S = (int)S;
+    t->k.cache = (int2*)(base + off);
+    off += b_cache;
+    t->k.count = (int*)(base + off);
+    off += b_count;
+    t->k.nhits = (unsigned*)(base + off);
+    off += b_nhits;
+    t->k.hit_pix = (uint32_t*)(base + off);
+    off += b_pix;
+    t->k.hit_data = (float*)(base + off);
+    off += b_data;
+    t->k.grad = (float*)(base + off);
+    off += b_grad;
+    t->k.trail = (float*)(base + off);
+    off += b_trail;
+    t->k.stats = (unsigned*)(base + off);
+    off += b_stats;
+    if (diffraction_rgb) {
+        t->sprite = (uint8_t*)(base + off);
+        hipError_t e = hipMemcpyAsync(t->sprite, diffraction_rgb, (size_t)3 * fs * fs, hipMemcpyHostToDevice, c.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+        if (e != hipSuccess) {
+            hipFree(base);
+            delete t;
+            return fail(BT_ERR_HIP, std::string("bt_trails_create: ") + hipGetErrorString(e));
+        }
+    }
+    *out = t;
+    return BT_OK;
+}
+
+void bt_trails_reset(bt_trails* t) {
+    if (t) t->frame = 0;
+}
+
+int bt_trails_stats(bt_trails* t, int64_t* frames, int64_t* dropped) {
+    if (!t) return fail(BT_ERR_INVALID, "bt_trails_stats: null handle");
+    std::lock_guard<std::recursive_mutex> lk(t->ctx->mu);
+    BT_HIP(hipSetDevice(t->ctx->device));
+    unsigned st[2] = {0, 0};
+    BT_HIP(hipMemcpyAsync(st, t->k.stats, sizeof(st), hipMemcpyDeviceToHost, t->ctx->stream));
+    BT_HIP(hipStreamSynchronize(t->ctx->stream));
+    if (frames) *frames = t->frame;
+    if (dropped) *dropped = t->frame > 0 ? st[1] : 0;
+    return BT_OK;
+}
+
+void bt_trails_free(bt_trails* t) {
+    if (!t) return;
+    if (t->ctx) {
+        std::lock_guard<std::recursive_mutex> lk(t->ctx->mu);
+        hipSetDevice(t->ctx->device);
+        hipStreamSynchronize(t->ctx->stream);
+        if (t->block) hipFree(t->block);
+    }
+    delete t;
+}
+
+int bt_render_frame_trails(bt_context* h, const bt_grid* g, const bt_scene* s, const bt_view* v, const float* cam7,
+                           const bt_render_flags* flags, int step, bt_trails* trails, uint8_t* out_bgra,
+                           void* d_out_bgra, float* star, float* trail) {
+    if (!trails) return fail(BT_ERR_INVALID, "bt_render_frame_trails: null trail state");
+    return frame_impl(h, g, nullptr, 0.f, s, v, cam7, flags, step, out_bgra, d_out_bgra, nullptr, nullptr, nullptr,
+                      nullptr, star, nullptr, trails, trail);
 }
 
 int bt_make_image(bt_context* h, const float* stars, const int32_t* star_tree_in, int star_size,

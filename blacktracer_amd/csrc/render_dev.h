@@ -35,6 +35,7 @@ __device__ __forceinline__ float f_sin(float x) { return (float)sin((double)x); 
 __device__ __forceinline__ float f_cos(float x) { return (float)cos((double)x); }
 __device__ __forceinline__ float f_atan(float x) { return (float)atan((double)x); }
 __device__ __forceinline__ float f_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
+__device__ __forceinline__ float f_acos(float x) { return (float)acos((double)x); }
 __device__ __forceinline__ float f_pow(float x, float y) { return (float)pow((double)x, (double)y); }
 __device__ __forceinline__ float f_exp(float x) { return (float)exp((double)x); }
 __device__ __forceinline__ float f_log10(float x) { return (float)log10((double)x); }

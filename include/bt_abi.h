@@ -260,6 +260,34 @@ int  bt_render_frame_blend(bt_context* ctx, const bt_grid* g0, const bt_grid* g1
                            const bt_scene* s, const bt_view* v, const float* cam14,
                            const bt_render_flags* flags, int step, uint8_t* out_bgra,
                            void* d_out_bgra, float* thphi, float* border);
+/* Star trails and diffraction of a frame sequence: the effects the
+ * reference's display loop adds to consecutive frames (clearArrays,
+ * makeGradField, addTrails, addDiffraction; MakeImageKernel.cu:98-174,
+ * 517-537,618-629,775-800, called at :1020-1031).  A bt_trails holds, per
+ * star, the pixels of its images in the last two frames; each
+ * bt_render_frame_trails call renders the next frame of the sequence
+ * (frame 0 after create or reset) with the trails from the previous frame
+ * drawn into the star light, and, when a diffraction sprite was given
+ * ((width/16)^2 RGB8 pixels, the reference's sprite size), the sprite
+ * scattered around every pixel brighter than 255^2.  Where the reference
+ * depends on thread scheduling (which 30 images of a star are kept, which
+ * of two crossing trails wins a pixel) this build keeps a star's images in
+ * pixel order and the per-channel maximum of crossing trails.  Optional
+ * outputs: star (w*h*3, summed star light with trails and diffraction),
+ * trail (w*h*3, this frame's trails).  The frame size, scene and context
+ * must be the ones the state was made for. */
+typedef struct bt_trails bt_trails;
+int  bt_trails_create(bt_context* ctx, const bt_scene* s, int width, int height,
+                      const uint8_t* diffraction_rgb, bt_trails** out);
+void bt_trails_reset(bt_trails* t);
+/* frames rendered since create/reset; images of the last frame dropped
+ * because a star had more than 32 in one frame */
+int  bt_trails_stats(bt_trails* t, int64_t* frames, int64_t* dropped);
+void bt_trails_free(bt_trails* t);
+int  bt_render_frame_trails(bt_context* ctx, const bt_grid* g, const bt_scene* s, const bt_view* v,
+                            const float* cam7, const bt_render_flags* flags, int step,
+                            bt_trails* trails, uint8_t* out_bgra, void* d_out_bgra, float* star,
+                            float* trail);
 /* Device bytes a frame of this size works in (intermediates, queues and
  * output; not the grid, scene or view): what the server's admission
  * reserves per frame.  No reference counterpart. */

@@ -157,6 +157,20 @@ int    btor_frame_blend(const float* grid_a, const float* grid_b, int GN, int GM
                         int sky_cols, int star, int map, int lr, float hor, float ver,
                         float offset, int step, uint8_t* out_bgra, float* o_thphi,
                         float* o_border);
+/* A frame sequence with star trails and diffraction (MakeImageKernel.cu:
+ * 98-174,517-537,618-629,775-800): the state holds, per star, the pixels
+ * of its images in the last two frames; each btor_frame_trails call is the
+ * next frame.  sprite: (w/16)^2 RGB8 or NULL.  o_star (w*h*3, light with
+ * trails and diffraction) and o_trail (w*h*3) optional. */
+typedef struct btor_trails btor_trails;
+btor_trails* btor_trails_new(int S, int w, int h, const uint8_t* sprite);
+void   btor_trails_free(btor_trails* t);
+int    btor_frame_trails(btor_trails* t, const float* grid, int GN, int GM, int gridlvl,
+                         const float* view, int w, int h, const float* cam7, const int32_t* tree,
+                         int tree_level, const float* star_pos, const float* star_mag, int S,
+                         const float* sum_table, int sky_rows, int sky_cols, int star, int map,
+                         int lr, float hor, float ver, float offset, int step, uint8_t* out_bgra,
+                         float* o_star, float* o_trail);
 /* Helpers exposed for pinning tests. */
 void   btor_rgb_to_hsp(float r, float g, float b, float* hsp);
 void   btor_hsp_to_rgb(float h, float s, float p, float* rgb);

@@ -44,6 +44,17 @@
  * rounds to 2 pi).  findBhCenter's extremes start from (100, 0) on every
  * call; the reference's accumulate over the grid pairs of a session.
  *
+ * Star trails and diffraction of a frame sequence (clearArrays,
+ * makeGradField, addTrails, addDiffraction; MakeImageKernel.cu:98-174,
+ * 517-537,618-629,775-800) are restated in btor_frame_trails with the
+ * scheduling-independent rules of the device: a star's images are kept in
+ * pixel order (the reference keeps the first 30 in atomic order), crossing
+ * trails keep the per-channel maximum (the reference's last writer wins),
+ * diffraction sources and thresholds are the light before diffraction, the
+ * trail walk is bounded (the reference's never ends for some moves) and
+ * stays in the frame, gradient reads past the corner array count as
+ * invalid, and shadow corners get a zero gradient (unwritten there).
+ *
  * Pinning: stage outputs are checked by tests against closed forms
  * (Hermite identity on grid points, spherical-triangle areas summing to
  * 4 pi, summed-table identity, HSP round trip, star-tree leaf membership by
@@ -76,6 +87,7 @@ static float f_sin(float x) { return (float)sin((double)x); }
 static float f_cos(float x) { return (float)cos((double)x); }
 static float f_atan(float x) { return (float)atan((double)x); }
 static float f_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
+static float f_acos(float x) { return (float)acos((double)x); }
 static float f_pow(float x, float y) { return (float)pow((double)x, (double)y); }
 static float f_exp(float x) { return (float)exp((double)x); }
 static float f_log10(float x) { return (float)log10((double)x); }
@@ -1052,12 +1064,176 @@ static f2 blend_corner(const gridv* ga, const gridv* gb, int gridlvl, const floa
     return r;
 }
 
+/* ------------------------------------------------------------ trails and diffraction */
+
+#define TRAIL_NUM 30  /* MakeImageKernel.cu:1255 */
+#define TRAIL_HITS 32 /* images recorded per star and frame (the device's bound) */
+
+struct btor_trails {
+    int S, W, H, fs;
+    long frame;
+    int* cache; /* [2][S][TRAIL_NUM][2] (row, column) */
+    int* count; /* [2][S] */
+    uint8_t* sprite;
+};
+
+btor_trails* btor_trails_new(int S, int W, int H, const uint8_t* sprite) {
+    btor_trails* t = (btor_trails*)calloc(1, sizeof(btor_trails));
+    t->S = S;
+    t->W = W;
+    t->H = H;
+    t->fs = W / 16;
+    t->cache = (int*)calloc((size_t)2 * (S > 0 ? S : 1) * TRAIL_NUM * 2, sizeof(int));
+    t->count = (int*)calloc((size_t)2 * (S > 0 ? S : 1), sizeof(int));
+    if (sprite) {
+        t->sprite = (uint8_t*)malloc((size_t)3 * t->fs * t->fs);
+        memcpy(t->sprite, sprite, (size_t)3 * t->fs * t->fs);
+    }
+    return t;
+}
+
+void btor_trails_free(btor_trails* t) {
+    if (!t) return;
+    free(t->cache);
+    free(t->count);
+    free(t->sprite);
+    free(t);
+}
+
+typedef struct {
+    int q;
+    uint32_t pix;
+    float part, frac;
+    int index;
+} trail_hit;
+
+static int hit_cmp(const void* a, const void* b) {
+    const trail_hit* x = (const trail_hit*)a;
+    const trail_hit* y = (const trail_hit*)b;
+    if (x->q != y->q) return x->q < y->q ? -1 : 1;
+    return x->pix < y->pix ? -1 : (x->pix > y->pix ? 1 : 0);
+}
+
+static void trail_put(float* trail, size_t ij, float r2, float g2, float b2) {
+    float v[3] = {r2, g2, b2};
+    for (int c = 0; c < 3; c++)
+        if (v[c] >= 0.f && v[c] > trail[3 * ij + c]) trail[3 * ij + c] = v[c];
+}
+
+/* makeGradField (MakeImageKernel.cu:517-537) */
+static void trail_grad(const float* thphi, int W, int H, float* grad) {
+    const int M1 = W + 1;
+    const size_t NC = (size_t)M1 * (H + 1);
+    for (int i = 0; i <= H; i++)
+        for (int j = 0; j <= W; j++) {
+#define TH(c) ((size_t)(c) < NC ? thphi[2 * (size_t)(c)] : -1.f)
+            float up = TH((size_t)(i + 1 < H ? i + 1 : H) * M1 + j);
+            float down = TH((size_t)(i - 1 > 0 ? i - 1 : 0) * M1 + j);
+            float left = TH((size_t)i * M1 + (j - 1 > 0 ? j - 1 : 0));
+            float right = TH((size_t)i * M1 + (j + 1 < M1 ? j + 1 : M1));
+            float mid = TH((size_t)i * M1 + j);
+#undef TH
+            float gx = 0.f, gy = 0.f;
+            if (mid > 0) {
+                float xdir = 0;
+                if (up > 0 && down > 0) xdir = .5f * (up - mid) - .5f * (down - mid);
+                float ydir = 0;
+                if (left > 0 && right > 0) ydir = .5f * (right - mid) - .5f * (left - mid);
+                gx = -ydir;
+                gy = xdir;
+            }
+            grad[2 * ((size_t)i * M1 + j)] = gx;
+            grad[2 * ((size_t)i * M1 + j) + 1] = gy;
+        }
+}
+
+/* addTrails (MakeImageKernel.cu:106-172) for one image */
+static void trail_line(const btor_trails* T, int prev_half, const trail_hit* h, const float* grad, float* trail) {
+    const int W = T->W, H = T->H, M = W, M1 = W + 1;
+    const int i = (int)(h->pix / (uint32_t)W), j = (int)(h->pix % (uint32_t)W);
+    const int pc = T->count[(size_t)prev_half * T->S + h->q];
+    const int* pcache = T->cache + ((size_t)prev_half * T->S + h->q) * TRAIL_NUM * 2;
+    float traildist = (float)M, angle = FPI2;
+    int px = 0, py = 0, line = 0;
+    for (int w = 0; w < pc; w++) {
+        int prx = pcache[2 * w], pry = pcache[2 * w + 1];
+        int dx = prx - i, dy = pry - j;
+        int dxx = dx * dx, dyy = dy * dy;
+        if (dxx <= 1 && dyy <= 1) {
+            line = 0;
+            break;
+        }
+        float grx = grad[2 * ((size_t)prx * M1 + pry)], gry = grad[2 * ((size_t)prx * M1 + pry) + 1];
+        float dist = sqrtf((float)(dxx + dyy));
+        if (dist > M / 25.f) continue;
+        float div = (dist * sqrtf(grx * grx + gry * gry));
+        float a1 = f_acos((1.f * dx * grx + 1.f * dy * gry) / div);
+        float a2 = f_acos((1.f * dx * -grx + 1.f * dy * -gry) / div);
+        float a = fminf(a1, a2);
+        if (a > angle || a > FPI * .25f) continue;
+        if (a < angle) {
+            angle = a;
+            traildist = dist;
+            px = prx;
+            py = pry;
+            line = 1;
+        }
+    }
+    if (!line) return;
+    int deltax = i - px, deltay = j - py;
+    int sgnX = deltax < 0 ? -1 : 1, sgnY = deltay < 0 ? -1 : 1;
+    float deltaerr = deltay == 0 ? fabsf((float)deltax) : fabsf(deltax / (1.f * deltay));
+    float error = 0.f;
+    int y = py, x = px;
+    float appMag = h->part - 2.5f * f_log10(h->frac);
+    float bright0 = 100 * f_exp10(-.4f * appMag);
+    float r0 = (float)kTempRGB[3 * h->index], g0 = (float)kTempRGB[3 * h->index + 1],
+          b0 = (float)kTempRGB[3 * h->index + 2];
+    float hh, ss, pp;
+    rgb_to_hsp(r0 / 255.f, g0 / 255.f, b0 / 255.f, &hh, &ss, &pp);
+    int maxit = 2 * (abs(deltax) + abs(deltay)) + 2;
+    for (int it = 0; it < maxit && (y != j || x != i); it++) {
+        if (error < 1.f) {
+            y += sgnY;
+            error += deltaerr;
+        }
+        if (error >= 0.5f) {
+            x += sgnX;
+            error -= 1.f;
+        }
+        if (x < 0 || x >= H || y < 0 || y >= W) break;
+        float dist = (float)((x - i) * (x - i) + (y - j) * (y - j));
+        float fade = (traildist - sqrtf(dist)) * (traildist - sqrtf(dist)) / (traildist * traildist);
+        float brightness = bright0 * fade;
+        float r, g, b;
+        hsp_to_rgb(hh, ss, pp * brightness, &r, &g, &b);
+        r *= 255.f;
+        g *= 255.f;
+        b *= 255.f;
+        trail_put(trail, (size_t)x * M + y, r * r, g * g, b * b);
+        if (dist <= 1.f) break;
+    }
+}
+
 static int frame_impl(const float* grid, const float* grid_b, float alpha, const float* border, int GN, int GM,
                       int gridlvl, const float* view, int W, int H, const float* cam, const int32_t* tree,
                       int tree_level, const float* star_pos, const float* star_mag, int S,
                       const float* sum_table, int sky_rows, int sky_cols, int star, int map, int lr, float hor,
                       float ver, float offset, int step, uint8_t* out_bgra, float* o_thphi, int32_t* o_gap,
-                      uint8_t* o_bh, float* o_area, float* o_star);
+                      uint8_t* o_bh, float* o_area, float* o_star, btor_trails* tr, float* o_trail);
+
+int btor_frame_trails(btor_trails* t, const float* grid, int GN, int GM, int gridlvl, const float* view,
+                      int W, int H, const float* cam, const int32_t* tree, int tree_level,
+                      const float* star_pos, const float* star_mag, int S, const float* sum_table, int sky_rows,
+                      int sky_cols, int star, int map, int lr, float hor, float ver, float offset, int step,
+                      uint8_t* out_bgra, float* o_star, float* o_trail) {
+    if (!t || t->W != W || t->H != H || t->S != S) return -1;
+    int rc = frame_impl(grid, NULL, 0.f, NULL, GN, GM, gridlvl, view, W, H, cam, tree, tree_level, star_pos,
+                        star_mag, S, sum_table, sky_rows, sky_cols, star, map, lr, hor, ver, offset, step, out_bgra,
+                        NULL, NULL, NULL, NULL, o_star, t, o_trail);
+    t->frame++;
+    return rc;
+}
 
 int btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* view, int W, int H,
                const float* cam, const int32_t* tree, int tree_level, const float* star_pos,
@@ -1067,7 +1243,7 @@ int btor_frame(const float* grid, int GN, int GM, int gridlvl, const float* view
                float* o_star) {
     return frame_impl(grid, NULL, 0.f, NULL, GN, GM, gridlvl, view, W, H, cam, tree, tree_level, star_pos,
                       star_mag, S, sum_table, sky_rows, sky_cols, star, map, lr, hor, ver, offset, step, out_bgra,
-                      o_thphi, o_gap, o_bh, o_area, o_star);
+                      o_thphi, o_gap, o_bh, o_area, o_star, NULL, NULL);
 }
 
 int btor_frame_blend(const float* grid_a, const float* grid_b, int GN, int GM, int gridlvl, float alpha,
@@ -1081,7 +1257,7 @@ int btor_frame_blend(const float* grid_a, const float* grid_b, int GN, int GM, i
     for (int q = 0; q < 7; q++) cam[q] = (1.f - alpha) * cam14[q] + alpha * cam14[7 + q]; /* camUpdate */
     int rc = frame_impl(grid_a, grid_b, alpha, border, GN, GM, gridlvl, view, W, H, cam, tree, tree_level,
                         star_pos, star_mag, S, sum_table, sky_rows, sky_cols, star, map, lr, hor, ver, offset,
-                        step, out_bgra, o_thphi, NULL, NULL, NULL, NULL);
+                        step, out_bgra, o_thphi, NULL, NULL, NULL, NULL, NULL, NULL);
     if (!o_border) free(border);
     return rc;
 }
@@ -1091,7 +1267,7 @@ static int frame_impl(const float* grid, const float* grid_b, float alpha, const
                       int tree_level, const float* star_pos, const float* star_mag, int S,
                       const float* sum_table, int sky_rows, int sky_cols, int star, int map, int lr, float hor,
                       float ver, float offset, int step, uint8_t* out_bgra, float* o_thphi, int32_t* o_gap,
-                      uint8_t* o_bh, float* o_area, float* o_star) {
+                      uint8_t* o_bh, float* o_area, float* o_star, btor_trails* tr, float* o_trail) {
     const int M = W, N = H, M1 = W + 1, N1 = H + 1;
     const size_t NP = (size_t)N * M, NC = (size_t)N1 * M1;
     gridv g = {grid, GN, GM};
@@ -1270,8 +1446,101 @@ static int frame_impl(const float* grid, const float* grid_b, float alpha, const
             }
             free(nodes);
         }
+        float* trail = NULL;
+        if (tr) {
+            /* clearArrays (MakeImageKernel.cu:618-629) */
+            const int cur = (int)(tr->frame % 2);
+            for (int q = 0; q < S; q++) {
+                tr->count[(size_t)cur * S + q] = 0;
+                if (tr->frame == 0) tr->count[(size_t)(1 - cur) * S + q] = 0;
+            }
+            float* grad = (float*)malloc(sizeof(float) * 2 * NC);
+            trail_grad(thphi, W, H, grad);
+            trail = (float*)calloc(NP * 3, sizeof(float));
+            /* the frame's images (MakeImageKernel.cu:273-303) */
+            size_t nh = 0, caph = 1024;
+            trail_hit* hits = (trail_hit*)malloc(sizeof(trail_hit) * caph);
+            int* nodes = (int*)malloc(sizeof(int) * (1 << (tree_level + 1)));
+            for (int i = 0; i < N; i++)
+                for (int j = 0; j < M; j++) {
+                    size_t ij = (size_t)i * M + j;
+                    if (bh[ij] != 0) continue;
+                    float t[4], p[4];
+                    int picheck = 0;
+                    int ind = i * M1 + j;
+                    retrieve_corners(thphi, W, ind, t, p, &picheck, offset);
+                    float mxv[2] = {fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3])),
+                                    fmaxf(fmaxf(p[0], p[1]), fmaxf(p[2], p[3]))};
+                    float mnv[2] = {fminf(fminf(t[0], t[1]), fminf(t[2], t[3])),
+                                    fminf(fminf(p[0], p[1]), fminf(p[2], p[3]))};
+                    int pos = search_tree(tree, mnv, mxv, tree_level, nodes, 1 << (tree_level + 1), 0);
+                    int starsToCheck = 0;
+                    for (int s2 = 0; s2 < pos; s2++) {
+                        int node = nodes[s2];
+                        int startN = 0;
+                        if (node != 0 && ((node + 1) & node) != 0) startN = tree[node - 1];
+                        starsToCheck += tree[node] - startN;
+                    }
+                    if (!(starsToCheck < S / 100)) continue;
+                    float orient = (t[1] - t[0]) * (p[1] + p[0]) + (t[2] - t[1]) * (p[2] + p[1]) +
+                                   (t[3] - t[2]) * (p[3] + p[2]) + (t[0] - t[3]) * (p[0] + p[3]);
+                    int sgn = orient < 0 ? -1 : 1;
+                    float redshft = 1.f, frac = 1.f;
+                    if (lr) lensing_redshift(t, p, W, ind, cam, view, &frac, &redshft, area[ij]);
+                    float red = 4.f * f_log10(redshft);
+                    for (int s2 = 0; s2 < pos; s2++) {
+                        int node = nodes[s2];
+                        int startN = 0;
+                        if (node != 0 && ((node + 1) & node) != 0) startN = tree[node - 1];
+                        for (int q = startN; q < tree[node]; q++) {
+                            float st = star_pos[2 * q], sp = star_pos[2 * q + 1];
+                            int in = star_in_polygon(t, p, st, sp, sgn);
+                            if (picheck && !in && sp < FPI2 * .2f) {
+                                sp += FPI2;
+                                in = star_in_polygon(t, p, st, sp, sgn);
+                            }
+                            if (!in) continue;
+                            float bv = star_mag[2 * q + 1];
+                            float temp_k = 46.f / redshft * ((1.f / ((0.92f * bv) + 1.7f)) +
+                                                             (1.f / ((0.92f * bv) + 0.62f))) - 10.f;
+                            int index = cvt_i32(floorf(temp_k));
+                            if (index > 390) index = 390;
+                            if (index < 0) index = 0;
+                            if (nh == caph) {
+                                caph *= 2;
+                                hits = (trail_hit*)realloc(hits, sizeof(trail_hit) * caph);
+                            }
+                            trail_hit h = {q, (uint32_t)ij, star_mag[2 * q] + red, frac, index};
+                            hits[nh++] = h;
+                        }
+                    }
+                }
+            free(nodes);
+            qsort(hits, nh, sizeof(trail_hit), hit_cmp);
+            /* the first TRAIL_HITS images of each star are recorded; the first
+             * TRAIL_NUM become this frame's cache */
+            size_t a = 0;
+            while (a < nh) {
+                size_t b = a;
+                while (b < nh && hits[b].q == hits[a].q) b++;
+                const int q = hits[a].q;
+                const size_t n = b - a < TRAIL_HITS ? b - a : TRAIL_HITS;
+                const size_t keep = n < TRAIL_NUM ? n : TRAIL_NUM;
+                int* c = tr->cache + ((size_t)cur * S + q) * TRAIL_NUM * 2;
+                for (size_t k = 0; k < keep; k++) {
+                    c[2 * k] = (int)(hits[a + k].pix / (uint32_t)W);
+                    c[2 * k + 1] = (int)(hits[a + k].pix % (uint32_t)W);
+                }
+                tr->count[(size_t)cur * S + q] = (int)keep;
+                for (size_t k = 0; k < n; k++) trail_line(tr, 1 - cur, &hits[a + k], grad, trail);
+                a = b;
+            }
+            free(hits);
+            free(grad);
+        }
         /* sumStarLight (MakeImageKernel.cu:753-773) and makePix (:802-835) */
         img_star = (uint8_t*)malloc(NP * 4);
+        float* light = (tr && tr->sprite) ? (float*)malloc(sizeof(float) * 3 * NP) : NULL;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static)
 #endif
@@ -1289,6 +1558,19 @@ static int frame_impl(const float* grid, const float* grid_b, float alpha, const
                         bg += factor * s[1];
                         bb += factor * s[2];
                     }
+                if (trail) {
+                    size_t ij = (size_t)i * M + j;
+                    br += trail[3 * ij];
+                    bg += trail[3 * ij + 1];
+                    bb += trail[3 * ij + 2];
+                }
+                if (light) {
+                    size_t ij = (size_t)i * M + j;
+                    light[3 * ij] = br;
+                    light[3 * ij + 1] = bg;
+                    light[3 * ij + 2] = bb;
+                    continue;
+                }
                 if (o_star) {
                     size_t ij = (size_t)i * M + j;
                     o_star[3 * ij] = br;
@@ -1314,6 +1596,62 @@ static int frame_impl(const float* grid, const float* grid_b, float alpha, const
                 o[3] = 255;
             }
         free(temp);
+        if (light) {
+            /* addDiffraction (MakeImageKernel.cu:775-800) from the light before it */
+            float* acc = (float*)malloc(sizeof(float) * 3 * NP);
+            memcpy(acc, light, sizeof(float) * 3 * NP);
+            const int fs = tr->fs, fh = fs / 2;
+            for (int i = 0; i < N; i++)
+                for (int j = 0; j < M; j++) {
+                    size_t ij = (size_t)i * M + j;
+                    float lx = light[3 * ij], ly = light[3 * ij + 1], lz = light[3 * ij + 2];
+                    float mx = fmaxf(fmaxf(lx, ly), lz);
+                    if (!(mx > 65025.f)) continue;
+                    int startx = 0, endx = fs;
+                    if (i < fh) startx = fh - i;
+                    if (i > (N - fh)) endx = N - i + fh;
+                    float div = 10000 * f_pow(mx, 0.6f);
+                    for (int qq = startx; qq < endx; qq++) {
+                        int row = i - fh + qq;
+                        if (row < 0 || row >= N) continue;
+                        for (int pp = 0; pp < fs; pp++) {
+                            const uint8_t* sp = tr->sprite + 3 * ((size_t)qq * fs + pp);
+                            float dx = lx / div * (float)sp[0], dy = ly / div * (float)sp[1],
+                                  dz = lz / div * (float)sp[2];
+                            size_t o = (size_t)M * row + (size_t)(((j - fh + pp) % M + M) % M);
+                            acc[3 * o] += fminf(65025.f, dx * dx);
+                            acc[3 * o + 1] += fminf(65025.f, dy * dy);
+                            acc[3 * o + 2] += fminf(65025.f, dz * dz);
+                        }
+                    }
+                }
+            for (size_t ij = 0; ij < NP; ij++) {
+                if (o_star) memcpy(o_star + 3 * ij, acc + 3 * ij, sizeof(float) * 3);
+                float rgb[3] = {sqrtf(acc[3 * ij]), sqrtf(acc[3 * ij + 1]), sqrtf(acc[3 * ij + 2])};
+                float mxc = fmaxf(fmaxf(rgb[0], rgb[1]), rgb[2]);
+                if (mxc > 255.f) {
+                    float hh, ss, pp;
+                    rgb_to_hsp(rgb[0] / 255.f, rgb[1] / 255.f, rgb[2] / 255.f, &hh, &ss, &pp);
+                    pp = 1.0f;
+                    hsp_to_rgb(hh, ss, pp, &rgb[0], &rgb[1], &rgb[2]);
+                    rgb[0] *= 255;
+                    rgb[1] *= 255;
+                    rgb[2] *= 255;
+                }
+                uint8_t* o = img_star + 4 * ij;
+                int cb = cvt_i32(rgb[2] + 0.f), cg = cvt_i32(rgb[1] + 0.f), cr = cvt_i32(rgb[0] + 0.f);
+                o[0] = (uint8_t)(cb < 255 ? cb : 255);
+                o[1] = (uint8_t)(cg < 255 ? cg : 255);
+                o[2] = (uint8_t)(cr < 255 ? cr : 255);
+                o[3] = 255;
+            }
+            free(acc);
+            free(light);
+        }
+        if (trail) {
+            if (o_trail) memcpy(o_trail, trail, sizeof(float) * 3 * NP);
+            free(trail);
+        }
     }
 
     if (map) {

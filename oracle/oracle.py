@@ -97,6 +97,13 @@ def lib():
                                        C.c_int, FP, IP, C.c_int, FP, FP, C.c_int, FP, C.c_int,
                                        C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
                                        C.c_float, C.c_int, U8P, FP, FP]
+        L.btor_trails_new.argtypes = [C.c_int, C.c_int, C.c_int, U8P]
+        L.btor_trails_new.restype = C.c_void_p
+        L.btor_trails_free.argtypes = [C.c_void_p]
+        L.btor_frame_trails.argtypes = [C.c_void_p, FP, C.c_int, C.c_int, C.c_int, FP, C.c_int, C.c_int,
+                                        FP, IP, C.c_int, FP, FP, C.c_int, FP, C.c_int, C.c_int,
+                                        C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                        C.c_int, U8P, FP, FP]
         L.btor_rgb_to_hsp.argtypes = [C.c_float] * 3 + [FP]
         L.btor_hsp_to_rgb.argtypes = [C.c_float] * 3 + [FP]
         for n in ("btor_calc_area4", "btor_calc_area3"):
@@ -321,3 +328,49 @@ def frame_blend(grid_a, grid_b, alpha, gridlvl, view, cam14, tree, tree_level, s
     if stages:
         return out, dict(thphi=thphi, border=border)
     return out
+
+
+class Trails:
+    """Oracle state of a frame sequence with star trails (and diffraction
+    when a (w // 16, w // 16, 3) uint8 sprite is given)."""
+
+    def __init__(self, n_stars, w, h, sprite=None):
+        self.w, self.h, self.S = w, h, n_stars
+        sp = None
+        if sprite is not None:
+            self._sprite = np.ascontiguousarray(sprite, dtype=np.uint8)
+            sp = _ptr(self._sprite, U8P)
+        self.handle = lib().btor_trails_new(n_stars, w, h, sp)
+
+    def frame(self, grid, gridlvl, view, cam7, tree, tree_level, star_pos, star_mag, table,
+              star=True, map=True, lr=True, hor=0.0, ver=0.0, offset=0.0, step=1):
+        """The next frame: (BGRA (h, w, 4), summed star light (h, w, 3), trails (h, w, 3))."""
+        grid = f32(grid)
+        view = f32(view)
+        h, w = view.shape[0] - 1, view.shape[1] - 1
+        assert (w, h) == (self.w, self.h)
+        GN, GM = grid.shape[:2]
+        out = np.zeros((h, w, 4), dtype=np.uint8)
+        light = np.zeros((h, w, 3), dtype=np.float32)
+        trail = np.zeros((h, w, 3), dtype=np.float32)
+        pos, mag = f32(star_pos), f32(star_mag)
+        table = f32(table)
+        rc = lib().btor_frame_trails(self.handle, _ptr(grid, FP), GN, GM, gridlvl, _ptr(view, FP), w, h,
+                                     _ptr(f32(cam7), FP), _ptr(np.ascontiguousarray(tree, np.int32), IP),
+                                     tree_level, _ptr(pos, FP), _ptr(mag, FP), pos.shape[0],
+                                     _ptr(table, FP), table.shape[0], table.shape[1], int(star),
+                                     int(map), int(lr), hor, ver, offset, step, _ptr(out, U8P),
+                                     _ptr(light, FP), _ptr(trail, FP))
+        assert rc == 0
+        return out, light, trail
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().btor_trails_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
