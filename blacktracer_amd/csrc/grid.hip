@@ -18,8 +18,13 @@
 // ascending.  Within one level, fixes on different horizontal lines touch
 // disjoint points, and so do fixes on different vertical lines except that
 // a meridian c and its antipode c + M/2 meet through the pole branches of
-// checkAdjacentBlock (Grid.h:146-153).  So one thread walks each horizontal
-// line, and one thread each meridian pair, in key order -- which reproduces
+// checkAdjacentBlock (Grid.h:146-153).  The north pole branch also ties
+// row 0 to the meridians: a top edge on row 0 reads the antipodal
+// meridian's points in rows 1..gap-1, which that meridian's first segment
+// rewrites.  So per level one thread first walks the row-0 blocks in key
+// order (left meridian segment, top edge, right meridian segment, as each
+// block does), then one thread walks each other horizontal line and one
+// each meridian pair (from row gap down), in key order -- which reproduces
 // the sequential pinned order exactly.
 //
 // Compiled with -ffp-contract=off (refinement compares and Hermite
@@ -400,11 +405,37 @@ __device__ __forceinline__ bool block_fixable(const GridK& G, uint32_t i, uint32
 // level changes), then lane 0 runs checkAdjacentBlock for the hits in
 // ascending key order (ballot bit order), which keeps the sequential
 // order of the fixes.
+// The row-0 blocks of level L in key order, with the three edges each
+// repairs that touch the north pole rows (its bottom edge is on row gap and
+// is left to that row's walker).  One wavefront.
+__device__ __forceinline__ void fix_pole(const GridK& G, int L) {
+    const unsigned lane = threadIdx.x & 63;
+    const uint32_t M = (uint32_t)G.M;
+    const int gap = 1 << (G.max_level - L);
+    const uint32_t ug = (uint32_t)gap;
+    if (ug > (uint32_t)G.N - 1) return;
+    for (uint32_t j0 = 0; j0 < M; j0 += 64 * ug) {
+        const uint32_t j = j0 + lane * ug;
+        unsigned long long m = __ballot(j < M && block_fixable(G, 0, j, L));
+        while (m) {
+            const int src = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const uint32_t jj = j0 + (uint32_t)src * ug, l = (jj + ug) % M;
+            if (lane == 0) {
+                check_adjacent(G, Edge{0, jj, ug, jj, L, gap}, 1);
+                check_adjacent(G, Edge{0, jj, 0, l, L, gap}, 0);
+                check_adjacent(G, Edge{0, l, ug, l, L, gap}, 1);
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ void fix_line(const GridK& G, int L, unsigned nrows, unsigned npairs, unsigned t) {
     const unsigned lane = threadIdx.x & 63;
     const uint32_t M = (uint32_t)G.M, N = (uint32_t)G.N;
     const int gap = 1 << (G.max_level - L);
     const uint32_t ug = (uint32_t)gap, half = ug / 2;
+    if (t == 0) return;   // row 0: fix_pole
     if (t < nrows) {
         const uint32_t y = t * ug;
         for (int pass = 0; pass < 2; pass++) {
@@ -446,7 +477,8 @@ __device__ __forceinline__ void fix_line(const GridK& G, int L, unsigned nrows, 
         const uint32_t col = left ? js : (js + ug) % M;   // the edge's column
         for (uint32_t i0 = 0; i0 + ug <= N - 1; i0 += 16 * ug) {
             const uint32_t i = i0 + (lane >> 2) * ug;
-            bool hit = i + ug <= N - 1 && block_fixable(G, i, js, L) && present_at(G, i + half, col);
+            // the segment from row 0 is fix_pole's
+            bool hit = i > 0 && i + ug <= N - 1 && block_fixable(G, i, js, L) && present_at(G, i + half, col);
             unsigned long long m = __ballot(hit);
             while (m) {
                 const int src = __ffsll((long long)m) - 1;
@@ -463,6 +495,8 @@ __global__ __launch_bounds__(256) void fix_level(GridK G, int L, unsigned nrows,
     fix_line(G, L, nrows, npairs, (blockIdx.x * blockDim.x + threadIdx.x) >> 6);
 }
 
+__global__ __launch_bounds__(64) void fix_level_pole(GridK G, int L) { fix_pole(G, L); }
+
 // The same for several grids in one launch: grid g owns line items
 // [item0[g], item0[g+1]) (rows first, then meridian pairs, as above).
 struct FixJob {
@@ -470,6 +504,12 @@ struct FixJob {
     unsigned nrows, npairs, item0;
     int L;
 };
+
+// fix_pole for job blockIdx.x (one wavefront each)
+__global__ __launch_bounds__(64) void fix_pole_multi(const FixJob* jobs) {
+    const FixJob& J = jobs[blockIdx.x];
+    fix_pole(J.G, J.L);
+}
 
 __global__ __launch_bounds__(256) void fix_level_multi(const FixJob* jobs, int njobs) {
     const unsigned t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -495,6 +535,7 @@ int grid_fix_tvertices(Context* c, const GridK& G) {
         unsigned npairs = (unsigned)((G.M / 2) / gap);
         unsigned n = nrows + npairs;
         KTimer t(c, "fix_tvertices");
+        fix_level_pole<<<1, 64, 0, c->stream>>>(G, L);
         fix_level<<<blocks_for((int64_t)n * 64, 256), 256, 0, c->stream>>>(G, L, nrows, npairs);
         BT_HIP(hipGetLastError());
     }
@@ -533,6 +574,7 @@ int grid_fix_tvertices_multi(Context* c, const std::vector<bt_grid*>& gs) {
         BT_HIP(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(FixJob) * jobs.size(), hipMemcpyHostToDevice, c->stream));
         BT_HIP(hipStreamSynchronize(c->stream));
         KTimer t(c, "fix_tvertices");
+        fix_pole_multi<<<(unsigned)jobs.size(), 64, 0, c->stream>>>(d_jobs);
         fix_level_multi<<<blocks_for((int64_t)items * 64, 256), 256, 0, c->stream>>>(d_jobs, (int)jobs.size());
         BT_HIP(hipGetLastError());
     }

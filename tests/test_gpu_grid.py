@@ -46,15 +46,37 @@ def test_grid_matches_oracle(ctx, pose, lv, start):
     assert err < 1e-9 or np.mean(np.abs(g.cel - og.cel)[present] > 1e-9) < 5e-4
 
 
-def test_fix_tvertices_bit_exact(ctx):
+def _pole_tvertices(levels, present, max_level, start_level):
+    """Meridian T-vertices in the first row of blocks (rows 0..gap): the
+    repairs that interleave with the row-0 top edges in the pinned order."""
+    n = 0
+    M = levels.shape[1]
+    for L in range(start_level, max_level):
+        ug = 1 << (max_level - L)
+        for j in range(0, M, ug):
+            if levels[0, j] == L:
+                n += int(present[ug // 2, j]) + int(present[ug // 2, (j + ug) % M])
+    return n
+
+
+@pytest.mark.parametrize("pose,lv,pole", [
+    ((0.999, math.pi / 2, 0.0, 5.0), 9, False),
+    ((0.999, 0.3, 0.0, 3.0), 8, True),     # close cameras: T-vertices on the
+    ((0.5, math.pi / 2, 0.0, 2.5), 8, True),   # meridians next to the north pole
+    ((0.5, 0.3, 0.0, 3.0), 8, True),
+])
+def test_fix_tvertices_bit_exact(ctx, pose, lv, pole):
     """Device line walkers == oracle's sequential pinned order, on identical
     pre-fix values (the device grid built without the fix)."""
-    a, th, ph, r = 0.999, math.pi / 2, 0.0, 5.0
+    print(f"fix_tvertices pose={pose} Grid({lv},1)", flush=True)
+    a, th, ph, r = pose
     bh = bt.BlackHole(a)
     cam = bt.Camera(th, ph, r, black_hole=bh)
-    g_nofix = bt.Grid(9, 1, True, cam, bh, fix_tvertices=False, ctx=ctx)
-    g_fix = bt.Grid(9, 1, True, cam, bh, fix_tvertices=True, ctx=ctx)
-    og = O.OracleGrid(O.pose(a, th, ph, r), O.params(), 9, 1, True, fix_tvertices=False)
+    g_nofix = bt.Grid(lv, 1, True, cam, bh, fix_tvertices=False, ctx=ctx)
+    g_fix = bt.Grid(lv, 1, True, cam, bh, fix_tvertices=True, ctx=ctx)
+    og = O.OracleGrid(O.pose(a, th, ph, r), O.params(), lv, 1, True, fix_tvertices=False)
+    if pole:
+        assert _pole_tvertices(g_nofix.levels, g_nofix.cel[..., 0] != -2, lv, 1) > 0
     N, M = og.N, og.M
     cel = np.ctypeslib.as_array(og.g.cel, shape=(N, M, 2))
     cel[...] = g_nofix.cel
