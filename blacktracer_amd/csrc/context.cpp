@@ -362,7 +362,7 @@ void bt_integ_params_default(bt_integ_params* p) {
     p->r_capture_factor = 0.0;
     p->newton_cap = 100000;
     p->precision = BT_PREC_FP64;
-    p->crit_eps = 0.01;
+    p->crit_eps = 0.03;
     p->hmin_fp64 = 1e-6;
 }
 

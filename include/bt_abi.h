@@ -94,7 +94,7 @@ typedef struct {
     int    newton_cap;        /* cap on checkB_Q's Newton loop (reference: none) */
     int    precision;         /* BT_PREC_FP64 / BT_PREC_MIXED */
     double crit_eps;          /* mixed: FP64 for rays whose (b, q) lie within crit_eps of the
-                                 critical curve (distance in the (b, q) plane; 0.01) */
+                                 critical curve (distance in the (b, q) plane; 0.03) */
     double hmin_fp64;         /* mixed: switch a ray to FP64 once |h| < hmin_fp64 */
 } bt_integ_params;
 

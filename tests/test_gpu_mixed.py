@@ -78,12 +78,15 @@ def test_mixed_all_critical_is_fp64(ctx):
 
 
 def test_mixed_attempt_cap_restarts_in_fp64(ctx):
-    """Rays that near the attempt cap restart in FP64, so a capped ray
-    returns the FP64 path's sentinel (the camera's own angles)."""
+    """Rays of the FP32 bulk that near the attempt cap restart in FP64, so a
+    capped ray returns the FP64 path's sentinel (the camera's own angles).
+    crit_eps = 0 sends every ray through the FP32 bulk (near-critical rays
+    run the regrouped FP64 form, which has FP64 accuracy but not the exact
+    path's roundings, and are not restarted)."""
     cam = bt.Camera(math.pi / 2, 0.0, 5.0, black_hole=bt.BlackHole(0.999))
     t, p = _lattice(6)
     prm64 = bt.IntegParams(bt.BT_PREC_FP64, max_steps=40)
-    prmm = bt.IntegParams(bt.BT_PREC_MIXED, max_steps=40)
+    prmm = bt.IntegParams(bt.BT_PREC_MIXED, max_steps=40, crit_eps=0.0)
     t64, p64, s64 = bt.trace_rays(cam, t, p, params=prm64, ctx=ctx)
     tm, pm, sm, st = bt.trace_rays(cam, t, p, params=prmm, ctx=ctx, return_stats=True)
     capped = s64 == 39
