@@ -116,7 +116,10 @@ def err_stats(cel_a, cel_b):
 
 
 def cpu_baseline(seconds):
-    """The C oracle tracing every 64th ray of the 2049 x 4096 sky on the host cores."""
+    """The reference's own CPU integrator (Metric.h's checkCelest and
+    rkckIntegrate1, compiled from its sources into oracle/_ref) tracing every
+    64th ray of the 2049 x 4096 sky on the host cores; the C oracle where
+    that build is absent."""
     from oracle import oracle as O
     threads = min(16, os.cpu_count() or 1)
     ps = O.pose(POSE["a"], POSE["theta"], POSE["phi"], POSE["r"])
@@ -125,9 +128,25 @@ def cpu_baseline(seconds):
     i, j = np.meshgrid(np.arange(N), np.arange(M), indexing="ij")
     th = (i / (N - 1) * math.pi).ravel()[::64]
     ph = (j / M * 2 * math.pi).ravel()[::64]
-    rays, t0, rounds = 0, time.perf_counter(), 0
+    rays, rounds = 0, 0
     chunk = 16384
     k = 0
+    if O.load_ref() is not None:
+        # initial conditions (Grid.h:524-547, restated) outside the timed region
+        ic = O.initial_conditions_batch(ps, th, ph)
+        el = 0.0
+        while el < seconds:
+            part = ic[k:k + chunk]
+            t0 = time.perf_counter()
+            O.ref_trace_batch(POSE["a"], POSE["r"], POSE["theta"], POSE["phi"], part, nthreads=threads)
+            el += time.perf_counter() - t0
+            rays += part.shape[0]
+            k = (k + chunk) % th.size
+        return {"value": rays / el, "unit": "rays/s", "cores": threads, "kind": "reference",
+                "sample": f"{rays} rays of every 64th direction of the 2049x4096 sky (reference settings): "
+                          f"the reference's checkCelest + rkckIntegrate1 built from its sources, OpenMP over "
+                          f"rays, {el:.1f} s (initial conditions computed beforehand, untimed)"}
+    t0 = time.perf_counter()
     while True:
         O.trace_rays(ps, O.params(), th[k:k + chunk], ph[k:k + chunk], nthreads=threads)
         rays += min(chunk, th.size - k)

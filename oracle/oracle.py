@@ -117,6 +117,30 @@ def lib():
     return _LIB
 
 
+def initial_conditions_batch(ps, theta, phi):
+    """(pR, pTheta, b, q) of every ray, (n, 4) float64 (Grid.h:524-547)."""
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    ph = np.ascontiguousarray(phi, dtype=np.float64)
+    out = np.zeros((th.size, 4))
+    for k in range(th.size):
+        lib().bto_initial_conditions(C.byref(ps), float(th[k]), float(ph[k]), _ptr(out[k], DP))
+    return out
+
+
+def ref_trace_batch(a, r, theta_cam, phi_cam, ic, nthreads=1):
+    """The reference's capture test + rkckIntegrate1 (its own compiled code)
+    over rays with initial conditions ic (n, 4): returns (theta', phi', step)."""
+    R = load_ref()
+    ic = np.ascontiguousarray(ic, dtype=np.float64)
+    n = ic.shape[0]
+    cols = [np.ascontiguousarray(ic[:, k]) for k in range(4)]
+    th, ph = np.zeros(n), np.zeros(n)
+    st = np.zeros(n, dtype=np.int32)
+    R.btref_trace_batch(a, r, theta_cam, phi_cam, n, _ptr(cols[0], DP), _ptr(cols[1], DP), _ptr(cols[2], DP),
+                        _ptr(cols[3], DP), _ptr(th, DP), _ptr(ph, DP), _ptr(st, IP), int(nthreads))
+    return th, ph, st
+
+
 def load_ref():
     """The reference's own Metric.h/Camera.h/BlackHole.h, or None."""
     global _REF
@@ -139,6 +163,7 @@ def load_ref():
         R.btref_integrate.argtypes = [C.c_double] * 8 + [DP, DP, IP]
         R.btref_wrap_to_pi.argtypes = [DP, DP]
         R.btref_wrap_to_pi_f.argtypes = [FP, FP]
+        R.btref_trace_batch.argtypes = [C.c_double] * 4 + [C.c_int, DP, DP, DP, DP, DP, DP, IP, C.c_int]
         _REF = R
     return _REF
 

@@ -86,6 +86,27 @@ void btref_integrate(double a, double r, double theta, double phi, double pR, do
     *step = st;
 }
 
+// The loop body of integration_wrapper (Grid.h:548-553) after the initial
+// conditions, over a batch of rays: the a-priori capture test and
+// rkckIntegrate1 per ray, spread over OpenMP threads.  The spin is the
+// reference's process-global, set once before the threads start and only
+// read by them.  Captured rays get (-1, -1) and step 0, as there.
+void btref_trace_batch(double a, double r, double theta, double phi, int n, const double* pR,
+                       const double* pTheta, const double* b, const double* q, double* thetaOut,
+                       double* phiOut, int* step, int nthreads) {
+    metric::setAngVel(a);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+    for (int i = 0; i < n; i++) {
+        double th = -1, ph = -1;
+        int st = 0;
+        if (metric::checkCelest(pR[i], r, theta, b[i], q[i]))
+            metric::rkckIntegrate1(r, theta, phi, pR[i], b[i], q[i], pTheta[i], th, ph, st);
+        thetaOut[i] = th;
+        phiOut[i] = ph;
+        step[i] = st;
+    }
+}
+
 void btref_wrap_to_pi(double* theta, double* phi) {
     double t = *theta, p = *phi;
     metric::wrapToPi(t, p);

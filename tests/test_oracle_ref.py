@@ -98,6 +98,18 @@ def test_rays_bit_exact(pose):
     assert n_int > 100
 
 
+def test_reference_batch_equals_oracle():
+    """The reference's batch loop (threads over rays, the CPU baseline of the
+    benchmark) gives the oracle's results bit for bit."""
+    a, th, ph, r = POSES[0]
+    ps = O.pose(a, th, ph, r)
+    t, p = _dirs(800, 11)
+    ot, op, ost, _ = O.trace_rays(ps, O.params(), t, p, nthreads=4)
+    ic = O.initial_conditions_batch(ps, t, p)
+    rt, rp, rs = O.ref_trace_batch(a, r, th, ph, ic, nthreads=4)
+    assert np.array_equal(rt, ot) and np.array_equal(rp, op) and np.array_equal(rs, ost)
+
+
 def test_wrap_to_pi_bit_exact():
     L = O.lib()
     rng = np.random.default_rng(9)
