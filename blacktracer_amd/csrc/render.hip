@@ -2197,6 +2197,15 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         smooth_v<<<grid2d(W, H), blk, 0, c->stream>>>(area_s, bh, gap, W, H, area);
     }
     BT_HIP(hipGetLastError());
+    // The sky pass runs on the auxiliary stream beside the star pass: both
+    // read the per-pixel stages made above and write their own image, and
+    // each has latency-bound kernels (the lensed pixels' queues) that leave
+    // the other room.  Joined before the composite.
+    const bool overlap = f.star && f.map;
+    if (overlap) {
+        BT_HIP(hipEventRecord(c->ev_fork, c->stream));
+        BT_HIP(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+    }
     if (f.star) {
         const int fw = 2 * step + 1;
         if ((rc = c->buffer("r_taps", sizeof(float) * 3 * fw * fw * NP, &p))) return rc;
@@ -2307,26 +2316,39 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         }
     }
     if (f.map) {
-        if ((rc = c->buffer("r_env_q", sizeof(uint32_t) * 2 * (NP + 64), &p))) return rc;
+        hipStream_t es = overlap ? c->aux : c->stream;
+        auto join = [&]() {
+            if (!overlap) return;
+            hipEventRecord(c->ev_join, c->aux);
+            hipStreamWaitEvent(c->stream, c->ev_join, 0);
+        };
+        if ((rc = c->buffer("r_env_q", sizeof(uint32_t) * 2 * (NP + 64), &p))) {
+            join();
+            return rc;
+        }
         uint32_t* qlane = (uint32_t*)p;
         uint32_t* qwave = (uint32_t*)p + NP + 64;
-        if ((rc = c->buffer("r_env_qn", 64, &p))) return rc;
+        if ((rc = c->buffer("r_env_qn", 64, &p))) {
+            join();
+            return rc;
+        }
         unsigned* nq = (unsigned*)p;
-        zero_u32_r<<<1, 64, 0, c->stream>>>(nq, 2);
+        zero_u32_r<<<1, 64, 0, es>>>(nq, 2);
         {
-            KTimer t(c, "env_map");
-            env_map<<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view,
-                                                         img_map, qlane, qwave, nq);
-            env_map_lane<<<(unsigned)c->num_cus * 8, 256, 0, c->stream>>>(thphi, s, cam, W, f.offset, f.lr, area,
-                                                                         v.view, img_map, qlane, nq);
+            KTimer t(c, "env_map", es);
+            env_map<<<grid2d(W, H), blk, 0, es>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view, img_map,
+                                                  qlane, qwave, nq);
+            env_map_lane<<<(unsigned)c->num_cus * 8, 256, 0, es>>>(thphi, s, cam, W, f.offset, f.lr, area, v.view,
+                                                                  img_map, qlane, nq);
         }
-        BT_HIP(hipGetLastError());
         {
-            KTimer t(c, "env_heavy");
-            env_map_heavy<<<(unsigned)c->num_cus * 8, 256, 0, c->stream>>>(thphi, s, cam, W, f.offset, f.lr, area,
-                                                                          v.view, img_map, qwave, nq + 1);
+            KTimer t(c, "env_heavy", es);
+            env_map_heavy<<<(unsigned)c->num_cus * 8, 256, 0, es>>>(thphi, s, cam, W, f.offset, f.lr, area, v.view,
+                                                                   img_map, qwave, nq + 1);
         }
-        BT_HIP(hipGetLastError());
+        const hipError_t e = hipGetLastError();
+        join();
+        if (e != hipSuccess) return fail(BT_ERR_HIP, std::string("render: sky pass launch failed: ") + hipGetErrorString(e));
     }
     {
         KTimer t(c, "composite");
