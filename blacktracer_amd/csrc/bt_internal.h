@@ -152,6 +152,7 @@ struct Context {
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;                 // overlapping work, joined before a call returns
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_hand = nullptr, ev_back = nullptr;   // mid-call hand-offs between the two streams
     std::recursive_mutex mu;
     bool timing = false;
     std::vector<PendingTimer> pending;

@@ -262,7 +262,9 @@ int bt_context_create(int device, bt_context** out) {
     if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c.ev_hand, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c.ev_back, hipEventDisableTiming) != hipSuccess) {
         delete h;
         return fail(BT_ERR_HIP, "hipStreamCreate failed");
     }
@@ -286,6 +288,8 @@ void bt_context_destroy(bt_context* h) {
     for (auto e : c.free_events) hipEventDestroy(e);
     hipEventDestroy(c.ev_fork);
     hipEventDestroy(c.ev_join);
+    hipEventDestroy(c.ev_hand);
+    hipEventDestroy(c.ev_back);
     hipStreamDestroy(c.aux);
     hipStreamDestroy(c.stream);
     delete h;

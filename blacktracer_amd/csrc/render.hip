@@ -2202,6 +2202,34 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
     // each has latency-bound kernels (the lensed pixels' queues) that leave
     // the other room.  Joined before the composite.
     const bool overlap = f.star && f.map;
+    uint32_t *qlane = nullptr, *qwave = nullptr;
+    unsigned* nq = nullptr;
+    if (f.map) {
+        if ((rc = c->buffer("r_env_q", sizeof(uint32_t) * 2 * (NP + 64), &p))) return rc;
+        qlane = (uint32_t*)p;
+        qwave = (uint32_t*)p + NP + 64;
+        if ((rc = c->buffer("r_env_qn", 64, &p))) return rc;
+        nq = (unsigned*)p;
+    }
+    // distortEnvironmentMap: pixels with few sky columns inline and one lane
+    // each; then the wide ones, one wave each
+    auto sky_light = [&](hipStream_t es) {
+        zero_u32_r<<<1, 64, 0, es>>>(nq, 2);
+        KTimer t(c, "env_map", es);
+        env_map<<<grid2d(W, H), blk, 0, es>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view, img_map, qlane,
+                                              qwave, nq);
+        env_map_lane<<<(unsigned)c->num_cus * 8, 256, 0, es>>>(thphi, s, cam, W, f.offset, f.lr, area, v.view,
+                                                              img_map, qlane, nq);
+    };
+    auto sky_heavy = [&](hipStream_t es) {
+        KTimer t(c, "env_heavy", es);
+        env_map_heavy<<<(unsigned)c->num_cus * 8, 256, 0, es>>>(thphi, s, cam, W, f.offset, f.lr, area, v.view,
+                                                               img_map, qwave, nq + 1);
+    };
+    auto join = [&]() {
+        hipEventRecord(c->ev_join, c->aux);
+        hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    };
     if (overlap) {
         BT_HIP(hipEventRecord(c->ev_fork, c->stream));
         BT_HIP(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
@@ -2263,13 +2291,26 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
             KTimer t1(c, "star_tree");                                                                      \
             star_tree_ranges<<<nred, 256, 0, c->stream>>>(thphi, W, f.offset, s, cam, Q);                   \
         }                                                                                                   \
+        if (overlap) {                                                                                      \
+            /* the odd pixels' chunks (few, long, latency-bound) on the auxiliary stream, between */       \
+            /* the sky pass's light and heavy kernels, beside the other chunks */                           \
+            hipEventRecord(c->ev_hand, c->stream);                                                          \
+            sky_light(c->aux);                                                                              \
+            hipStreamWaitEvent(c->aux, c->ev_hand, 0);                                                      \
+            star_map_chunk<S, true><<<nred, 256, 0, c->aux>>>(thphi, bh, s, cam, W, H, f.offset, f.lr,      \
+                                                              area, v.view, taps, Q);                       \
+            hipEventRecord(c->ev_back, c->aux);                                                             \
+            sky_heavy(c->aux);                                                                              \
+        }                                                                                                   \
         {                                                                                                   \
             KTimer t2(c, "star_chunk");                                                                     \
             star_map_chunk<S, false><<<nwave, 256, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr,  \
                                                                    area, v.view, taps, Q);                  \
-            star_map_chunk<S, true><<<nred, 256, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr,    \
-                                                                 area, v.view, taps, Q);                    \
+            if (!overlap)                                                                                   \
+                star_map_chunk<S, true><<<nred, 256, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset,     \
+                                                                     f.lr, area, v.view, taps, Q);          \
         }                                                                                                   \
+        if (overlap) hipStreamWaitEvent(c->stream, c->ev_back, 0);                                          \
         KTimer t3(c, "star_reduce");                                                                        \
         star_chunk_reduce<S><<<nwave, 256, 0, c->stream>>>(W, H, taps, Q);                                  \
     }
@@ -2315,41 +2356,13 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
             BT_HIP(hipGetLastError());
         }
     }
-    if (f.map) {
-        hipStream_t es = overlap ? c->aux : c->stream;
-        auto join = [&]() {
-            if (!overlap) return;
-            hipEventRecord(c->ev_join, c->aux);
-            hipStreamWaitEvent(c->stream, c->ev_join, 0);
-        };
-        if ((rc = c->buffer("r_env_q", sizeof(uint32_t) * 2 * (NP + 64), &p))) {
-            join();
-            return rc;
-        }
-        uint32_t* qlane = (uint32_t*)p;
-        uint32_t* qwave = (uint32_t*)p + NP + 64;
-        if ((rc = c->buffer("r_env_qn", 64, &p))) {
-            join();
-            return rc;
-        }
-        unsigned* nq = (unsigned*)p;
-        zero_u32_r<<<1, 64, 0, es>>>(nq, 2);
-        {
-            KTimer t(c, "env_map", es);
-            env_map<<<grid2d(W, H), blk, 0, es>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view, img_map,
-                                                  qlane, qwave, nq);
-            env_map_lane<<<(unsigned)c->num_cus * 8, 256, 0, es>>>(thphi, s, cam, W, f.offset, f.lr, area, v.view,
-                                                                  img_map, qlane, nq);
-        }
-        {
-            KTimer t(c, "env_heavy", es);
-            env_map_heavy<<<(unsigned)c->num_cus * 8, 256, 0, es>>>(thphi, s, cam, W, f.offset, f.lr, area, v.view,
-                                                                   img_map, qwave, nq + 1);
-        }
-        const hipError_t e = hipGetLastError();
+    if (overlap) {
         join();
-        if (e != hipSuccess) return fail(BT_ERR_HIP, std::string("render: sky pass launch failed: ") + hipGetErrorString(e));
+    } else if (f.map) {
+        sky_light(c->stream);
+        sky_heavy(c->stream);
     }
+    BT_HIP(hipGetLastError());
     {
         KTimer t(c, "composite");
         composite<<<(unsigned)((NP + 255) / 256), 256, 0, c->stream>>>(img_star, img_map, (int64_t)NP, f.star, f.map,
