@@ -493,8 +493,10 @@ int frame_impl(bt_context* h, const bt_grid* g, const bt_grid* g1, float alpha, 
     if (tr) {
         // the reference's frame counter advances with every displayed frame
         tr->frame++;
-        if (trail_out)
+        if (trail_out && f.star)
             BT_HIP(hipMemcpyAsync(trail_out, tr->k.trail, sizeof(float) * 3 * NP, hipMemcpyDeviceToHost, c.stream));
+        else if (trail_out)
+            std::memset(trail_out, 0, sizeof(float) * 3 * NP);   // no star pass: no trails this frame
     }
     if (border_out && g1)
         BT_HIP(hipMemcpyAsync(border_out, blend.border, sizeof(float) * 2 * (2 + 2 * kAngleNum), hipMemcpyDeviceToHost,
