@@ -2265,13 +2265,17 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         zero_u32_r<<<1, 64, 0, c->stream>>>(Q.ctr, 4);
         const unsigned nwave = (unsigned)c->num_cus * 16, nred = (unsigned)c->num_cus * 4;
         if (tf) {
-            KTimer t(c, "trails");
+            // with both passes on, the trails go first on the auxiliary stream:
+            // the main stream waits there (after the odd chunks) before the
+            // chunk reduction, and so before star_sum reads the trails
+            const hipStream_t ts = overlap ? c->aux : c->stream;
+            KTimer t(c, "trails", ts);
             const TrailK& T = tf->k;
-            trail_clear<<<(unsigned)c->num_cus * 4, 256, 0, c->stream>>>(T, tf->first ? 1 : 0, 3 * NP);
-            trail_grad<<<grid2d(W + 1, H + 1), blk, 0, c->stream>>>(thphi, W, H, T.grad);
-            trail_hits<<<grid2d(W, H), blk, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view, T);
-            trail_cache<<<(unsigned)((T.S + 255) / 256), 256, 0, c->stream>>>(T, W);
-            trail_lines<<<(unsigned)(((size_t)T.S * kTrailHits + 255) / 256), 256, 0, c->stream>>>(T, cam, T.grad, W, H);
+            trail_clear<<<(unsigned)c->num_cus * 4, 256, 0, ts>>>(T, tf->first ? 1 : 0, 3 * NP);
+            trail_grad<<<grid2d(W + 1, H + 1), blk, 0, ts>>>(thphi, W, H, T.grad);
+            trail_hits<<<grid2d(W, H), blk, 0, ts>>>(thphi, bh, s, cam, W, H, f.offset, f.lr, area, v.view, T);
+            trail_cache<<<(unsigned)((T.S + 255) / 256), 256, 0, ts>>>(T, W);
+            trail_lines<<<(unsigned)(((size_t)T.S * kTrailHits + 255) / 256), 256, 0, ts>>>(T, cam, T.grad, W, H);
             BT_HIP(hipGetLastError());
         }
 #define BT_STAR_LAUNCH(S)                                                                                    \
