@@ -2235,6 +2235,8 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         BT_HIP(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
     }
     if (f.star) {
+      // the star pass; on an error the streams are joined before returning
+      const int src = [&]() -> int {
         const int fw = 2 * step + 1;
         if ((rc = c->buffer("r_taps", sizeof(float) * 3 * fw * fw * NP, &p))) return rc;
         float* taps = (float*)p;
@@ -2359,6 +2361,12 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
             if (dbg && dbg->star) copy_f32<<<b3, 256, 0, c->stream>>>(acc, dbg->star, (int64_t)(3 * NP));
             BT_HIP(hipGetLastError());
         }
+        return BT_OK;
+      }();
+      if (src) {
+          if (overlap) join();
+          return src;
+      }
     }
     if (overlap) {
         join();
