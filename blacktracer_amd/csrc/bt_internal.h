@@ -175,6 +175,16 @@ struct Context {
 
     // Returns a device buffer of at least `bytes` for `name`.
     int buffer(const std::string& name, size_t bytes, void** out);
+
+    // Pinned host staging for small tables copied to the device on the main
+    // stream: stage() waits until the copies from the previous staging have
+    // run (stage_done() marks them on the stream), so host-to-device copies
+    // of these tables need no synchronisation of their own.
+    void* host_stage = nullptr;
+    size_t host_stage_bytes = 0;
+    hipEvent_t ev_stage = nullptr;
+    int stage(size_t bytes, void** out);
+    int stage_done();
     int timer_begin(const char* name, int* slot, hipStream_t st = nullptr);
     int timer_end(int slot);
     int flush_timers();

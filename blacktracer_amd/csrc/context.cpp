@@ -40,6 +40,29 @@ int Context::buffer(const std::string& name, size_t bytes, void** out) {
     return BT_OK;
 }
 
+int Context::stage(size_t bytes, void** out) {
+    if (ev_stage && hipEventSynchronize(ev_stage) != hipSuccess)
+        return fail(BT_ERR_HIP, "staging: waiting for the previous copies failed");
+    if (host_stage_bytes < bytes) {
+        if (host_stage) hipHostFree(host_stage);
+        host_stage = nullptr;
+        host_stage_bytes = 0;
+        const size_t want = bytes + bytes / 2 + 4096;
+        if (hipHostMalloc(&host_stage, want, 0) != hipSuccess) {
+            host_stage = nullptr;
+            return fail(BT_ERR_NOMEM, "hipHostMalloc failed for the staging area");
+        }
+        host_stage_bytes = want;
+    }
+    *out = host_stage;
+    return BT_OK;
+}
+
+int Context::stage_done() {
+    if (hipEventRecord(ev_stage, stream) != hipSuccess) return fail(BT_ERR_HIP, "staging: event record failed");
+    return BT_OK;
+}
+
 int Context::pool_alloc(size_t bytes, void** out, size_t* got) {
     auto it = pool.lower_bound(bytes);
     if (it != pool.end() && it->first <= 2 * bytes) {
@@ -264,7 +287,8 @@ int bt_context_create(int device, bt_context** out) {
         hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c.ev_hand, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c.ev_back, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c.ev_back, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c.ev_stage, hipEventDisableTiming) != hipSuccess) {
         delete h;
         return fail(BT_ERR_HIP, "hipStreamCreate failed");
     }
@@ -290,6 +314,8 @@ void bt_context_destroy(bt_context* h) {
     hipEventDestroy(c.ev_join);
     hipEventDestroy(c.ev_hand);
     hipEventDestroy(c.ev_back);
+    hipEventDestroy(c.ev_stage);
+    if (c.host_stage) hipHostFree(c.host_stage);
     hipStreamDestroy(c.aux);
     hipStreamDestroy(c.stream);
     delete h;

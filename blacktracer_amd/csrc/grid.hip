@@ -549,9 +549,9 @@ int grid_fix_tvertices_multi(Context* c, const std::vector<bt_grid*>& gs) {
         lmin = std::min(lmin, g->k.start_level);
         lmax = std::max(lmax, g->k.max_level);
     }
-    void* p;
-    if (int rc = c->buffer("fix_jobs", sizeof(FixJob) * gs.size(), &p)) return rc;
-    FixJob* d_jobs = (FixJob*)p;
+    // every level's jobs, uploaded in one copy before the first launch
+    std::vector<FixJob> all;
+    std::vector<std::pair<size_t, unsigned>> lv;   // (first job, items) per level
     std::vector<FixJob> jobs;
     for (int L = lmin; L < lmax; L++) {
         jobs.clear();
@@ -570,14 +570,23 @@ int grid_fix_tvertices_multi(Context* c, const std::vector<bt_grid*>& gs) {
             jobs.push_back(j);
         }
         if (jobs.empty()) continue;
-        // the previous level's launch may still read d_jobs
-        BT_HIP(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(FixJob) * jobs.size(), hipMemcpyHostToDevice, c->stream));
-        BT_HIP(hipStreamSynchronize(c->stream));
+        lv.push_back({all.size(), items});
+        all.insert(all.end(), jobs.begin(), jobs.end());
+    }
+    if (all.empty()) return BT_OK;
+    void* p;
+    if (int rc = c->buffer("fix_jobs", sizeof(FixJob) * all.size(), &p)) return rc;
+    FixJob* d_all = (FixJob*)p;
+    BT_HIP(hipMemcpyAsync(d_all, all.data(), sizeof(FixJob) * all.size(), hipMemcpyHostToDevice, c->stream));
+    for (size_t v = 0; v < lv.size(); v++) {
+        const size_t first = lv[v].first, njobs = (v + 1 < lv.size() ? lv[v + 1].first : all.size()) - first;
         KTimer t(c, "fix_tvertices");
-        fix_pole_multi<<<(unsigned)jobs.size(), 64, 0, c->stream>>>(d_jobs);
-        fix_level_multi<<<blocks_for((int64_t)items * 64, 256), 256, 0, c->stream>>>(d_jobs, (int)jobs.size());
+        fix_pole_multi<<<(unsigned)njobs, 64, 0, c->stream>>>(d_all + first);
+        fix_level_multi<<<blocks_for((int64_t)lv[v].second * 64, 256), 256, 0, c->stream>>>(d_all + first,
+                                                                                           (int)njobs);
         BT_HIP(hipGetLastError());
     }
+    // the pageable job table is read by the copy: synchronise before it goes
     BT_HIP(hipStreamSynchronize(c->stream));
     return BT_OK;
 }

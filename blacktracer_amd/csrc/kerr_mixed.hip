@@ -29,6 +29,8 @@
 // with X = pTheta^2 + pR^2 Delta - Theta - P Ph + partR (zero on the null
 // cone; kept so the flow is the reference's off the cone as well).
 // Compiled with -ffp-contract=fast.
+#include <cstring>
+
 #include "kerr_dev.h"
 
 namespace bt {
@@ -384,17 +386,21 @@ int trace_batch(Context* c, const BatchHost& b, const ParamsK& prm, StatsK* d_st
     int rc;
     MultiSrc S;
     S.nseg = nseg;
-    if ((rc = c->buffer("batch_poses", sizeof(PoseK) * nseg, &p))) return rc;
+    // the segment table, through pinned staging: one copy, no synchronisation
+    const size_t o_ios = (sizeof(PoseK) * nseg + 255) & ~(size_t)255;
+    const size_t o_start = o_ios + ((sizeof(RayIO) * nseg + 255) & ~(size_t)255);
+    const size_t tbytes = o_start + sizeof(int64_t) * (nseg + 1);
+    void* hs;
+    if ((rc = c->stage(tbytes, &hs))) return rc;
+    std::memcpy(hs, b.poses.data(), sizeof(PoseK) * nseg);
+    std::memcpy((char*)hs + o_ios, b.ios.data(), sizeof(RayIO) * nseg);
+    std::memcpy((char*)hs + o_start, start.data(), sizeof(int64_t) * (nseg + 1));
+    if ((rc = c->buffer("batch_table", tbytes, &p))) return rc;
     S.poses = (const PoseK*)p;
-    BT_HIP(hipMemcpyAsync(p, b.poses.data(), sizeof(PoseK) * nseg, hipMemcpyHostToDevice, c->stream));
-    if ((rc = c->buffer("batch_ios", sizeof(RayIO) * nseg, &p))) return rc;
-    S.ios = (const RayIO*)p;
-    BT_HIP(hipMemcpyAsync(p, b.ios.data(), sizeof(RayIO) * nseg, hipMemcpyHostToDevice, c->stream));
-    if ((rc = c->buffer("batch_start", sizeof(int64_t) * (nseg + 1), &p))) return rc;
-    S.start = (const int64_t*)p;
-    BT_HIP(hipMemcpyAsync(p, start.data(), sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice, c->stream));
-    // pageable sources: wait until the copies have consumed the host vectors
-    BT_HIP(hipStreamSynchronize(c->stream));
+    S.ios = (const RayIO*)((char*)p + o_ios);
+    S.start = (const int64_t*)((char*)p + o_start);
+    BT_HIP(hipMemcpyAsync(p, hs, tbytes, hipMemcpyHostToDevice, c->stream));
+    if ((rc = c->stage_done())) return rc;
     return prm.precision == BT_PREC_MIXED ? trace_mixed_src(c, S, prm, n, d_stats)
                                           : trace_f64_src(c, S, prm, n, d_stats);
 }
