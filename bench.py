@@ -115,7 +115,7 @@ def err_stats(cel_a, cel_b):
             "frac_above_1arcsec": float(np.mean(e > 1.0))}
 
 
-def cpu_baseline(seconds):
+def cpu_baseline(seconds, keep=None):
     """The reference's own CPU integrator (Metric.h's checkCelest and
     rkckIntegrate1, compiled from its sources into oracle/_ref) tracing every
     64th ray of the 2049 x 4096 sky on the host cores; the C oracle where
@@ -135,13 +135,20 @@ def cpu_baseline(seconds):
         # initial conditions (Grid.h:524-547, restated) outside the timed region
         ic = O.initial_conditions_batch(ps, th, ph)
         el = 0.0
+        res = [np.full(th.size, np.nan), np.full(th.size, np.nan), np.zeros(th.size, np.int32)]
         while el < seconds:
             part = ic[k:k + chunk]
             t0 = time.perf_counter()
-            O.ref_trace_batch(POSE["a"], POSE["r"], POSE["theta"], POSE["phi"], part, nthreads=threads)
+            r_th, r_ph, r_st = O.ref_trace_batch(POSE["a"], POSE["r"], POSE["theta"], POSE["phi"], part,
+                                                  nthreads=threads)
             el += time.perf_counter() - t0
+            if rays < th.size:   # the first pass over the sample: kept for the parity check
+                res[0][k:k + part.shape[0]], res[1][k:k + part.shape[0]], res[2][k:k + part.shape[0]] = \
+                    r_th, r_ph, r_st
             rays += part.shape[0]
             k = (k + chunk) % th.size
+        if keep is not None:
+            keep.extend(res)
         return {"value": rays / el, "unit": "rays/s", "cores": threads, "kind": "reference",
                 "sample": f"{rays} rays of every 64th direction of the 2049x4096 sky (reference settings): "
                           f"the reference's checkCelest + rkckIntegrate1 built from its sources, OpenMP over "
@@ -158,6 +165,29 @@ def cpu_baseline(seconds):
     return {"value": rays / el, "unit": "rays/s", "cores": threads, "kind": "port",
             "sample": f"{rays} rays of every 64th direction of the 2049x4096 sky (reference settings), "
                       f"C oracle, OpenMP over rays, {el:.1f} s"}
+
+
+def fp64_vs_reference(gpu, ref):
+    """The FP64 grid against the reference's own integrator (the CPU
+    baseline's first pass) on every 64th direction of the sky, the pole rows
+    left out (the grid copies its pole rays along them, Grid.h:306-312)."""
+    N = 2 ** LEVEL + 1
+    M = 2 * (N - 1)
+    idx = np.arange(N * M)[::64]
+    rows = idx // M
+    m = (rows > 0) & (rows < N - 1) & np.isfinite(ref[0])
+    gt, gp, gs = gpu[0][m], gpu[1][m], gpu[2][m]
+    rt, rp, rs = ref[0][m], ref[1][m], ref[2][m]
+    cap_g, cap_r = gt == -1, rt == -1
+    esc = ~cap_g & ~cap_r
+    dt = np.abs(gt[esc] - rt[esc])
+    dp = np.abs(gp[esc] - rp[esc])
+    dp = np.minimum(dp, 2 * math.pi - dp)
+    return {"rays": int(m.sum()), "capture_flags_equal": bool(np.array_equal(cap_g, cap_r)),
+            "steps_equal_frac": float(np.mean(gs == rs)), "max_abs_rad": float(max(dt.max(initial=0), dp.max(initial=0))),
+            "frac_beyond_1e-9_rad": float(np.mean(np.maximum(dt, dp) > 1e-9)),
+            "note": "GPU FP64 path vs the reference's checkCelest + rkckIntegrate1 (built from its sources), "
+                    "every 64th direction of the 2049x4096 sky, pole rows excluded"}
 
 
 def frames(bt, S, ctx, nframes):
@@ -524,6 +554,9 @@ def main():
     prec["fp64_attempt_share"] = stm["attempts_fp64"] / max(1, stm["attempts"])
     prec["fallback_rays"] = stm["fallback_rays"]
     sub64 = g64.cel[::4, ::4]
+    # the FP64 grid's values at the CPU baseline's sample (every 64th direction)
+    ref_sample = (g64.cel[..., 0].ravel()[::64].copy(), g64.cel[..., 1].ravel()[::64].copy(),
+                  g64.steps.ravel()[::64].copy())
     gm.close()
     g64.close()
     conv = bt.Grid(LEVEL, LEVEL, True, cam, bh, ctx=ctx,
@@ -573,7 +606,10 @@ def main():
     if not args.no_serve:
         out["serving"] = serve(bt, S, ctx, args.serve_seconds)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        kept = []
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, kept)
+        if kept:
+            out["fp64_vs_reference"] = fp64_vs_reference(ref_sample, kept)
     print(json.dumps(out))
 
 
