@@ -218,6 +218,11 @@ def frames(bt, S, ctx, nframes):
     kern = {n: ctx.timer(n) for n in RENDER_KERNELS}
     ctx.timing(False)
     nf = len(ms)
+    # one frame to the host, for the CPU leg's check against the oracle
+    off_chk = 2 * math.pi * 7 / (0.25 * speed * W)
+    check = {"img": bt.render_frame(g, sc, view, flags=bt.RenderFlags(offset=off_chk)),
+             "grid": g.cel.astype(np.float32), "gridlvl": g.MAXLEVEL, "cam7": cam.getParamArray(),
+             "offset": off_chk, "stars": stars, "sky": sky}
     extra = display_effects(bt, ctx, g, sc, view, buf, cam, bh, speed)
     kern_ms = sum(v[1] for v in kern.values()) / nf
     b_frame = 8 * g.N * g.M + 16 * stars.shape[0] + 32 * W * H + 4 * W * H
@@ -232,7 +237,28 @@ def frames(bt, S, ctx, nframes):
             "bytes_per_frame": b_frame, "frame_ms_p50": pct(ms, 0.5), "kernel_ms_per_frame": kern_ms,
             "per_kernel_ms": {n: v[1] / max(1, v[0]) for n, v in kern.items()}}
     g.close()
-    return out, roof
+    return out, roof, check
+
+
+def cpu_render(check, threads):
+    """The C oracle's 1080p frame of the full-frame config on the host cores
+    (a port: the reference renders only on CUDA), and the GPU frame checked
+    against it."""
+    from oracle import oracle as O
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    tree, pos, mag = O.make_tree(check["stars"], 8)
+    table = O.sum_table(check["sky"])
+    view = O.camera_view(math.pi / 2, 0.0, 0.25 * math.pi / 4, W, H)
+    t0 = time.perf_counter()
+    img = O.frame(check["grid"], check["gridlvl"], view, check["cam7"], tree, 8, pos, mag, table,
+                  offset=check["offset"])
+    el = time.perf_counter() - t0
+    d = np.abs(img.astype(np.int16) - check["img"].astype(np.int16))
+    return {"frame_ms": 1e3 * el, "cores": threads, "kind": "port",
+            "sample": "one 1920x1080 frame (stars, sky, lensing) by the C oracle; the star tree and summed table "
+                      "are built beforehand, untimed",
+            "gpu_frame_vs_oracle": {"max_channel_diff": int(d.max()),
+                                    "frac_pixels_differing": float((d.max(axis=-1) > 0).mean())}}
 
 
 def display_effects(bt, ctx, g, sc, view, buf, cam, bh, speed, nframes=20):
@@ -417,7 +443,7 @@ def profile_run(bt, S, ctx):
             g = bt.Grid(LEVEL, LEVEL, True, cam, bh, params=bt.IntegParams(prec), ctx=ctx)
             out[name] = g.stats()
             g.close()
-    fr, _ = frames(bt, S, ctx, 2)
+    fr, _, _ = frames(bt, S, ctx, 2)
     out["frame_ms"] = fr["frame_ms_p50"]
     print(json.dumps(out))
 
@@ -565,7 +591,7 @@ def main():
     prec["fp64_vs_converged"]["note"] = "FP64 path (eps 1e-5) against FP64 at eps 1e-10, 513x1024 sub-lattice"
     conv.close()
 
-    fr, render_roof = frames(bt, S, ctx, args.frames)
+    fr, render_roof, frame_check = frames(bt, S, ctx, args.frames)
     out = {
         "metric": "rays/s (mixed vs FP64 at <=1 arcsec err); p50/p99 frame ms at 1080p",
         "value": value,
@@ -610,6 +636,7 @@ def main():
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, kept)
         if kept:
             out["fp64_vs_reference"] = fp64_vs_reference(ref_sample, kept)
+        out["cpu_render_baseline"] = cpu_render(frame_check, out["cpu_baseline"]["cores"])
     print(json.dumps(out))
 
 
