@@ -519,6 +519,8 @@ class Trails:
         sp = None
         if diffraction is not None:
             fs = width // 16
+            if fs < 1:
+                raise ValueError("a diffraction sprite needs a frame at least 16 pixels wide")
             self._sprite = np.ascontiguousarray(diffraction, dtype=np.uint8).reshape(fs, fs, 3)
             sp = self._sprite.ctypes.data_as(_lib.U8P)
         h = C.c_void_p()

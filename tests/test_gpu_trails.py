@@ -93,3 +93,6 @@ def test_trails_reset_and_bad_state(ctx, world):
     with pytest.raises(bt.BtError):
         T.render(world["grid"], world["scene"], small)
     T.close()
+    # the sprite is (width / 16)^2 pixels: no sprite fits a frame narrower than 16
+    with pytest.raises((bt.BtError, ValueError)):
+        bt.Trails(world["scene"], 8, 8, diffraction=np.zeros((1, 1, 3), np.uint8), ctx=ctx)
