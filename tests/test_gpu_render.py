@@ -108,3 +108,38 @@ def test_edge_sizes(ctx, world):
         img = bt.render_frame(world["grid"], world["scene"], bt.View(W, H, ctx=ctx), flags=f)
         oimg, _ = oracle_frame(world, W, H, f)
         compare_frames(img, oimg, max_frac=2e-3)
+
+
+def test_half_sky_frame_matches_oracle(ctx, world):
+    """A half-sky grid (Grid(..., angle = false, ...)) is mirrored to 2 GN - 1
+    rows before the frame (makeGrid, MakeImageKernel.cu:369-384,1259-1260).
+    The resident path and the perfect-hash path both render the oracle's
+    frame of the oracle's mirror of the grid's own hash tables."""
+    W, H = 320, 180
+    g = bt.Grid(9, 1, False, world["cam"], world["bh"], ctx=ctx)
+    assert g.equafactor == 0 and g.M / g.N > 3
+    ht, ot, tag, hw, ow = g.psh()
+    gridf = O.make_grid(ht, ot, tag, hw, ow, g.N, g.M)
+    assert gridf.shape == (2 * g.N - 1, g.M, 2)
+    print(f"half-sky frame {W}x{H} from a {g.N}x{g.M} grid", flush=True)
+    f = bt.RenderFlags()
+    img, st = bt.render_frame(g, world["scene"], bt.View(W, H, ctx=ctx), flags=f, stages=True)
+    view = O.camera_view(math.pi / 2, 0.0, 0.25 * math.pi / 4, W, H)
+    tree, pos, mag = O.make_tree(world["stars"], 8)
+    oimg, ost = O.frame(gridf, g.MAXLEVEL, view, world["cam"].getParamArray(), tree, 8, pos, mag,
+                        O.sum_table(world["sky"]), stages=True)
+    assert np.array_equal(st["thphi"], ost["thphi"])
+    assert np.array_equal(st["gap"], ost["gap"])
+    assert np.array_equal(st["bh"], ost["bh"])
+    nb = st["bh"] == 0
+    assert np.array_equal(st["area"][nb], ost["area"][nb])
+    assert np.allclose(st["star"], ost["star"], rtol=1e-4, atol=1e-3, equal_nan=True)
+    compare_frames(img, oimg)
+    # the view reaches below the equator, so the mirrored rows are read
+    assert (ost["thphi"][..., 0] > math.pi / 2 + 0.1).sum() > 1000
+    vm = bt.viewer_matrix(math.pi / 2, 0.0, 0.25 * math.pi / 4, W, H)
+    btree, bpos, bmag = bt.star_tree(world["stars"], 8)
+    img2 = bt.make_image(bpos, btree, world["cam"].getParamArray(), bmag, 8, W, H, 1, world["sky"], vm,
+                         g.M, g.N, g.MAXLEVEL, ot, ht, tag, hw, ow, flags=f, ctx=ctx)
+    assert np.array_equal(img2, img)
+    g.close()

@@ -187,3 +187,102 @@ def test_fix_tvertices_moves_real_tvertices():
     assert changed.sum() > 100
     assert (g0.present[changed] == 1).all()
     assert np.array_equal(g0.level, g1.level)
+
+
+@pytest.mark.parametrize("pose,lv,start", [((0.999, math.pi / 2, 0.0, 5.0), 6, 6),
+                                           ((0.999, math.pi / 2, 0.0, 5.0), 7, 1),
+                                           ((0.5, math.pi / 2, 0.0, 12.0), 7, 1)])
+def test_half_sky_grid_is_upper_half_of_full_grid(pose, lv, start):
+    """Grid(angle = false) (Grid.h:605-615,291-333): N = 2^L / 2 + 1 rows over
+    theta in [0, pi/2], the north pole the only start point and the equator
+    row traced as a lattice row.  Row i has the direction of the full grid's
+    row i (i / (N - 1) * pi / (2 - equafactor) is the same double: the
+    divisions are by powers of two), and a block's refinement reads only its
+    corners, so before fixTvertices the half grid is the full grid's upper
+    rows bit for bit.  On the equator row the full grid also holds the points
+    of refined blocks below it."""
+    ps = O.pose(*pose)
+    h = O.OracleGrid(ps, O.params(), lv, start, False, fix_tvertices=False, nthreads=4)
+    f = O.OracleGrid(ps, O.params(), lv, start, True, fix_tvertices=False, nthreads=4)
+    n = h.N
+    assert (h.N, h.M) == (2 ** (lv - 1) + 1, f.M) and f.N == 2 * n - 1
+    up = slice(0, n - 1)
+    assert np.array_equal(h.level[up], f.level[up])
+    assert (h.level[n - 1] == -1).all() and (f.level[n - 1] >= 0).any()
+    assert np.array_equal(h.present[up], f.present[up])
+    assert np.array_equal(h.cel[up], f.cel[up])
+    assert np.array_equal(h.steps[up], f.steps[up])
+    eq = h.present[n - 1] == 1
+    assert eq.sum() >= h.M // 2 ** (lv - start) and (f.present[n - 1][eq] == 1).all()
+    assert np.array_equal(h.cel[n - 1][eq], f.cel[n - 1][eq])
+
+
+def identity_psh(cel, present):
+    """A perfect hash with offset table (0, 0) and one slot per (i, j): the
+    lookup of hashLookup (kernel.cuh:705-733) returns the point's own value."""
+    N, M = present.shape
+    hw = max(N, M)
+    ht = np.zeros((hw * hw, 2), np.float32)
+    tag = np.full((hw * hw, 2), -1, np.int32)
+    ii, jj = np.nonzero(present == 1)
+    ht[ii * hw + jj] = cel[ii, jj].astype(np.float32)
+    tag[ii * hw + jj, 0], tag[ii * hw + jj, 1] = ii, jj
+    return ht, np.zeros((1, 2), np.int32), tag, hw, 1
+
+
+@pytest.mark.parametrize("pose,lv,start", [((0.999, math.pi / 2, 0.0, 5.0), 6, 6),
+                                           ((0.999, math.pi / 2, 0.0, 5.0), 7, 1),
+                                           ((0.5, math.pi / 2, 0.0, 12.0), 7, 1)])
+def test_half_sky_mirror_is_the_traced_lower_half(pose, lv, start):
+    """makeGrid's mirror of a half grid (MakeImageKernel.cu:369-384,1259-1260:
+    GM / GN > 3, row GN1 - 1 - i gets pi - theta) against the traced lower
+    half of the full grid: a camera in the equatorial plane sees a sky that
+    is symmetric under theta -> pi - theta, so both are the same rays up to
+    the integrator's own error.  A missing flip or a shifted row errs by the
+    grid spacing (about 0.05 rad); the bound is 1e-4 rad."""
+    ps = O.pose(*pose)
+    h = O.OracleGrid(ps, O.params(), lv, start, False, fix_tvertices=False, nthreads=4)
+    f = O.OracleGrid(ps, O.params(), lv, start, True, fix_tvertices=False, nthreads=4)
+    n = h.N
+    g = O.make_grid(*identity_psh(h.cel, h.present), h.N, h.M)
+    assert g.shape == (2 * n - 1, h.M, 2)
+    assert np.array_equal(g[:n - 1], np.where(h.present[:n - 1, :, None] == 1,
+                                              h.cel[:n - 1].astype(np.float32), np.float32(-2)))
+    lo = g[n - 1:]
+    full = f.cel[n - 1:].astype(np.float32)
+    both = (f.present[n - 1:] == 1) & (lo[..., 0] != -2)
+    assert both.sum() > 0.9 * (h.present == 1).sum()
+    assert np.array_equal(lo[..., 0][both] == -1, full[..., 0][both] == -1)
+    vis = both & (lo[..., 0] != -1)
+    dt = np.abs(lo[..., 0] - full[..., 0])[vis]
+    dp = np.abs(lo[..., 1] - full[..., 1])[vis]
+    dp = np.minimum(dp, 2 * np.pi - dp)
+    assert max(dt.max(), dp.max()) < 1e-4
+
+
+def test_half_sky_frame_is_the_full_sky_frame():
+    """For a camera in the equatorial plane, a frame rendered from the
+    mirrored half-sky grid (makeGrid's sym branch, MakeImageKernel.cu:369-384)
+    shows the same sky as the frame of the full grid: the two differ only
+    where the mirrored rays' integrator error moves a pixel across a colour
+    step, which is rare (measured 0.09% of pixels, max 11/255 on one pixel in
+    20000)."""
+    from blacktracer_amd import scene
+    ps = O.pose(0.999, math.pi / 2, 0.0, 5.0)
+    cam7 = [ps.speed, ps.alpha, ps.w, ps.wbar, ps.br, ps.btheta, ps.bphi]
+    h = O.OracleGrid(ps, O.params(), 9, 1, False, nthreads=4)
+    f = O.OracleGrid(ps, O.params(), 9, 1, True, nthreads=4)
+    mirrored = O.make_grid(*identity_psh(h.cel, h.present), h.N, h.M)
+    assert mirrored.shape == f.cel.shape
+    W, H = 320, 180
+    view = O.camera_view(math.pi / 2, 0.0, 0.25 * math.pi / 4, W, H)
+    tree, pos, mag = O.make_tree(scene.synthetic_stars(20000, 21), 8)
+    table = O.sum_table(scene.synthetic_sky(512, seed=5))
+    a, sa = O.frame(mirrored, 9, view, cam7, tree, 8, pos, mag, table, stages=True)
+    b = O.frame(f.cel.astype(np.float32), 9, view, cam7, tree, 8, pos, mag, table)
+    below = sa["thphi"][..., 0] > math.pi / 2 + 0.1
+    assert below.sum() > 1000
+    d = np.abs(a.astype(np.int16) - b.astype(np.int16)).max(axis=-1)
+    assert (d > 0).mean() < 5e-3
+    assert (d > 8).mean() < 1e-4
+    assert (a[..., :3].max(axis=-1) > 0).mean() > 0.3
