@@ -143,3 +143,26 @@ def test_half_sky_frame_matches_oracle(ctx, world):
                          g.M, g.N, g.MAXLEVEL, ot, ht, tag, hw, ow, flags=f, ctx=ctx)
     assert np.array_equal(img2, img)
     g.close()
+
+
+@pytest.mark.parametrize("motion", [dict(br=0.6, btheta=0.0, bphi=0.8), dict(br=-0.3, btheta=0.5, bphi=math.sqrt(0.66)),
+                                    dict(speed=0.3)])
+def test_moving_camera_frame_matches_oracle(ctx, world, motion):
+    """A camera moving along B = (br, btheta, bphi) or at a caller's speed:
+    the grid's initial conditions and the redshift of every pixel
+    (kernel.cuh:348-365) take the motion from the camera's parameter array."""
+    W, H = 320, 180
+    print(f"moving camera frame {motion}", flush=True)
+    cam = bt.Camera(math.pi / 2, 0.0, 5.0, black_hole=world["bh"], **motion)
+    g = bt.Grid(8, 1, True, cam, world["bh"], ctx=ctx)
+    f = bt.RenderFlags()
+    img, st = bt.render_frame(g, world["scene"], bt.View(W, H, ctx=ctx), flags=f, cam7=cam.getParamArray(),
+                              stages=True)
+    view = O.camera_view(math.pi / 2, 0.0, 0.25 * math.pi / 4, W, H)
+    tree, pos, mag = O.make_tree(world["stars"], 8)
+    oimg, ost = O.frame(g.cel.astype(np.float32), g.MAXLEVEL, view, cam.getParamArray(), tree, 8, pos, mag,
+                        O.sum_table(world["sky"]), stages=True)
+    assert np.array_equal(st["thphi"], ost["thphi"])
+    assert np.allclose(st["star"], ost["star"], rtol=1e-4, atol=1e-3, equal_nan=True)
+    compare_frames(img, oimg)
+    g.close()

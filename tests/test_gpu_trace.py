@@ -54,6 +54,52 @@ def test_trace_rays_fp64_matches_oracle(ctx, pose):
     print(f"  max |d angle| = {err:.3g} rad", flush=True)
 
 
+@pytest.mark.parametrize("pose,motion", [
+    ((0.999, math.pi / 2, 0.0, 5.0), dict(br=0.6, btheta=0.0, bphi=0.8)),
+    ((0.7, 1.1, 0.4, 6.0), dict(br=-0.3, btheta=0.5, bphi=math.sqrt(0.66))),
+    ((0.5, 1.3, 0.0, 9.0), dict(speed=0.3)),     # Camera(theta, phi, r, speed)
+    ((0.5, 1.3, 0.0, 9.0), dict(speed=0.0)),
+])
+def test_trace_rays_moving_camera_matches_oracle(ctx, pose, motion):
+    """Cameras moving along another direction than the orbit (Camera.h:40-52)
+    or at a speed of the caller's (Camera.h:27-38): the pose constants and
+    the parameter array are bit-exact, the traced rays within 1e-9 rad."""
+    a, th0, ph0, r = pose
+    print(f"trace_rays moving camera pose={pose} motion={motion}", flush=True)
+    cam = bt.Camera(th0, ph0, r, black_hole=bt.BlackHole(a), **motion)
+    if "speed" in motion:
+        ps = O.pose(a, th0, ph0, r, speed=motion["speed"])
+    else:
+        ps = O.pose(a, th0, ph0, r, motion["br"], motion["btheta"], motion["bphi"])
+    cam7 = np.array([ps.speed, ps.alpha, ps.w, ps.wbar, ps.br, ps.btheta, ps.bphi], np.float32)
+    assert np.array_equal(cam.getParamArray(), cam7)
+    assert [cam.speed, cam.alpha, cam.w, cam.wbar, cam.Delta, cam.ro] == \
+        [ps.speed, ps.alpha, ps.w, ps.wbar, ps.Delta, ps.ro]
+    t, p = _dirs(20000, 5)
+    th, ph, st = bt.trace_rays(cam, t, p, ctx=ctx)
+    oth, oph, ost, _ = O.trace_rays(ps, O.params(), t, p)
+    err = _compare(th, ph, st, oth, oph, ost)
+    print(f"  max |d angle| = {err:.3g} rad", flush=True)
+
+
+def test_grid_moving_camera_matches_oracle(ctx):
+    a, th0, ph0, r = 0.9, 1.2, 0.0, 6.0
+    motion = (0.5, 0.3, math.sqrt(1 - 0.34))
+    cam = bt.Camera(th0, ph0, r, *motion, black_hole=bt.BlackHole(a))
+    g = bt.Grid(8, 1, True, cam, bt.BlackHole(a), ctx=ctx)
+    og = O.OracleGrid(O.pose(a, th0, ph0, r, *motion), O.params(), 8, 1, True)
+    assert g.rays_traced == og.rays_traced
+    assert np.array_equal(g.levels, og.level)
+    present = g.cel[..., 0] != -2
+    assert np.array_equal(present, og.present == 1)
+    assert np.mean(g.steps != og.steps) < 1e-3
+    m = present & (g.cel[..., 0] != -1)
+    assert np.array_equal(g.cel[..., 0] == -1, og.cel[..., 0] == -1)
+    d = np.abs(g.cel - og.cel)[m]
+    d[:, 1] = np.minimum(d[:, 1], np.abs(d[:, 1] - 2 * math.pi))
+    assert np.mean(d > 1e-9) < 5e-4
+
+
 def test_trace_rays_fp64_matches_reference_vectors(ctx):
     """Against the reference's own outputs (tests/golden/ref_rays.npz)."""
     import os

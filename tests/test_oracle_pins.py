@@ -286,3 +286,45 @@ def test_half_sky_frame_is_the_full_sky_frame():
     assert (d > 0).mean() < 5e-3
     assert (d > 8).mean() < 1e-4
     assert (a[..., :3].max(axis=-1) > 0).mean() > 0.3
+
+
+def _fido_direction(ps, t, p):
+    """The unit direction (r, theta, phi) in the FIDO frame that the initial
+    conditions of camera direction (t, p) encode (Grid.h:524-542), recovered
+    from (pR, pTheta, b): b = eF wbar n_phi with eF = 1 / (alpha + w wbar
+    n_phi), pTheta = eF ro n_theta, pR = eF ro n_r / sqrtf(Delta)."""
+    pR, pT, b, _ = O.initial_conditions(ps, t, p)
+    nph = ps.alpha * b / (ps.wbar * (1 - ps.w * b))
+    eF = 1 / (ps.alpha + ps.w * ps.wbar * nph)
+    sq_delta = float(np.sqrt(np.float32(ps.Delta)))
+    return np.array([pR * sq_delta / (eF * ps.ro), pT / (eF * ps.ro), nph])
+
+
+@pytest.mark.parametrize("pose,motion", [((0.999, math.pi / 2, 0.0, 5.0), (0.6, 0.0, 0.8)),
+                                         ((0.7, 1.1, 0.4, 6.0), (-0.3, 0.5, math.sqrt(0.66))),
+                                         ((0.2, 2.0, 1.0, 15.0), (0.0, -0.6, 0.8))])
+def test_camera_motion_direction_is_a_rotation(pose, motion):
+    """Camera(theta, phi, r, br, btheta, bphi) (Camera.h:40-52) moves along
+    the unit direction B = (br, btheta, bphi); the initial conditions
+    aberrate a camera direction by the camera's speed along its y axis and
+    then rotate the camera frame onto the FIDO frame (Grid.h:528-537).  The
+    rotation's columns, recovered from the camera's x, y and z axes, must be
+    orthonormal with determinant +1, and the y axis must go to -B (the
+    aberration maps y = 1 to y = -1 at any speed)."""
+    ps = O.pose(*pose, *motion)
+    B = np.array(motion)
+    sp = ps.speed
+    s = float(np.sqrt(np.float32(1 - sp * sp)))
+    ny = _fido_direction(ps, math.pi / 2, math.pi / 2)
+    nx = _fido_direction(ps, math.pi / 2, 0.0)
+    nz = _fido_direction(ps, 0.0, 0.0)
+    assert np.allclose(ny, -B, atol=1e-6)
+    ex = (sp * B - nx) / s
+    ez = (sp * B - nz) / s
+    R = np.stack([ex, B, ez], axis=1)
+    assert np.allclose(R.T @ R, np.eye(3), atol=1e-6)
+    assert np.linalg.det(R) == pytest.approx(1.0, abs=1e-6)
+    # every aberrated direction stays a unit (null) direction
+    rng = np.random.default_rng(4)
+    for t, p in zip(np.arccos(1 - 2 * rng.random(50)), 2 * math.pi * rng.random(50)):
+        assert np.linalg.norm(_fido_direction(ps, t, p)) == pytest.approx(1.0, abs=1e-6)
