@@ -57,6 +57,44 @@ def test_batch_mixed_precision(ctx):
         assert np.array_equal(grids[k].cel, single.cel)
 
 
+HALF_SKY = [  # cameras in the equatorial plane, moving within it: (a, r, motion, max_level, start_level)
+    (0.999, 5.0, dict(), 9, 1),
+    (0.5, 2.5, dict(br=0.6, btheta=0.0, bphi=0.8), 8, 1),
+    (0.3, 15.0, dict(speed=0.2), 7, 7),
+]
+MOVING = [  # (a, r, theta, motion, max_level, start_level)
+    (0.9, 4.0, 1.2, dict(br=-0.3, btheta=0.5, bphi=math.sqrt(0.66)), 8, 2),
+    (0.6, 9.0, 0.7, dict(br=0.0, btheta=-0.6, bphi=0.8), 7, 1),
+]
+
+
+@pytest.mark.parametrize("angle", [False, True])
+@pytest.mark.parametrize("prec", [bt.BT_PREC_FP64, bt.BT_PREC_MIXED])
+def test_batch_half_sky_and_moving_cameras(ctx, angle, prec):
+    """Half-sky grids (angle = false) and cameras moving along another
+    direction than the orbit, or at a caller's speed: every grid of the
+    batch equals its single build bit for bit."""
+    if angle:
+        specs = [(a, r, th, m, ml, sl) for a, r, th, m, ml, sl in MOVING]
+    else:
+        specs = [(a, r, math.pi / 2, m, ml, sl) for a, r, m, ml, sl in HALF_SKY]
+    bhs = [bt.BlackHole(s[0]) for s in specs]
+    cams = [bt.Camera(th, 0.0, r, black_hole=bh, **m) for (a, r, th, m, _, _), bh in zip(specs, bhs)]
+    prm = bt.IntegParams(prec)
+    grids, st = bt.Grid.build_batch(cams, bhs, [s[4] for s in specs], [s[5] for s in specs], angle=angle,
+                                    params=prm, ctx=ctx)
+    for k, (cam, bh, s) in enumerate(zip(cams, bhs, specs)):
+        print(f"batch angle={angle} prec={prec} grid {k} spec={s}", flush=True)
+        single = bt.Grid(s[4], s[5], angle, cam, bh, params=prm, ctx=ctx)
+        g = grids[k]
+        assert (g.N, g.M, g.equafactor) == (single.N, single.M, 1 if angle else 0)
+        assert g.rays_traced == single.rays_traced
+        assert np.array_equal(g.cel, single.cel)
+        assert np.array_equal(g.levels, single.levels)
+        single.close()
+        g.close()
+
+
 def test_batch_rejects_bad_sizes(ctx):
     cams, bhs = _grids(POSES[:1])
     with pytest.raises(bt.BtError):
