@@ -105,22 +105,38 @@ __global__ void pole_rows(GridK G, int gap, uint32_t* blocks) {
     }
 }
 
-// integrateFirst(gap), Grid.h:325-338.
+// integrateFirst(gap), Grid.h:325-338.  The lattice index grows with t, so
+// the lanes of a wave whose points share a presence word are contiguous: a
+// segmented OR over the wave gathers each run's bits into its first lane,
+// which sets them with one atomic (a gap-1 lattice would otherwise put 32
+// atomics on every word).
 __global__ void start_lattice(GridK G, int gap, uint32_t nrows, uint32_t ncols, uint32_t* trace,
                               uint32_t* blocks, int mark_level) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nrows * ncols) return;
-    uint32_t i = (uint32_t)gap * (t / ncols + 1);
-    uint32_t j = (uint32_t)gap * (t % ncols);
-    uint32_t idx = i * (uint32_t)G.M + j;
-    trace[t] = idx;
-    set_present(G, idx);
-    if (i != (uint32_t)(G.N - 1)) {
-        if (mark_level >= 0)
-            G.level[idx] = (int8_t)mark_level;
-        else
-            blocks[t] = idx;   // the excluded row is the last one, so t stays dense
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < nrows * ncols;
+    uint32_t word = 0xFFFFFFFFu, bits = 0u;
+    if (live) {
+        uint32_t i = (uint32_t)gap * (t / ncols + 1);
+        uint32_t j = (uint32_t)gap * (t % ncols);
+        uint32_t idx = i * (uint32_t)G.M + j;
+        trace[t] = idx;
+        word = idx >> 5;
+        bits = 1u << (idx & 31);
+        if (i != (uint32_t)(G.N - 1)) {
+            if (mark_level >= 0)
+                G.level[idx] = (int8_t)mark_level;
+            else
+                blocks[t] = idx;   // the excluded row is the last one, so t stays dense
+        }
     }
+    const int lane = (int)(threadIdx.x & 63u);
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t ow = __shfl_down(word, off, 64);
+        uint32_t ob = __shfl_down(bits, off, 64);
+        if (lane + off < 64 && ow == word) bits |= ob;
+    }
+    uint32_t pw = __shfl_up(word, 1, 64);
+    if (live && (lane == 0 || pw != word)) atomicOr(&G.present[word], bits);
 }
 
 __device__ __forceinline__ void claim(const GridK& G, uint32_t i, uint32_t j, uint32_t* trace,
