@@ -84,11 +84,8 @@ __global__ void pole_keys(uint32_t* keys, uint32_t south) {
 }
 
 // raytrace(), Grid.h:303-315: copy the pole values along the pole rows at
-// the start-level spacing and queue the pole-row blocks.
-__global__ void pole_rows(GridK G, int gap, uint32_t* blocks) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t ncols = (uint32_t)G.M / (uint32_t)gap;
-    if (t >= ncols) return;
+// the start-level spacing and queue the pole-row blocks (column t < M / gap).
+__device__ __forceinline__ void pole_row_point(const GridK& G, int gap, uint32_t* blocks, uint32_t t) {
     uint32_t j = t * (uint32_t)gap;
     uint32_t M = (uint32_t)G.M;
     G.cel[2 * j] = G.cel[0];
@@ -105,15 +102,21 @@ __global__ void pole_rows(GridK G, int gap, uint32_t* blocks) {
     }
 }
 
-// integrateFirst(gap), Grid.h:325-338.  The lattice index grows with t, so
-// the lanes of a wave whose points share a presence word are contiguous: a
-// segmented OR over the wave gathers each run's bits into its first lane,
-// which sets them with one atomic (a gap-1 lattice would otherwise put 32
-// atomics on every word).
-__global__ void start_lattice(GridK G, int gap, uint32_t nrows, uint32_t ncols, uint32_t* trace,
-                              uint32_t* blocks, int mark_level) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = t < nrows * ncols;
+__global__ void pole_rows(GridK G, int gap, uint32_t* blocks) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint32_t)G.M / (uint32_t)gap) return;
+    pole_row_point(G, gap, blocks, t);
+}
+
+// integrateFirst(gap), Grid.h:325-338, lattice point t.  The lattice index
+// grows with t, so the lanes of a wave whose points share a presence word
+// are contiguous: a segmented OR over the wave gathers each run's bits into
+// its first lane, which sets them with one atomic (a gap-1 lattice would
+// otherwise put 32 atomics on every word).  Every lane of the wave calls
+// this (the OR is a wave collective); lanes with live false touch no memory.
+__device__ __forceinline__ void lattice_point(const GridK& G, int gap, uint32_t nrows, uint32_t ncols,
+                                              uint32_t* trace, uint32_t* blocks, int mark_level, uint32_t t,
+                                              bool live) {
     uint32_t word = 0xFFFFFFFFu, bits = 0u;
     if (live) {
         uint32_t i = (uint32_t)gap * (t / ncols + 1);
@@ -137,6 +140,77 @@ __global__ void start_lattice(GridK G, int gap, uint32_t nrows, uint32_t ncols, 
     }
     uint32_t pw = __shfl_up(word, 1, 64);
     if (live && (lane == 0 || pw != word)) atomicOr(&G.present[word], bits);
+}
+
+__global__ void start_lattice(GridK G, int gap, uint32_t nrows, uint32_t ncols, uint32_t* trace,
+                              uint32_t* blocks, int mark_level) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    lattice_point(G, gap, nrows, ncols, trace, blocks, mark_level, t, t < nrows * ncols);
+}
+
+// The set-up of every grid of a batch (init_grid + pole_keys, start_lattice,
+// pole_rows), one launch per step for all grids.  Job g owns the items
+// [first_g, first_{g+1}) of each launch; its lattice items start on a
+// multiple of 64, so that no wavefront spans two grids.
+struct SetupJob {
+    GridK G;
+    uint32_t* keys;     // [north | south | lattice] ray keys
+    uint32_t* blocks;   // [pole-row blocks | lattice blocks]
+    int64_t init0, lat0, pole0;
+    int64_t nm, nwords;
+    int gap, mark_level;
+    uint32_t nrows, ncols, npole;
+};
+
+template <int STEP>
+__device__ __forceinline__ const SetupJob& setup_job(const SetupJob* jobs, int njobs, int64_t t) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int64_t f = STEP == 0 ? jobs[mid].init0 : STEP == 1 ? jobs[mid].lat0 : jobs[mid].pole0;
+        if (f <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return jobs[lo];
+}
+
+__global__ void init_grid_multi(const SetupJob* jobs, int njobs, int64_t items) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= items) return;
+    const SetupJob& J = setup_job<0>(jobs, njobs, t);
+    const int64_t i = t - J.init0;
+    if (i >= J.nm) return;
+    const GridK& G = J.G;
+    G.cel[2 * i] = -2.0;
+    G.cel[2 * i + 1] = -2.0;
+    G.steps[i] = 0;
+    G.level[i] = -1;
+    if (i < J.nwords) G.present[i] = 0u;
+    if (i == 0) {
+        J.keys[0] = 0u;
+        J.keys[1] = (uint32_t)(G.N - 1) * (uint32_t)G.M;
+    }
+}
+
+// items is a multiple of 64: a wavefront lies wholly inside one job's range
+// or wholly past the end.
+__global__ void start_lattice_multi(const SetupJob* jobs, int njobs, int64_t items) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = t < items;
+    const SetupJob& J = setup_job<1>(jobs, njobs, in ? t : items - 1);
+    const int64_t lt = t - J.lat0;
+    const bool live = in && lt < (int64_t)J.nrows * J.ncols;
+    lattice_point(J.G, J.gap, J.nrows, J.ncols, J.keys + 1 + J.G.eq, J.blocks + J.npole, J.mark_level,
+                  live ? (uint32_t)lt : 0u, live);
+}
+
+__global__ void pole_rows_multi(const SetupJob* jobs, int njobs, int64_t items) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= items) return;
+    const SetupJob& J = setup_job<2>(jobs, njobs, t);
+    const int64_t lt = t - J.pole0;
+    if (lt >= (int64_t)J.npole) return;
+    pole_row_point(J.G, J.gap, J.blocks, (uint32_t)lt);
 }
 
 __device__ __forceinline__ void claim(const GridK& G, uint32_t i, uint32_t j, uint32_t* trace,
@@ -741,6 +815,12 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
     if (int zr = zero_words((unsigned*)d_stats, (int)(sizeof(StatsK) / 4), c->stream)) return zr;
     const ParamsK& prm = gs[0]->prm;
 
+    // raytrace() and integrateFirst(gap) of every grid (Grid.h:295-301,
+    // 325-338): per grid the keys [poles | start lattice]; the set-up of all
+    // grids goes through one job table, one launch per step.
+    std::vector<SetupJob> sj(G);
+    std::vector<int64_t> counts(G);
+    int64_t n_init = 0, n_lat = 0, n_pole = 0;
     for (int k = 0; k < G; k++) {
         bt_grid* g = gs[k];
         GridK& Gk = g->k;
@@ -752,9 +832,40 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
         per[k].next = (uint32_t*)p;
         if ((rc = c->buffer("gbatch_t" + sfx, sizeof(uint32_t) * (NM + 64), &p))) return rc;
         per[k].keys = (uint32_t*)p;
-        const int64_t nwords = (NM + 31) / 32;
-        init_grid<<<(unsigned)std::min<int64_t>(blocks_for(NM, 256), 4096), 256, 0, c->stream>>>(Gk, NM, nwords);
-        BT_HIP(hipGetLastError());
+        const int N = g->N, M = g->M, eq = g->eq;
+        const int gap0 = 1 << (g->max_level - g->start_level);
+        const unsigned npole = (unsigned)(M / gap0);
+        unsigned nrows = 0;
+        for (int i = gap0; i < N - eq; i += gap0) nrows++;
+        const unsigned ncols = (unsigned)(M / gap0);
+        const int64_t nlat = (int64_t)nrows * ncols;
+        const bool uniform = (g->max_level == g->start_level);
+        SetupJob& J = sj[k];
+        J.G = Gk;
+        J.keys = per[k].keys;
+        J.blocks = per[k].blocks;
+        J.nm = NM;
+        J.nwords = (NM + 31) / 32;
+        J.gap = gap0;
+        J.mark_level = uniform ? g->start_level : -1;
+        J.nrows = nrows;
+        J.ncols = ncols;
+        J.npole = npole;
+        J.init0 = n_init;
+        J.lat0 = n_lat;
+        J.pole0 = n_pole;
+        n_init += NM;
+        n_lat += (nlat + 63) / 64 * 64;
+        n_pole += npole;
+        counts[k] = 1 + eq + nlat;
+        per[k].nb = npole;
+        if (!uniform) {
+            unsigned lat_blocks = nrows * ncols;
+            if (!eq) lat_blocks -= ncols;   // row N-1 is traced but is no block
+            per[k].nb += lat_blocks;
+        }
+        per[k].level = g->start_level;
+        per[k].active = true;
         RayIO& io = per[k].io;
         std::memset(&io, 0, sizeof(io));
         io.N = g->N;
@@ -766,9 +877,20 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
         io.out_step = Gk.steps;
         io.keys = per[k].keys;
         g->rays = 0;
-        pole_keys<<<1, 64, 0, c->stream>>>(per[k].keys, (uint32_t)(g->N - 1) * (uint32_t)g->M);
-        BT_HIP(hipGetLastError());
     }
+    // the job table, through pinned staging: one copy, no synchronisation
+    const size_t sj_bytes = sizeof(SetupJob) * G;
+    void* hs;
+    if ((rc = c->stage(sj_bytes, &hs))) return rc;
+    std::memcpy(hs, sj.data(), sj_bytes);
+    if ((rc = c->buffer("gbatch_setup", sj_bytes, &p))) return rc;
+    const SetupJob* d_sj = (const SetupJob*)p;
+    BT_HIP(hipMemcpyAsync(p, hs, sj_bytes, hipMemcpyHostToDevice, c->stream));
+    if ((rc = c->stage_done())) return rc;
+    init_grid_multi<<<blocks_for(n_init, 256), 256, 0, c->stream>>>(d_sj, G, n_init);
+    BT_HIP(hipGetLastError());
+    start_lattice_multi<<<blocks_for(n_lat, 256), 256, 0, c->stream>>>(d_sj, G, n_lat);
+    BT_HIP(hipGetLastError());
     auto run_batch = [&](const std::vector<int64_t>& counts) -> int {
         BatchHost b;
         for (int k = 0; k < G; k++) {
@@ -780,40 +902,11 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
         }
         return trace_batch(c, b, prm, d_stats);
     };
-    // raytrace() and integrateFirst(gap) of every grid (Grid.h:295-301,
-    // 325-338), in one batch: per grid the keys [poles | start lattice]; the
-    // pole rows are filled once the pole rays are done.
-    std::vector<int64_t> counts(G);
-    for (int k = 0; k < G; k++) {
-        bt_grid* g = gs[k];
-        const int N = g->N, M = g->M, eq = g->eq;
-        const int gap0 = 1 << (g->max_level - g->start_level);
-        const unsigned npole = (unsigned)(M / gap0);
-        unsigned nrows = 0;
-        for (int i = gap0; i < N - eq; i += gap0) nrows++;
-        const unsigned ncols = (unsigned)(M / gap0);
-        const int64_t nlat = (int64_t)nrows * ncols;
-        const bool uniform = (g->max_level == g->start_level);
-        start_lattice<<<blocks_for(nlat, 256), 256, 0, c->stream>>>(
-            g->k, gap0, nrows, ncols, per[k].keys + 1 + eq, per[k].blocks + npole, uniform ? g->start_level : -1);
-        BT_HIP(hipGetLastError());
-        counts[k] = 1 + eq + nlat;
-        per[k].nb = npole;
-        if (!uniform) {
-            unsigned lat_blocks = nrows * ncols;
-            if (!eq) lat_blocks -= ncols;   // row N-1 is traced but is no block
-            per[k].nb += lat_blocks;
-        }
-        per[k].level = g->start_level;
-        per[k].active = true;
-    }
+    // the poles and start lattices of all grids in one batch; the pole rows
+    // are filled once the pole rays are done
     if ((rc = run_batch(counts))) return rc;
-    for (int k = 0; k < G; k++) {
-        const int gap0 = 1 << (gs[k]->max_level - gs[k]->start_level);
-        const unsigned npole = (unsigned)(gs[k]->M / gap0);
-        pole_rows<<<blocks_for(npole, 256), 256, 0, c->stream>>>(gs[k]->k, gap0, per[k].blocks);
-        BT_HIP(hipGetLastError());
-    }
+    pole_rows_multi<<<blocks_for(n_pole, 256), 256, 0, c->stream>>>(d_sj, G, n_pole);
+    BT_HIP(hipGetLastError());
     // adaptiveBlockIntegration(startLevel), all grids one level per round
     std::vector<unsigned> h_ctr((size_t)kCtrStride * G);
     std::vector<RefineJob> rjobs;
