@@ -24,8 +24,8 @@ int Context::buffer(const std::string& name, size_t bytes, void** out) {
     if (b.bytes < bytes) {
         if (b.ptr) {
             // the stream may still read the old buffer
-            hipStreamSynchronize(stream);
-            hipFree(b.ptr);
+            (void)hipStreamSynchronize(stream);
+            (void)hipFree(b.ptr);
             b.ptr = nullptr;
             b.bytes = 0;
         }
@@ -44,7 +44,7 @@ int Context::stage(size_t bytes, void** out) {
     if (ev_stage && hipEventSynchronize(ev_stage) != hipSuccess)
         return fail(BT_ERR_HIP, "staging: waiting for the previous copies failed");
     if (host_stage_bytes < bytes) {
-        if (host_stage) hipHostFree(host_stage);
+        if (host_stage) (void)hipHostFree(host_stage);
         host_stage = nullptr;
         host_stage_bytes = 0;
         const size_t want = bytes + bytes / 2 + 4096;
@@ -74,7 +74,7 @@ int Context::pool_alloc(size_t bytes, void** out, size_t* got) {
     }
     if (hipMalloc(out, bytes) != hipSuccess) {
         // give the pooled blocks back and try once more
-        for (auto& kv : pool) hipFree(kv.second);
+        for (auto& kv : pool) (void)hipFree(kv.second);
         pool.clear();
         pool_bytes = 0;
         if (hipMalloc(out, bytes) != hipSuccess) {
@@ -89,7 +89,7 @@ int Context::pool_alloc(size_t bytes, void** out, size_t* got) {
 void Context::pool_free(void* p, size_t bytes) {
     if (!p) return;
     if (pool_bytes + bytes > kPoolCap) {
-        hipFree(p);
+        (void)hipFree(p);
         return;
     }
     pool.emplace(bytes, p);
@@ -299,25 +299,25 @@ int bt_context_create(int device, bt_context** out) {
 void bt_context_destroy(bt_context* h) {
     if (!h) return;
     Context& c = h->impl;
-    hipSetDevice(c.device);
-    hipStreamSynchronize(c.stream);
-    hipStreamSynchronize(c.aux);
+    (void)hipSetDevice(c.device);
+    (void)hipStreamSynchronize(c.stream);
+    (void)hipStreamSynchronize(c.aux);
     for (auto& kv : c.scratch)
-        if (kv.second.ptr) hipFree(kv.second.ptr);
-    for (auto& kv : c.pool) hipFree(kv.second);
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    for (auto& kv : c.pool) (void)hipFree(kv.second);
     for (auto& t : c.pending) {
-        hipEventDestroy(t.start);
-        hipEventDestroy(t.stop);
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
     }
-    for (auto e : c.free_events) hipEventDestroy(e);
-    hipEventDestroy(c.ev_fork);
-    hipEventDestroy(c.ev_join);
-    hipEventDestroy(c.ev_hand);
-    hipEventDestroy(c.ev_back);
-    hipEventDestroy(c.ev_stage);
-    if (c.host_stage) hipHostFree(c.host_stage);
-    hipStreamDestroy(c.aux);
-    hipStreamDestroy(c.stream);
+    for (auto e : c.free_events) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(c.ev_fork);
+    (void)hipEventDestroy(c.ev_join);
+    (void)hipEventDestroy(c.ev_hand);
+    (void)hipEventDestroy(c.ev_back);
+    (void)hipEventDestroy(c.ev_stage);
+    if (c.host_stage) (void)hipHostFree(c.host_stage);
+    (void)hipStreamDestroy(c.aux);
+    (void)hipStreamDestroy(c.stream);
     delete h;
 }
 
@@ -481,9 +481,9 @@ int bt_device_alloc(bt_context* h, size_t bytes, void** out) {
 void bt_device_free(bt_context* h, void* ptr) {
     if (!h || !ptr) return;
     std::lock_guard<std::recursive_mutex> lk(h->impl.mu);
-    hipSetDevice(h->impl.device);
-    hipStreamSynchronize(h->impl.stream);
-    hipFree(ptr);
+    (void)hipSetDevice(h->impl.device);
+    (void)hipStreamSynchronize(h->impl.stream);
+    (void)hipFree(ptr);
 }
 
 int bt_memcpy(bt_context* h, void* dst, const void* src, size_t bytes, int to_device) {

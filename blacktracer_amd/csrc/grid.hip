@@ -1016,11 +1016,11 @@ int grid_alloc(Context* c, bt_grid* g) {
 }
 
 void grid_release(bt_grid* g) {
-    if (g->gridf) hipFree(g->gridf);
+    if (g->gridf) (void)hipFree(g->gridf);
     g->gridf = nullptr;
     if (g->block) {
         if (g->ctx) g->ctx->pool_free(g->block, g->block_bytes);
-        else hipFree(g->block);
+        else (void)hipFree(g->block);
     }
     g->block = nullptr;
     g->block_bytes = 0;

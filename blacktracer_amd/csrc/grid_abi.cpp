@@ -67,7 +67,7 @@ int bt_grid_build(bt_context* h, const bt_pose* pose, const bt_integ_params* prm
     int rc;
     if ((rc = grid_create(c, pose, prm, max_level, start_level, angle, &g))) return rc;
     if ((rc = grid_build(&c, g, gp))) {
-        hipStreamSynchronize(c.stream);
+        (void)hipStreamSynchronize(c.stream);
         grid_destroy(g);
         return rc;
     }
@@ -92,7 +92,7 @@ int bt_grid_build_batch(bt_context* h, int n, const bt_pose* poses, const bt_int
     StatsK s{};
     if (!rc) rc = grid_build_batch(&c, gs, gp, &s);
     if (rc) {
-        hipStreamSynchronize(c.stream);
+        (void)hipStreamSynchronize(c.stream);
         for (bt_grid* g : gs) grid_destroy(g);
         return rc;
     }
@@ -179,8 +179,8 @@ void bt_grid_free(bt_grid* g) {
     if (!g) return;
     if (g->ctx) {
         std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
-        hipSetDevice(g->ctx->device);
-        hipStreamSynchronize(g->ctx->stream);
+        (void)hipSetDevice(g->ctx->device);
+        (void)hipStreamSynchronize(g->ctx->stream);
         grid_release(g);
     }
     delete g;

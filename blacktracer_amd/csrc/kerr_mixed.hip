@@ -330,23 +330,24 @@ int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, Sta
     rc = trace_f64_queue(c, S, prm, n < crit_lanes ? n : crit_lanes, Qc, d_stats, "rk_f64_critical",
                          nullptr, c->aux, 1, true);
     // join (also on error, so that nothing is left running on the aux stream)
-    auto join = [&]() {
-        hipEventRecord(c->ev_join, c->aux);
-        hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    auto join = [&]() -> hipError_t {
+        const hipError_t e = hipEventRecord(c->ev_join, c->aux);
+        const hipError_t w = hipStreamWaitEvent(c->stream, c->ev_join, 0);
+        return e != hipSuccess ? e : w;
     };
     if (rc) {
-        join();
+        (void)join();
         return rc;
     }
     if ((rc = trace_f64_queue(c, S, prm, n, Qw, d_stats, "rk_f64_warmup", &cont))) {
-        join();
+        (void)join();
         return rc;
     }
     {
         KTimer t(c, "rk_mixed");
         Launch L;
         if ((rc = persistent_launch(c, rk_mixed<Src>, n, &L))) {
-            join();
+            (void)join();
             return rc;
         }
         // rays that near the attempt cap restart in FP64, so that a capped
@@ -356,13 +357,15 @@ int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, Sta
         rk_mixed<Src><<<L.blocks, L.threads, 0, c->stream>>>(S, prm, Qw, head32, Qr, cont, d_stats,
                                                              restart_at, c->tune.chunk);
         if (hipGetLastError() != hipSuccess) {
-            join();
+            (void)join();
             return fail(BT_ERR_HIP, "rk_mixed launch failed");
         }
     }
     rc = trace_f64_queue(c, S, prm, n, Qr, d_stats, "rk_f64_fallback", nullptr);
-    join();
-    return rc;
+    const hipError_t je = join();
+    if (rc) return rc;
+    BT_HIP(je);
+    return BT_OK;
 }
 
 template int trace_mixed_src<MultiSrc>(Context*, const MultiSrc&, const ParamsK&, int64_t, StatsK*);

@@ -2226,9 +2226,10 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         env_map_heavy<<<(unsigned)c->num_cus * 8, 256, 0, es>>>(thphi, s, cam, W, f.offset, f.lr, area, v.view,
                                                                img_map, qwave, nq + 1);
     };
-    auto join = [&]() {
-        hipEventRecord(c->ev_join, c->aux);
-        hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    auto join = [&]() -> hipError_t {
+        const hipError_t e = hipEventRecord(c->ev_join, c->aux);
+        const hipError_t w = hipStreamWaitEvent(c->stream, c->ev_join, 0);
+        return e != hipSuccess ? e : w;
     };
     if (overlap) {
         BT_HIP(hipEventRecord(c->ev_fork, c->stream));
@@ -2300,12 +2301,12 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         if (overlap) {                                                                                      \
             /* the odd pixels' chunks (few, long, latency-bound) on the auxiliary stream, between */       \
             /* the sky pass's light and heavy kernels, beside the other chunks */                           \
-            hipEventRecord(c->ev_hand, c->stream);                                                          \
+            BT_HIP(hipEventRecord(c->ev_hand, c->stream));                                                  \
             sky_light(c->aux);                                                                              \
-            hipStreamWaitEvent(c->aux, c->ev_hand, 0);                                                      \
+            BT_HIP(hipStreamWaitEvent(c->aux, c->ev_hand, 0));                                              \
             star_map_chunk<S, true><<<nred, 256, 0, c->aux>>>(thphi, bh, s, cam, W, H, f.offset, f.lr,      \
                                                               area, v.view, taps, Q);                       \
-            hipEventRecord(c->ev_back, c->aux);                                                             \
+            BT_HIP(hipEventRecord(c->ev_back, c->aux));                                                     \
             sky_heavy(c->aux);                                                                              \
         }                                                                                                   \
         {                                                                                                   \
@@ -2316,7 +2317,7 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
                 star_map_chunk<S, true><<<nred, 256, 0, c->stream>>>(thphi, bh, s, cam, W, H, f.offset,     \
                                                                      f.lr, area, v.view, taps, Q);          \
         }                                                                                                   \
-        if (overlap) hipStreamWaitEvent(c->stream, c->ev_back, 0);                                          \
+        if (overlap) BT_HIP(hipStreamWaitEvent(c->stream, c->ev_back, 0));                                  \
         KTimer t3(c, "star_reduce");                                                                        \
         star_chunk_reduce<S><<<nwave, 256, 0, c->stream>>>(W, H, taps, Q);                                  \
     }
@@ -2364,12 +2365,12 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         return BT_OK;
       }();
       if (src) {
-          if (overlap) join();
+          if (overlap) (void)join();
           return src;
       }
     }
     if (overlap) {
-        join();
+        BT_HIP(join());
     } else if (f.map) {
         sky_light(c->stream);
         sky_heavy(c->stream);

@@ -158,14 +158,14 @@ void cam7_of(const PoseK& P, float* cam7) {
 using namespace bt;
 
 static void scene_release(bt_scene* s) {
-    if (s->tree) hipFree(s->tree);
-    if (s->pos) hipFree(s->pos);
-    if (s->mag) hipFree(s->mag);
-    if (s->table) hipFree(s->table);
-    if (s->bin_start) hipFree(s->bin_start);
-    if (s->bin_pos) hipFree(s->bin_pos);
-    if (s->bin_mag) hipFree(s->bin_mag);
-    if (s->bin_idx) hipFree(s->bin_idx);
+    if (s->tree) (void)hipFree(s->tree);
+    if (s->pos) (void)hipFree(s->pos);
+    if (s->mag) (void)hipFree(s->mag);
+    if (s->table) (void)hipFree(s->table);
+    if (s->bin_start) (void)hipFree(s->bin_start);
+    if (s->bin_pos) (void)hipFree(s->bin_pos);
+    if (s->bin_mag) (void)hipFree(s->bin_mag);
+    if (s->bin_idx) (void)hipFree(s->bin_idx);
     s->bin_idx = nullptr;
     s->tree = nullptr;
     s->bin_start = nullptr;
@@ -257,7 +257,7 @@ int bt_scene_create(bt_context* h, const float* stars4, int star_count, int tree
     s->ctx = &c;
     rc = scene_upload(c, s, tree.data(), tree_level, pos.data(), mag.data(), star_count, sky_bgr, sky_rows, sky_cols);
     if (rc) {
-        hipStreamSynchronize(c.stream);
+        (void)hipStreamSynchronize(c.stream);
         scene_release(s);
         delete s;
         return rc;
@@ -283,7 +283,7 @@ int bt_scene_create_tree(bt_context* h, const int32_t* tree, int tree_level, con
     s->ctx = &c;
     int rc = scene_upload(c, s, tree, tree_level, pos, mag, star_count, sky_bgr, sky_rows, sky_cols);
     if (rc) {
-        hipStreamSynchronize(c.stream);
+        (void)hipStreamSynchronize(c.stream);
         scene_release(s);
         delete s;
         return rc;
@@ -307,8 +307,8 @@ void bt_scene_free(bt_scene* s) {
     if (!s) return;
     if (s->ctx) {
         std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
-        hipSetDevice(s->ctx->device);
-        hipStreamSynchronize(s->ctx->stream);
+        (void)hipSetDevice(s->ctx->device);
+        (void)hipStreamSynchronize(s->ctx->stream);
         scene_release(s);
     }
     delete s;
@@ -343,7 +343,7 @@ int bt_view_create(bt_context* h, double view_angle, double off_up, double off_r
         e = rc ? hipErrorUnknown : hipStreamSynchronize(c.stream);
     }
     if (e != hipSuccess) {
-        hipFree(v->view);
+        (void)hipFree(v->view);
         delete v;
         return rc ? rc : fail(BT_ERR_HIP, std::string("bt_view_create: ") + hipGetErrorString(e));
     }
@@ -355,9 +355,9 @@ void bt_view_free(bt_view* v) {
     if (!v) return;
     if (v->ctx) {
         std::lock_guard<std::recursive_mutex> lk(v->ctx->mu);
-        hipSetDevice(v->ctx->device);
-        hipStreamSynchronize(v->ctx->stream);
-        if (v->view) hipFree(v->view);
+        (void)hipSetDevice(v->ctx->device);
+        (void)hipStreamSynchronize(v->ctx->stream);
+        if (v->view) (void)hipFree(v->view);
     }
     delete v;
 }
@@ -380,7 +380,7 @@ int grid_f32_of(Context& c, const bt_grid* g, GridF* out) {
         int rc = render_grid_from_cel(&c, g->k.cel, g->N, g->M, p, &gn1);
         if (rc == BT_OK && hipStreamSynchronize(c.stream) != hipSuccess) rc = fail(BT_ERR_HIP, "grid conversion failed");
         if (rc) {
-            hipFree(p);
+            (void)hipFree(p);
             return rc;
         }
         g->gridf = p;
@@ -590,7 +590,7 @@ int bt_trails_create(bt_context* h, const bt_scene* s, int width, int height, co
         hipError_t e = hipMemcpyAsync(t->sprite, diffraction_rgb, (size_t)3 * fs * fs, hipMemcpyHostToDevice, c.stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
         if (e != hipSuccess) {
-            hipFree(base);
+            (void)hipFree(base);
             delete t;
             return fail(BT_ERR_HIP, std::string("bt_trails_create: ") + hipGetErrorString(e));
         }
@@ -619,9 +619,9 @@ void bt_trails_free(bt_trails* t) {
     if (!t) return;
     if (t->ctx) {
         std::lock_guard<std::recursive_mutex> lk(t->ctx->mu);
-        hipSetDevice(t->ctx->device);
-        hipStreamSynchronize(t->ctx->stream);
-        if (t->block) hipFree(t->block);
+        (void)hipSetDevice(t->ctx->device);
+        (void)hipStreamSynchronize(t->ctx->stream);
+        if (t->block) (void)hipFree(t->block);
     }
     delete t;
 }
@@ -664,7 +664,7 @@ int bt_make_image(bt_context* h, const float* stars, const int32_t* star_tree_in
     scene.ctx = &c;
     if ((rc = scene_upload(c, &scene, star_tree_in, tree_level, stars, magnitude, star_size, sky_bgr, sky_rows,
                            sky_cols))) {
-        hipStreamSynchronize(c.stream);
+        (void)hipStreamSynchronize(c.stream);
         scene_release(&scene);
         return rc;
     }
@@ -678,7 +678,7 @@ int bt_make_image(bt_context* h, const float* stars, const int32_t* star_tree_in
         (rc = c.buffer("mi_tag", sizeof(int32_t) * 2 * (size_t)htsize, &dtag)) ||
         (rc = c.buffer("mi_grid", sizeof(float) * 2 * (size_t)GN1 * GM, &dgrid)) ||
         (rc = c.buffer("mi_view", sizeof(float) * 2 * NC, &dview)) || (rc = c.buffer("mi_out", 4 * NP, &dout))) {
-        hipStreamSynchronize(c.stream);
+        (void)hipStreamSynchronize(c.stream);
         scene_release(&scene);
         return rc;
     }
@@ -704,7 +704,7 @@ int bt_make_image(bt_context* h, const float* stars, const int32_t* star_tree_in
         if (e != hipSuccess) rc = fail(BT_ERR_HIP, std::string("bt_make_image: ") + hipGetErrorString(e));
     }
 done:
-    hipStreamSynchronize(c.stream);
+    (void)hipStreamSynchronize(c.stream);
     scene_release(&scene);
     return rc;
 }

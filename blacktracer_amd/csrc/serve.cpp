@@ -163,7 +163,7 @@ bool make_room(bt_server* s, size_t need) {
 }
 
 void builder_main(bt_server* s) {
-    hipSetDevice(s->owner->impl.device);
+    (void)hipSetDevice(s->owner->impl.device);
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
         s->cv_build.wait(lk, [&] { return s->stop || !s->build_grids.empty() || !s->build_views.empty(); });
@@ -246,7 +246,7 @@ int launch(bt_server* s, Slot& sl, CachedGrid& cg, const bt_view* v) {
     const bt_request& r = sl.req.r;
     const size_t nbytes = (size_t)4 * r.width * r.height;
     if (sl.staging_bytes < nbytes) {
-        if (sl.staging) hipHostFree(sl.staging);
+        if (sl.staging) (void)hipHostFree(sl.staging);
         sl.staging = nullptr;
         sl.staging_bytes = 0;
         if (hipHostMalloc((void**)&sl.staging, nbytes, 0) != hipSuccess)
@@ -275,7 +275,7 @@ int launch(bt_server* s, Slot& sl, CachedGrid& cg, const bt_view* v) {
 }
 
 void dispatcher_main(bt_server* s) {
-    hipSetDevice(s->owner->impl.device);
+    (void)hipSetDevice(s->owner->impl.device);
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
         bool progress = false;
@@ -431,7 +431,7 @@ int bt_server_create(bt_context* h, const bt_scene* scene, const bt_server_confi
             hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
             if (!rc) rc = fail(BT_ERR_HIP, "bt_server_create: event creation failed");
             for (Slot& t : s->slots) {
-                if (t.done) hipEventDestroy(t.done);
+                if (t.done) (void)hipEventDestroy(t.done);
                 if (t.ctx) bt_context_destroy(t.ctx);
             }
             bt_context_destroy(s->builder);
@@ -533,9 +533,9 @@ void bt_server_destroy(bt_server* s) {
     if (s->dispatcher.joinable()) s->dispatcher.join();
     if (s->builder_thread.joinable()) s->builder_thread.join();
     for (Slot& sl : s->slots) {
-        if (sl.ctx) hipStreamSynchronize(sl.ctx->impl.stream);
-        if (sl.done) hipEventDestroy(sl.done);
-        if (sl.staging) hipHostFree(sl.staging);
+        if (sl.ctx) (void)hipStreamSynchronize(sl.ctx->impl.stream);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.staging) (void)hipHostFree(sl.staging);
         if (sl.ctx) bt_context_destroy(sl.ctx);
     }
     for (auto& kv : s->grids)

@@ -92,7 +92,7 @@ int bt_autotune(bt_context* h, const bt_pose* pose, const bt_integ_params* prm, 
     hipEvent_t e0, e1;
     BT_HIP(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
-        hipEventDestroy(e0);
+        (void)hipEventDestroy(e0);
         return fail(BT_ERR_HIP, "bt_autotune: event creation failed");
     }
     const Tune saved = c.tune;
@@ -146,8 +146,8 @@ int bt_autotune(bt_context* h, const bt_pose* pose, const bt_integ_params* prm, 
             }
         }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc) {
         c.tune = saved;
         return rc;
