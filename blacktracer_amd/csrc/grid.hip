@@ -174,21 +174,27 @@ __device__ __forceinline__ const SetupJob& setup_job(const SetupJob* jobs, int n
     return jobs[lo];
 }
 
-__global__ void init_grid_multi(const SetupJob* jobs, int njobs, int64_t items) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= items) return;
-    const SetupJob& J = setup_job<0>(jobs, njobs, t);
-    const int64_t i = t - J.init0;
-    if (i >= J.nm) return;
+// A job's init items start on a multiple of kInitSpan, so a workgroup of
+// 256 threads covers kInitSpan items of one grid: the job is found once per
+// workgroup (a uniform search) and each thread fills kInitSpan / 256 points.
+constexpr int kInitSpan = 4096;
+
+__global__ __launch_bounds__(256) void init_grid_multi(const SetupJob* jobs, int njobs) {
+    const int64_t t0 = (int64_t)blockIdx.x * kInitSpan;
+    const SetupJob& J = setup_job<0>(jobs, njobs, t0);
     const GridK& G = J.G;
-    G.cel[2 * i] = -2.0;
-    G.cel[2 * i + 1] = -2.0;
-    G.steps[i] = 0;
-    G.level[i] = -1;
-    if (i < J.nwords) G.present[i] = 0u;
-    if (i == 0) {
-        J.keys[0] = 0u;
-        J.keys[1] = (uint32_t)(G.N - 1) * (uint32_t)G.M;
+    const int64_t base = t0 - J.init0;
+    for (int k = 0; k < kInitSpan / 256; k++) {
+        const int64_t i = base + k * 256 + threadIdx.x;
+        if (i >= J.nm) break;
+        reinterpret_cast<double2*>(G.cel)[i] = make_double2(-2.0, -2.0);   // cel is 256-byte aligned
+        G.steps[i] = 0;
+        G.level[i] = -1;
+        if (i < J.nwords) G.present[i] = 0u;
+        if (i == 0) {
+            J.keys[0] = 0u;
+            J.keys[1] = (uint32_t)(G.N - 1) * (uint32_t)G.M;
+        }
     }
 }
 
@@ -854,7 +860,7 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
         J.init0 = n_init;
         J.lat0 = n_lat;
         J.pole0 = n_pole;
-        n_init += NM;
+        n_init += (NM + kInitSpan - 1) / kInitSpan * kInitSpan;
         n_lat += (nlat + 63) / 64 * 64;
         n_pole += npole;
         counts[k] = 1 + eq + nlat;
@@ -887,7 +893,7 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
     const SetupJob* d_sj = (const SetupJob*)p;
     BT_HIP(hipMemcpyAsync(p, hs, sj_bytes, hipMemcpyHostToDevice, c->stream));
     if ((rc = c->stage_done())) return rc;
-    init_grid_multi<<<blocks_for(n_init, 256), 256, 0, c->stream>>>(d_sj, G, n_init);
+    init_grid_multi<<<(unsigned)(n_init / kInitSpan), 256, 0, c->stream>>>(d_sj, G);
     BT_HIP(hipGetLastError());
     start_lattice_multi<<<blocks_for(n_lat, 256), 256, 0, c->stream>>>(d_sj, G, n_lat);
     BT_HIP(hipGetLastError());
