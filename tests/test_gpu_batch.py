@@ -95,6 +95,28 @@ def test_batch_half_sky_and_moving_cameras(ctx, angle, prec):
         g.close()
 
 
+@pytest.mark.parametrize("angle", [True, False])
+def test_batch_tiny_grids(ctx, angle):
+    """Grids smaller than one workgroup's share of the set-up launches
+    (levels 1-3, a few to a few hundred points) beside a larger one: each
+    equals its single build."""
+    specs = [(1, 1), (2, 1), (2, 2), (3, 1), (3, 3), (7, 1), (1, 1)]
+    bhs = [bt.BlackHole(0.9) for _ in specs]
+    cams = [bt.Camera(math.pi / 2 if not angle else 1.0 + 0.1 * k, 0.0, 4.0 + k, black_hole=bh)
+            for k, bh in enumerate(bhs)]
+    grids, st = bt.Grid.build_batch(cams, bhs, [s[0] for s in specs], [s[1] for s in specs], angle=angle, ctx=ctx)
+    for k, (cam, bh, s) in enumerate(zip(cams, bhs, specs)):
+        print(f"tiny batch angle={angle} grid {k} Grid{s}", flush=True)
+        single = bt.Grid(s[0], s[1], angle, cam, bh, ctx=ctx)
+        g = grids[k]
+        assert (g.N, g.M) == (single.N, single.M)
+        assert g.rays_traced == single.rays_traced
+        assert np.array_equal(g.cel, single.cel)
+        assert np.array_equal(g.levels, single.levels)
+        single.close()
+        g.close()
+
+
 def test_batch_rejects_bad_sizes(ctx):
     cams, bhs = _grids(POSES[:1])
     with pytest.raises(bt.BtError):

@@ -29,6 +29,9 @@ def _cmp_cel(a, b, present):
     ((0.9, 0.3, 0.0, 20.0), 7, 2, True),
     ((0.999, math.pi / 2, 0.0, 5.0), 9, 1, False),    # half sky (camera in the
     ((0.5, math.pi / 2, 0.0, 12.0), 8, 8, False),     # equatorial plane)
+    ((0.9, 1.0, 0.0, 4.0), 1, 1, True),               # the smallest grids
+    ((0.9, math.pi / 2, 0.0, 4.0), 2, 1, False),
+    ((0.9, 1.2, 0.0, 4.0), 3, 3, True),
 ])
 def test_grid_matches_oracle(ctx, pose, lv, start, angle):
     print(f"grid pose={pose} Grid({lv},{start},angle={angle})", flush=True)
