@@ -429,7 +429,10 @@ struct Edge {
 };
 
 // checkAdjacentBlock (Grid.h:132-177): the reference's pre-order recursion
-// run from an explicit stack.
+// run from an explicit stack.  GROUP issues all loads of a node together
+// (the line walkers, whose nodes are mostly present); without it the node's
+// presence is tested first (the pole walker, which meets many absent nodes).
+template <bool GROUP>
 __device__ void check_adjacent(const GridK& G, Edge root, int udlr) {
     const uint32_t M = (uint32_t)G.M, N = (uint32_t)G.N;
     double* cel = G.cel;
@@ -440,7 +443,6 @@ __device__ void check_adjacent(const GridK& G, Edge root, int udlr) {
         Edge f = st[--sp];
         uint32_t i = f.ii + udlr * f.gap / 2;
         uint32_t j = f.jj + (1 - udlr) * f.gap / 2;
-        if (!present_at(G, i, j)) continue;
         uint32_t jprev = (f.jj - (1 - udlr) * f.gap + M) % M;
         uint32_t jnext = (f.jj + (1 - udlr) * 2 * f.gap) % M;
         uint32_t iprev = f.ii - udlr * f.gap;
@@ -456,30 +458,36 @@ __device__ void check_adjacent(const GridK& G, Edge root, int udlr) {
             else
                 half = true;
         }
-        const uint32_t mid = i * M + j;
-        bool succes = false;
-        if (present_at(G, iprev, jprev) && present_at(G, inext, jnext)) {
-            uint32_t ip = iprev * M + jprev, in = inext * M + jnext;
-            uint32_t i1 = f.ii * M + f.jj, i2 = f.ii2 * M + f.jj2;
-            double t0 = cel[2 * ip], p0 = cel[2 * ip + 1];
-            double t1 = cel[2 * i1], p1 = cel[2 * i1 + 1];
-            double t2 = cel[2 * i2], p2 = cel[2 * i2 + 1];
-            double t3 = cel[2 * in], p3 = cel[2 * in + 1];
-            if ((t0 != -1 || p0 != -1) && (t3 != -1 || p3 != -1)) {
-                succes = true;
-                if (half) t3 = PI - t3;
-                double ph[4] = {p0, p1, p2, p3};
-                fix_2pi(ph, 4);
-                cel[2 * mid] = hermite1(0.5, t0, t1, t2, t3);
-                cel[2 * mid + 1] = hermite1(0.5, ph[0], ph[1], ph[2], ph[3]);
-            }
-        }
-        if (!succes) {
-            uint32_t i1 = f.ii * M + f.jj, i2 = f.ii2 * M + f.jj2;
-            double t0 = cel[2 * i1], t1 = cel[2 * i2];
-            double ph[2] = {cel[2 * i1 + 1], cel[2 * i2 + 1]};
+        if (!GROUP && !present_at(G, i, j)) continue;
+        // every load of the node at once (out-of-grid neighbours read the
+        // edge's own first point instead and are masked out), so that a
+        // walker pays one memory round trip per node
+        const uint32_t i1 = f.ii * M + f.jj, i2 = f.ii2 * M + f.jj2;
+        const bool in_m = i < N && j < M, in_p = iprev < N && jprev < M, in_n = inext < N && jnext < M;
+        const uint32_t mid = in_m ? i * M + j : i1;
+        const uint32_t ip = in_p ? iprev * M + jprev : i1;
+        const uint32_t in = in_n ? inext * M + jnext : i1;
+        const uint32_t wm = G.present[mid >> 5], wp = G.present[ip >> 5], wn = G.present[in >> 5];
+        const double t0 = cel[2 * ip], p0 = cel[2 * ip + 1];
+        const double t1 = cel[2 * i1], p1 = cel[2 * i1 + 1];
+        const double t2 = cel[2 * i2], p2 = cel[2 * i2 + 1];
+        const double t3 = cel[2 * in], p3 = cel[2 * in + 1];
+        // an empty statement that uses every loaded value keeps the compiler
+        // from sinking the loads below the presence test
+        if (GROUP)
+            __asm__ volatile("" : : "v"(wm), "v"(wp), "v"(wn), "v"(t0), "v"(p0), "v"(t1), "v"(p1), "v"(t2),
+                             "v"(p2), "v"(t3), "v"(p3));
+        if (!(in_m && ((wm >> (mid & 31)) & 1u))) continue;
+        const bool pp = in_p && ((wp >> (ip & 31)) & 1u), pn = in_n && ((wn >> (in & 31)) & 1u);
+        if (pp && pn && (t0 != -1 || p0 != -1) && (t3 != -1 || p3 != -1)) {
+            double ph[4] = {p0, p1, p2, p3};
+            fix_2pi(ph, 4);
+            cel[2 * mid] = hermite1(0.5, t0, t1, t2, half ? PI - t3 : t3);
+            cel[2 * mid + 1] = hermite1(0.5, ph[0], ph[1], ph[2], ph[3]);
+        } else {
+            double ph[2] = {p1, p2};
             fix_2pi(ph, 2);
-            cel[2 * mid] = 1. / 2. * (t1 + t0);
+            cel[2 * mid] = 1. / 2. * (t2 + t1);
             cel[2 * mid + 1] = 1. / 2. * (ph[1] + ph[0]);
         }
         if (f.level + 1 == G.max_level) continue;
@@ -518,9 +526,9 @@ __device__ __forceinline__ void fix_pole(const GridK& G, int L) {
             m &= m - 1;
             const uint32_t jj = j0 + (uint32_t)src * ug, l = (jj + ug) % M;
             if (lane == 0) {
-                check_adjacent(G, Edge{0, jj, ug, jj, L, gap}, 1);
-                check_adjacent(G, Edge{0, jj, 0, l, L, gap}, 0);
-                check_adjacent(G, Edge{0, l, ug, l, L, gap}, 1);
+                check_adjacent<false>(G, Edge{0, jj, ug, jj, L, gap}, 1);
+                check_adjacent<false>(G, Edge{0, jj, 0, l, L, gap}, 0);
+                check_adjacent<false>(G, Edge{0, l, ug, l, L, gap}, 1);
             }
         }
     }
@@ -548,7 +556,7 @@ __device__ __forceinline__ void fix_line(const GridK& G, int L, unsigned nrows, 
                     const int src = __ffsll((long long)m) - 1;
                     m &= m - 1;
                     const uint32_t jj = j0 + (uint32_t)src * ug;
-                    if (lane == 0) check_adjacent(G, Edge{y, jj, y, (jj + ug) % M, L, gap}, 0);
+                    if (lane == 0) check_adjacent<true>(G, Edge{y, jj, y, (jj + ug) % M, L, gap}, 0);
                 }
             }
         }
@@ -581,7 +589,7 @@ __device__ __forceinline__ void fix_line(const GridK& G, int L, unsigned nrows, 
                 m &= m - 1;
                 const uint32_t ii = i0 + (uint32_t)(src >> 2) * ug;
                 const uint32_t cc = __shfl(col, src);
-                if (lane == 0) check_adjacent(G, Edge{ii, cc, ii + ug, cc, L, gap}, 1);
+                if (lane == 0) check_adjacent<true>(G, Edge{ii, cc, ii + ug, cc, L, gap}, 1);
             }
         }
     }
