@@ -689,6 +689,89 @@ static float redshift(float theta, float phi, const float* cam) {
 }
 float btor_redshift(float theta, float phi, const float* cam) { return redshift(theta, phi, cam); }
 
+/* smoothAreaH then smoothAreaV (MakeImageKernel.cu:570-616): a box mean of
+ * radius floor(max corner gap / 2) over the non-shadow pixels of the row
+ * (wrapping in j), then of the column (clamped in i).  area (W x H) is
+ * replaced by the result; tmp holds the horizontal pass. */
+static void smooth_area(float* area, const uint8_t* bh, const int32_t* gap, int W, int H, float* tmp) {
+    const int M = W, N = H, M1 = W + 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < M; j++) {
+            size_t ij = (size_t)i * M + j;
+            tmp[ij] = 0.f;
+            if (bh[ij] == 1) continue;
+            int fs = gap[i * M1 + j];
+            if (gap[i * M1 + M1 + j] > fs) fs = gap[i * M1 + M1 + j];
+            if (gap[i * M1 + j + 1] > fs) fs = gap[i * M1 + j + 1];
+            if (gap[i * M1 + M1 + j + 1] > fs) fs = gap[i * M1 + M1 + j + 1];
+            fs = fs / 2;
+            float sum = 0.f;
+            int count = 0;
+            for (int h = -fs; h <= fs; h++) {
+                int jj = ((j + h) % M + M) % M;
+                if (bh[(size_t)i * M + jj] == 0) {
+                    sum += area[(size_t)i * M + jj];
+                    count++;
+                }
+            }
+            tmp[ij] = sum / (1.f * count);
+        }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < M; j++) {
+            size_t ij = (size_t)i * M + j;
+            if (bh[ij] == 1) continue;
+            int fs = gap[i * M1 + j];
+            if (gap[i * M1 + M1 + j] > fs) fs = gap[i * M1 + M1 + j];
+            if (gap[i * M1 + j + 1] > fs) fs = gap[i * M1 + j + 1];
+            if (gap[i * M1 + M1 + j + 1] > fs) fs = gap[i * M1 + M1 + j + 1];
+            fs = fs / 2;
+            float sum = 0.f;
+            int minv = i - fs > 0 ? i - fs : 0;
+            int maxv = i + fs < N - 1 ? i + fs : N - 1;
+            int count = 0;
+            for (int h = minv; h <= maxv; h++)
+                if (bh[(size_t)h * M + j] == 0) {
+                    sum += tmp[(size_t)h * M + j];
+                    count++;
+                }
+            area[ij] = sum / (1.f * count);
+        }
+}
+void btor_smooth_area(float* area, const uint8_t* bh, const int32_t* gap, int W, int H) {
+    float* tmp = (float*)malloc(sizeof(float) * (size_t)W * H);
+    smooth_area(area, bh, gap, W, H, tmp);
+    free(tmp);
+}
+
+/* makePix (MakeImageKernel.cu:802-835): summed squared light -> BGRA8.
+ * rgb = sqrt(light); if a channel exceeds 255 the colour is renormalised
+ * through HSP with P = 1. */
+static void make_pix(float br, float bg, float bb, uint8_t* o) {
+    float rgb[3] = {sqrtf(br), sqrtf(bg), sqrtf(bb)};
+    float mxc = fmaxf(fmaxf(rgb[0], rgb[1]), rgb[2]);
+    if (mxc > 255.f) {
+        float hh, ss, pp;
+        rgb_to_hsp(rgb[0] / 255.f, rgb[1] / 255.f, rgb[2] / 255.f, &hh, &ss, &pp);
+        pp = 1.0f;
+        hsp_to_rgb(hh, ss, pp, &rgb[0], &rgb[1], &rgb[2]);
+        rgb[0] *= 255;
+        rgb[1] *= 255;
+        rgb[2] *= 255;
+    }
+    int cb = cvt_i32(rgb[2] + 0.f), cg = cvt_i32(rgb[1] + 0.f), cr = cvt_i32(rgb[0] + 0.f);
+    o[0] = (uint8_t)(cb < 255 ? cb : 255);
+    o[1] = (uint8_t)(cg < 255 ? cg : 255);
+    o[2] = (uint8_t)(cr < 255 ? cr : 255);
+    o[3] = 255;
+}
+void btor_make_pix(float r, float g, float b, uint8_t* bgra) { make_pix(r, g, b, bgra); }
+
 /* ------------------------------------------------------------ frame stages */
 
 typedef struct {
@@ -815,6 +898,26 @@ static float gaussian(float dist) {
     float sigma = 1.f / 2.5f;
     return f_exp(-.5f * dist / (sigma * sigma)) * 1.f / (sigma * SQRT2PI);
 }
+
+float btor_gaussian(float dist) { return gaussian(dist); }
+void btor_interpolate_star(const float* t, const float* p, float st, float sp, int sgn, float* yx) {
+    interpolate_star(t[0], t[1], t[2], t[3], p[0], p[1], p[2], p[3], &st, &sp, sgn);
+    yx[0] = st;
+    yx[1] = sp;
+}
+int btor_pi_check(float* p, float factor) { return pi_check(p, factor); }
+void btor_hermite(float v, const float* x4, float* out) {
+    f2 x0 = {x4[0], x4[1]}, x1 = {x4[2], x4[3]}, x2 = {x4[4], x4[5]}, x3 = {x4[6], x4[7]};
+    f2 r = hermite_f(v, x0, x1, x2, x3);
+    out[0] = r.x;
+    out[1] = r.y;
+}
+void btor_intersection(const float* abcd, float* out) {
+    f2 r = intersection(abcd[0], abcd[1], abcd[2], abcd[3], abcd[4], abcd[5], abcd[6], abcd[7]);
+    out[0] = r.x;
+    out[1] = r.y;
+}
+void btor_wrap_to_pi(float* t, float* p) { wrap_to_pi_f(t, p); }
 
 /* MakeImageKernel.cu:27-96, searching the implicit star tree for leaves
  * (or nearly covered inner nodes) that overlap the pixel's bounding box.
@@ -1316,53 +1419,7 @@ static int frame_impl(const float* grid, const float* grid_b, float alpha, const
         }
 
     /* smoothAreaH / smoothAreaV (MakeImageKernel.cu:570-616) */
-#ifdef _OPENMP
-#pragma omp parallel for schedule(static)
-#endif
-    for (int i = 0; i < N; i++)
-        for (int j = 0; j < M; j++) {
-            size_t ij = (size_t)i * M + j;
-            areaSmooth[ij] = 0.f;
-            if (bh[ij] == 1) continue;
-            int fs = gap[i * M1 + j];
-            if (gap[i * M1 + M1 + j] > fs) fs = gap[i * M1 + M1 + j];
-            if (gap[i * M1 + j + 1] > fs) fs = gap[i * M1 + j + 1];
-            if (gap[i * M1 + M1 + j + 1] > fs) fs = gap[i * M1 + M1 + j + 1];
-            fs = fs / 2;
-            float sum = 0.f;
-            int count = 0;
-            for (int h = -fs; h <= fs; h++) {
-                int jj = ((j + h) % M + M) % M;
-                if (bh[(size_t)i * M + jj] == 0) {
-                    sum += area[(size_t)i * M + jj];
-                    count++;
-                }
-            }
-            areaSmooth[ij] = sum / (1.f * count);
-        }
-#ifdef _OPENMP
-#pragma omp parallel for schedule(static)
-#endif
-    for (int i = 0; i < N; i++)
-        for (int j = 0; j < M; j++) {
-            size_t ij = (size_t)i * M + j;
-            if (bh[ij] == 1) continue;
-            int fs = gap[i * M1 + j];
-            if (gap[i * M1 + M1 + j] > fs) fs = gap[i * M1 + M1 + j];
-            if (gap[i * M1 + j + 1] > fs) fs = gap[i * M1 + j + 1];
-            if (gap[i * M1 + M1 + j + 1] > fs) fs = gap[i * M1 + M1 + j + 1];
-            fs = fs / 2;
-            float sum = 0.f;
-            int minv = i - fs > 0 ? i - fs : 0;
-            int maxv = i + fs < N - 1 ? i + fs : N - 1;
-            int count = 0;
-            for (int h = minv; h <= maxv; h++)
-                if (bh[(size_t)h * M + j] == 0) {
-                    sum += areaSmooth[(size_t)h * M + j];
-                    count++;
-                }
-            area[ij] = sum / (1.f * count);
-        }
+    smooth_area(area, bh, gap, W, H, areaSmooth);
 
     uint8_t* img_star = NULL;
     uint8_t* img_map = NULL;
@@ -1577,23 +1634,7 @@ static int frame_impl(const float* grid, const float* grid_b, float alpha, const
                     o_star[3 * ij + 1] = bg;
                     o_star[3 * ij + 2] = bb;
                 }
-                float rgb[3] = {sqrtf(br), sqrtf(bg), sqrtf(bb)};
-                float mxc = fmaxf(fmaxf(rgb[0], rgb[1]), rgb[2]);
-                if (mxc > 255.f) {
-                    float hh, ss, pp;
-                    rgb_to_hsp(rgb[0] / 255.f, rgb[1] / 255.f, rgb[2] / 255.f, &hh, &ss, &pp);
-                    pp = 1.0f;
-                    hsp_to_rgb(hh, ss, pp, &rgb[0], &rgb[1], &rgb[2]);
-                    rgb[0] *= 255;
-                    rgb[1] *= 255;
-                    rgb[2] *= 255;
-                }
-                uint8_t* o = img_star + 4 * ((size_t)i * M + j);
-                int cb = cvt_i32(rgb[2] + 0.f), cg = cvt_i32(rgb[1] + 0.f), cr = cvt_i32(rgb[0] + 0.f);
-                o[0] = (uint8_t)(cb < 255 ? cb : 255);
-                o[1] = (uint8_t)(cg < 255 ? cg : 255);
-                o[2] = (uint8_t)(cr < 255 ? cr : 255);
-                o[3] = 255;
+                make_pix(br, bg, bb, img_star + 4 * ((size_t)i * M + j));
             }
         free(temp);
         if (light) {
@@ -1627,23 +1668,7 @@ static int frame_impl(const float* grid, const float* grid_b, float alpha, const
                 }
             for (size_t ij = 0; ij < NP; ij++) {
                 if (o_star) memcpy(o_star + 3 * ij, acc + 3 * ij, sizeof(float) * 3);
-                float rgb[3] = {sqrtf(acc[3 * ij]), sqrtf(acc[3 * ij + 1]), sqrtf(acc[3 * ij + 2])};
-                float mxc = fmaxf(fmaxf(rgb[0], rgb[1]), rgb[2]);
-                if (mxc > 255.f) {
-                    float hh, ss, pp;
-                    rgb_to_hsp(rgb[0] / 255.f, rgb[1] / 255.f, rgb[2] / 255.f, &hh, &ss, &pp);
-                    pp = 1.0f;
-                    hsp_to_rgb(hh, ss, pp, &rgb[0], &rgb[1], &rgb[2]);
-                    rgb[0] *= 255;
-                    rgb[1] *= 255;
-                    rgb[2] *= 255;
-                }
-                uint8_t* o = img_star + 4 * ij;
-                int cb = cvt_i32(rgb[2] + 0.f), cg = cvt_i32(rgb[1] + 0.f), cr = cvt_i32(rgb[0] + 0.f);
-                o[0] = (uint8_t)(cb < 255 ? cb : 255);
-                o[1] = (uint8_t)(cg < 255 ? cg : 255);
-                o[2] = (uint8_t)(cr < 255 ? cr : 255);
-                o[3] = 255;
+                make_pix(acc[3 * ij], acc[3 * ij + 1], acc[3 * ij + 2], img_star + 4 * ij);
             }
             free(acc);
             free(light);

@@ -113,6 +113,15 @@ def lib():
         L.btor_redshift.restype = C.c_float
         L.btor_star_in_polygon.argtypes = [FP, FP, C.c_float, C.c_float, C.c_int]
         L.btor_search_tree.argtypes = [IP, FP, FP, C.c_int, IP, C.c_int]
+        L.btor_smooth_area.argtypes = [FP, U8P, IP, C.c_int, C.c_int]
+        L.btor_make_pix.argtypes = [C.c_float] * 3 + [U8P]
+        L.btor_gaussian.argtypes = [C.c_float]
+        L.btor_gaussian.restype = C.c_float
+        L.btor_interpolate_star.argtypes = [FP, FP, C.c_float, C.c_float, C.c_int, FP]
+        L.btor_pi_check.argtypes = [FP, C.c_float]
+        L.btor_hermite.argtypes = [C.c_float, FP, FP]
+        L.btor_intersection.argtypes = [FP, FP]
+        L.btor_wrap_to_pi.argtypes = [FP, FP]
         _LIB = L
     return _LIB
 
