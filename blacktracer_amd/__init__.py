@@ -364,6 +364,16 @@ def star_tree(stars4, tree_level):
     return tree, pos, mag
 
 
+def sky_add_low_stars(sky_bgr, stars4, magnitude_cut):
+    """addLowLightStarsToImage (StarProcessor.h:108-158): a copy of the BGR8
+    sky with every star fainter than magnitude_cut painted in."""
+    sky = np.ascontiguousarray(sky_bgr, dtype=np.uint8).copy()
+    s4 = _f32(stars4).reshape(-1, 4)
+    check(_lib.load().bt_sky_add_low_stars(sky.ctypes.data_as(_lib.U8P), sky.shape[0], sky.shape[1],
+                                           s4.ctypes.data_as(_lib.FP), s4.shape[0], float(magnitude_cut)))
+    return sky
+
+
 class Scene:
     """Device-resident stars (star tree, StarProcessor.h) and summed-area sky
     table (MakeImageKernel.cu:1312-1331)."""
@@ -440,8 +450,21 @@ class View:
 
 
 def frame_bytes(width, height, step=1):
-    """Device bytes one frame of this size works in (what the server reserves)."""
+    """Device bytes one frame of this size works in."""
     return int(_lib.load().bt_frame_bytes(int(width), int(height), int(step)))
+
+
+def server_footprint(width, height, max_level=10, step=1):
+    """What a server counts against its HBM budget: {slot, view, grid, build}
+    bytes for frames of width x height and grids of max_level."""
+    L = _lib.load()
+    cfg = _lib.BtServerConfig()
+    L.bt_server_config_default(C.byref(cfg))
+    cfg.max_level = max_level
+    cfg.step = step
+    b = _lib.BtServerBytes()
+    check(L.bt_server_footprint(C.byref(cfg), int(width), int(height), C.byref(b)))
+    return {n: int(getattr(b, n)) for n, _ in _lib.BtServerBytes._fields_}
 
 
 def render_frame_device(grid, scene, view, out_buffer, flags=None, cam7=None, step=1, ctx=None):

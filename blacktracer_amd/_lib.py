@@ -41,7 +41,8 @@ class BtIntegParams(C.Structure):
                 ("max_steps", C.c_int), ("apriori_capture", C.c_int),
                 ("r_escape", C.c_double), ("r_capture_factor", C.c_double),
                 ("newton_cap", C.c_int), ("precision", C.c_int),
-                ("crit_eps", C.c_double), ("hmin_fp64", C.c_double)]
+                ("crit_eps", C.c_double), ("hmin_fp64", C.c_double),
+                ("noise_budget", C.c_double), ("decision_margin", C.c_double)]
 
 
 class BtTraceStats(C.Structure):
@@ -62,6 +63,10 @@ class BtServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("slots", C.c_int), ("max_level", C.c_int),
                 ("start_level", C.c_int), ("integ", BtIntegParams), ("step", C.c_int),
                 ("view_angle", C.c_double), ("off_up", C.c_double), ("off_right", C.c_double)]
+
+
+class BtServerBytes(C.Structure):
+    _fields_ = [("slot", C.c_size_t), ("view", C.c_size_t), ("grid", C.c_size_t), ("build", C.c_size_t)]
 
 
 class BtRequest(C.Structure):
@@ -103,6 +108,7 @@ SIGNATURES = {
     "bt_autotune": (C.c_int, [C.c_void_p, C.POINTER(BtPose), C.POINTER(BtIntegParams), C.c_int64,
                               C.POINTER(BtTune), DP]),
     "bt_server_config_default": (None, [C.POINTER(BtServerConfig)]),
+    "bt_server_footprint": (C.c_int, [C.POINTER(BtServerConfig), C.c_int, C.c_int, C.POINTER(BtServerBytes)]),
     "bt_server_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BtServerConfig), C.POINTER(C.c_void_p)]),
     "bt_server_submit": (C.c_int, [C.c_void_p, C.POINTER(BtRequest)]),
     "bt_server_poll": (C.c_int, [C.c_void_p, C.POINTER(BtCompletion), C.c_int, C.POINTER(C.c_int)]),
@@ -147,6 +153,7 @@ SIGNATURES = {
                                 C.c_int, C.c_int, C.POINTER(BtRenderFlags), U8P]),
     "bt_viewer_matrix": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, FP]),
     "bt_star_tree": (C.c_int, [FP, C.c_int, C.c_int, I32P, FP, FP]),
+    "bt_sky_add_low_stars": (C.c_int, [U8P, C.c_int, C.c_int, FP, C.c_int, C.c_float]),
     "bt_scene_create": (C.c_int, [C.c_void_p, FP, C.c_int, C.c_int, U8P, C.c_int, C.c_int,
                                   C.POINTER(C.c_void_p)]),
     "bt_scene_export_table": (C.c_int, [C.c_void_p, FP]),

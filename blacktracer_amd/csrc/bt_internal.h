@@ -46,6 +46,7 @@ struct PoseK {
     double sin2S;      // sin(thetaS)*sin(thetaS)
     double b1, b2;     // _b0(_r2), _b0(_r1)                           (Metric.h:164-165)
     double r_ph1, r_ph2;  // prograde / retrograde equatorial photon-orbit radii (Metric.h:146-147)
+    double crit_slope;    // max |dq0/db| along the critical curve (inf: not a graph over b)
     double r_cap;      // capture radius (0: off)
     double gr;         // golden ratio (sqrt(5)+1)/2                   (Metric.h:111)
     int    a_zero;     // checkRup returns false when a == 0           (Metric.h:128)
@@ -57,6 +58,7 @@ struct ParamsK {
     int    max_steps, apriori, newton_cap, precision;
     double crit_eps, hmin_fp64;
     double h0;         // h1 * sgn((float)z_end)                      (Metric.h:287)
+    float  noise_budget, decision_margin;
 };
 
 // Device counters, one set per launch.
@@ -165,6 +167,7 @@ struct Context {
     std::multimap<size_t, void*> pool;
     size_t pool_bytes = 0;
     static constexpr size_t kPoolCap = size_t(32) << 30;
+    size_t pool_cap = kPoolCap;   // pooled bytes kept at most (0: every freed block is freed)
 
     // A block of at least `bytes` (reused when one of up to 2x that size is
     // pooled); *got receives its actual size.
@@ -175,6 +178,11 @@ struct Context {
 
     // Returns a device buffer of at least `bytes` for `name`.
     int buffer(const std::string& name, size_t bytes, void** out);
+    // Device bytes the named buffers hold now.
+    size_t buffer_bytes() const;
+    // Frees every named buffer (after the stream has drained); the next
+    // call that needs one allocates it again.
+    void release_buffers();
 
     // Pinned host staging for small tables copied to the device on the main
     // stream: stage() waits until the copies from the previous staging have
@@ -217,9 +225,11 @@ struct QueueK {
 // ray): y holds six arrays of `stride` doubles (r, theta, phi, pR, pTheta, z).
 struct ContK {
     double* y;
-    float* h;
+    double* h;
     int32_t* nstp;
     int64_t stride;
+    QueueK rq;          // continued rays that near the attempt cap restart here
+    int restart_at;     // ... once they have made this many attempts
 };
 
 // ---------------------------------------------------------------- launchers

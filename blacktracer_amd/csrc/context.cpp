@@ -40,6 +40,20 @@ int Context::buffer(const std::string& name, size_t bytes, void** out) {
     return BT_OK;
 }
 
+size_t Context::buffer_bytes() const {
+    size_t t = 0;
+    for (const auto& kv : scratch) t += kv.second.bytes;
+    return t;
+}
+
+void Context::release_buffers() {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamSynchronize(aux);
+    for (auto& kv : scratch)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    scratch.clear();
+}
+
 int Context::stage(size_t bytes, void** out) {
     if (ev_stage && hipEventSynchronize(ev_stage) != hipSuccess)
         return fail(BT_ERR_HIP, "staging: waiting for the previous copies failed");
@@ -88,7 +102,7 @@ int Context::pool_alloc(size_t bytes, void** out, size_t* got) {
 
 void Context::pool_free(void* p, size_t bytes) {
     if (!p) return;
-    if (pool_bytes + bytes > kPoolCap) {
+    if (pool_bytes + bytes > pool_cap) {
         (void)hipFree(p);
         return;
     }
@@ -159,6 +173,30 @@ static double h_b0(double a, double r0) {
     return -(r0 * r0 * r0 - 3. * sq(r0) + asq * r0 + asq) / (a * (r0 - 1.));
 }
 
+// The steepest slope |dq0/db| of the critical curve {(b0(r), q0(r)) : r in
+// [r1, r2]} (for a = 0 the curve q = 27 - b^2, slope 2|b| <= 2 sqrt(27)),
+// sampled densely with a 5% margin; infinity where b0 is not monotonic
+// (then the curve is no graph over b).  Used only to rule rays out of the
+// near-critical set cheaply: a point at vertical distance D from a graph of
+// slope at most S lies at least D / sqrt(1 + S^2) from it.
+static double crit_curve_slope(double a, double r1, double r2) {
+    if (a == 0) return 1.05 * 2. * std::sqrt(27.);
+    const int n = 4096;
+    double smax = 0, prev_db = 0;
+    for (int k = 0; k <= n; k++) {
+        const double r = r1 + (r2 - r1) * k / n;
+        const double asq = a * a;
+        const double db = (asq + a - 2. * r * (r * r - 3. * r + 3.)) / (a * (r - 1.) * (r - 1.));
+        const double N = -r * r * r * (r * r * r - 6. * r * r + 9. * r - 4. * asq);
+        const double Nd = -(6. * std::pow(r, 5) - 30. * std::pow(r, 4) + 36. * r * r * r - 12. * asq * r * r);
+        const double dq = (Nd * (r - 1.) - 2. * N) / (asq * std::pow(r - 1., 3));
+        if (!(std::fabs(db) > 1e-12) || (k > 0 && db * prev_db <= 0)) return HUGE_VAL;
+        prev_db = db;
+        smax = std::max(smax, std::fabs(dq / db));
+    }
+    return 1.05 * smax;
+}
+
 double host_calc_speed(double a, double r, double theta) {
     double omega = 1. / (a + std::pow(r, 1.5));
     return h_wbar(a, r, theta) / h_alpha(a, r, theta) * (omega - h_w(a, r, theta));
@@ -214,6 +252,7 @@ int make_pose(const bt_pose* in, PoseK* P) {
     P->b2 = h_b0(a, r1);
     P->r_ph1 = r1;
     P->r_ph2 = r2;
+    P->crit_slope = crit_curve_slope(a, r1, r2);
     P->gr = (std::sqrt(5.0) + 1.) / 2.;
     P->a_zero = (a == 0) ? 1 : 0;
     P->r_cap = 0;
@@ -242,6 +281,10 @@ int make_params(const bt_integ_params* in, ParamsK* out) {
     out->precision = in->precision;
     out->crit_eps = in->crit_eps;
     out->hmin_fp64 = in->hmin_fp64;
+    if (!(in->noise_budget >= 0) || !(in->decision_margin >= 0))
+        return fail(BT_ERR_INVALID, "integ params: noise_budget and decision_margin must be >= 0");
+    out->noise_budget = (float)in->noise_budget;
+    out->decision_margin = (float)in->decision_margin;
     out->h0 = in->h1 * sgnf((float)in->z_end);   // Metric.h:287
     return BT_OK;
 }
@@ -394,6 +437,8 @@ void bt_integ_params_default(bt_integ_params* p) {
     p->precision = BT_PREC_FP64;
     p->crit_eps = 0.03;
     p->hmin_fp64 = 1e-6;
+    p->noise_budget = 0.13;
+    p->decision_margin = 4.0;
 }
 
 static int trace_common(bt_context* h, const bt_pose* pose, const bt_integ_params* prm,

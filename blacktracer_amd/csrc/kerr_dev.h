@@ -134,6 +134,25 @@ __device__ __forceinline__ double critical_distance(const PoseK& P, double b, do
     return hypot(b - b0(P.a, P.asq, r), q - q0(P.asq, r));
 }
 
+// True when (b, q) lies provably farther than eps from the critical curve,
+// without the projection: outside the curve's b range by more than eps, or
+// (with r0 the b-matched radius of the Newton solve, b0(r0) = b) at a
+// vertical distance |q - q0(r0)| > eps sqrt(1 + S^2), S the curve's steepest
+// slope over b (a point at vertical distance D from a graph of slope at most
+// S is at least D / sqrt(1 + S^2) from it).
+__device__ __forceinline__ bool far_from_critical(const PoseK& P, double b, double q, double r0, double eps) {
+    const double blo = fmin(P.b1, P.b2), bhi = fmax(P.b1, P.b2);
+    if (P.a_zero) {
+        if (fabs(b) > sqrt(27.) + eps) return true;
+        return fabs(q - (27. - b * b)) > eps * sqrt(1. + P.crit_slope * P.crit_slope);
+    }
+    if (b < blo - eps || b > bhi + eps) return true;
+    if (!(P.crit_slope < 1e30) || !(r0 >= P.r_ph1 && r0 <= P.r_ph2)) return false;
+    // the curve's height at b differs from q0(r0) by at most S |b0(r0) - b|
+    const double slack = P.crit_slope * fabs(b0(P.a, P.asq, r0) - b);
+    return fabs(q - q0(P.asq, r0)) - slack > eps * sqrt(1. + P.crit_slope * P.crit_slope);
+}
+
 // Metric.h:110-131
 __device__ __forceinline__ bool check_rup(const PoseK& P, double rV, double bV, double qV,
                                           unsigned& iters) {

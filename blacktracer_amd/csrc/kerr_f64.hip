@@ -21,8 +21,9 @@ using namespace kerr;
 
 // rk_f64 modes: the exact attempt run to the end (the plain FP64 path);
 // attempt_w through the step-size growth only (the mixed path's warm-up);
-// attempt_w run to the end.
-constexpr int kExact = 0, kWarm = 1, kFast = 2;
+// attempt_w run to the end; attempt_w run to the end from the state the
+// warm-up left in cont (the mixed path's re-traced rays).
+constexpr int kExact = 0, kWarm = 1, kFast = 2, kCont = 3;
 
 __global__ void zero_counters(unsigned* c, int n) {
     int i = threadIdx.x;
@@ -94,7 +95,8 @@ __global__ __launch_bounds__(256) void ic_capture(Src S, ParamsK prm, int64_t n,
             if (!escapes) ray_store(io, kl, -1, -1, 0);
             if (SPLIT && escapes) {
                 if (!prm.apriori && !P.a_zero) check_bq(P, b, q, prm.newton_cap, newton, &r0);
-                crit = critical_distance(P, b, q, r0) < prm.crit_eps;
+                crit = !far_from_critical(P, b, q, r0, prm.crit_eps) &&
+                       critical_distance(P, b, q, r0) < prm.crit_eps;
             }
         }
         if (escapes) {
@@ -160,16 +162,24 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
                         sg = S.seg(slot);
                         lp.load(S.pose(sg));
                     }
-                    var[0] = lp.r;
-                    var[1] = lp.theta;
-                    var[2] = lp.phi;
-                    var[3] = Q.pr[slot];
-                    var[4] = Q.pt[slot];
                     b = Q.b[slot];
                     q = Q.q[slot];
-                    z = 0.0;
-                    h = prm.h0;
-                    nstp = 0;
+                    if (MODE == kCont) {
+#pragma unroll
+                        for (int i = 0; i < 5; i++) var[i] = cont.y[i * cont.stride + slot];
+                        z = cont.y[5 * cont.stride + slot];
+                        h = cont.h[slot];
+                        nstp = cont.nstp[slot];
+                    } else {
+                        var[0] = lp.r;
+                        var[1] = lp.theta;
+                        var[2] = lp.phi;
+                        var[3] = Q.pr[slot];
+                        var[4] = Q.pt[slot];
+                        z = 0.0;
+                        h = prm.h0;
+                        nstp = 0;
+                    }
                 } else {
                     exhausted = true;
                 }
@@ -177,6 +187,8 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
         }
         if (__ballot(slot >= 0) == 0) break;
         update_wave_prio(prio, slot >= 0, nstp, cur_prio);
+        bool restart = false;
+        const int64_t my_slot = slot;
         if (slot >= 0) {
             double em;
             if (MODE != kExact) {
@@ -192,7 +204,12 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
             bool esc = z <= prm.z_end || (prm.r_escape > 0 && var[0] > prm.r_escape);
             bool cap = !esc && lp.r_cap > 0 && var[0] < lp.r_cap;
             bool out = !esc && !cap && nstp >= prm.max_steps;
-            if (esc || cap || out) {
+            // a continued ray that nears the attempt cap restarts in the exact
+            // path, so that a capped ray ends exactly as there
+            restart = MODE == kCont && !esc && !cap && !out && nstp >= cont.restart_at;
+            if (restart) {
+                slot = -1;
+            } else if (esc || cap || out) {
                 double th, ph;
                 if (cap) {
                     th = -1;
@@ -210,11 +227,12 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
 #pragma unroll
                 for (int i = 0; i < 5; i++) cont.y[i * cont.stride + slot] = var[i];
                 cont.y[5 * cont.stride + slot] = z;
-                cont.h[slot] = (float)h;
+                cont.h[slot] = h;
                 cont.nstp[slot] = nstp;
                 slot = -1;
             }
         }
+        if (MODE == kCont) queue_push(cont.rq, restart, (uint32_t)my_slot, lane);
     }
     unsigned long long s = wave_sum(attempts);
     if (lane == 0 && s) {
@@ -254,7 +272,7 @@ int queue_alloc(Context* c, int64_t n, QueueK* Q) {
 template <class Src>
 int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const QueueK& Q,
                 const QueueK* Q64, StatsK* d_stats) {
-    zero_counters<<<1, 64, 0, c->stream>>>(Q.counters, 8);
+    zero_counters<<<1, 64, 0, c->stream>>>(Q.counters, 16);   // the 64-byte q_ctr block
     KTimer t(c, Q64 ? "ic_split" : "ic_capture_f64");
     int64_t blocks = (n + 255) / 256;
     if (blocks > (int64_t)c->num_cus * 16) blocks = (int64_t)c->num_cus * 16;
@@ -273,7 +291,10 @@ int trace_f64_queue(Context* c, const Src& S, const ParamsK& prm, int64_t n_max,
     if (!stream) stream = c->stream;
     KTimer t(c, timer, stream);
     Launch L;
-    if (cont) {
+    if (cont && fast) {
+        if (int rc = persistent_launch(c, rk_f64<kCont, Src>, n_max, &L)) return rc;
+        rk_f64<kCont, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, *cont, prio);
+    } else if (cont) {
         if (int rc = persistent_launch(c, rk_f64<kWarm, Src>, n_max, &L)) return rc;
         rk_f64<kWarm, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, *cont, prio);
     } else if (fast) {

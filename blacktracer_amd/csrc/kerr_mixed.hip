@@ -14,7 +14,12 @@
 //         horizon or the poles; every ray near the critical curve, and
 //         every ray whose step collapses (|h| < hmin_fp64) or which nears
 //         the attempt cap, is restarted from its initial state in FP64
-//         (rk_f64, the plain path's kernel) and so ends exactly as there.
+//         (rk_f64, the plain path's kernel) and so ends exactly as there;
+//         every ray whose FP32 step control may have parted from the FP64
+//         one (an accept/reject decision within the rounding noise of the
+//         threshold, or too much noise summed over its attempts; see the
+//         noise accounting below) is traced again in FP64 from the state
+//         the warm-up left (rk_f64<kCont>).
 //   FP32  the six right-hand-side evaluations, stage combinations, error
 //         estimate and step-size control.
 //
@@ -80,8 +85,13 @@ struct RayC {
 };
 
 // Right-hand side at the stage point (r, theta) in FP64 and (pR, pTheta) in FP32.
+// NOISE: xn[i] receives the magnitude of the terms summed into o[i], so that
+// the FP32 rounding error of o[i] is of order u * xn[i] (u = 2^-24).  The
+// null-constraint term X is a difference of terms of order q, r^2 and
+// b^2 / sin^2 that cancel to (near) zero, so it dominates xn[3] and xn[4].
+template <bool NOISE = false>
 __device__ __forceinline__ void derivs_m(const RayC& C, double r_d, double th_d, float pr, float pt,
-                                         float* o) {
+                                         float* o, float* xn = nullptr) {
     double rm1_d = r_d - 1.0;
     double D_d = fma(rm1_d, rm1_d, -C.e);
     double P_d = fma(r_d, r_d, C.Poff);
@@ -102,15 +112,47 @@ __device__ __forceinline__ void derivs_m(const RayC& C, double r_d, double th_d,
     o[2] = iR * (C.a * Ph - C.a + C.b * is2);
     o[3] = iR * (2.f * r * Ph - rm1 * (Ph * Ph + prsq)) + r * iR2X;
     o[4] = iR * c * (C.bsq * is2 * is - C.asq * s) - C.asq * c * s * iR2X;
+    if (NOISE) {
+        const float Xa = pt * pt + fabsf(Theta) + fabsf(C.partR) + prsq * fabsf(D) + fabsf(P * Ph) +
+                         C.bsq * is2 * c2;
+        const float iR2Xa = iR * iR * Xa;
+        xn[0] = fabsf(o[0]);
+        xn[1] = fabsf(o[1]);
+        xn[2] = iR * (fabsf(C.a * Ph) + fabsf(C.a) + fabsf(C.b * is2));
+        xn[3] = iR * (fabsf(2.f * r * Ph) + fabsf(rm1) * (Ph * Ph + prsq)) + fabsf(r) * iR2Xa;
+        xn[4] = iR * fabsf(c) * (C.bsq * is2 * fabsf(is) + C.asq * fabsf(s)) + C.asq * fabsf(c * s) * iR2Xa;
+    }
 }
+
+// Rounding-noise accounting of the FP32 step control.  The FP32 error
+// estimate err_i = h * sum_j dc_j k_j carries an absolute rounding error of
+// about u |h| sum_j |dc_j| xn_i, so errmax carries
+//   nu = u |h| S max_i(xn_i / scal_i) / eps,   S = sum_j |dc_j| = 0.1155,
+// (xn from the first stage stands for all six).  The FP64 path decides with
+// an errmax free of that noise; the two step sequences stay together only
+// while the FP32 decisions agree with the FP64 ones:
+//  * an attempt whose errmax lies within margin * (nu + 5 errmax rho) of
+//    the accept threshold 1 might be decided the other way in FP64; rho is
+//    the relative deviation the noise has left in h (an accepted step's
+//    new h moves by 0.2 nu / errmax, a rejected one's by 0.25 of it and of
+//    the old deviation; errmax scales as h^5);
+//  * the noise summed over a ray's attempts bounds how far its FP32 state
+//    and step sizes may have wandered from the FP64 ones.
+// Either condition hands the ray to an FP64 re-trace from its warm-up state.
+constexpr float kU32 = 5.9604645e-8f;   // 2^-24
+constexpr float kSumAbsDc = 0.11554158f;
 
 // One Cash-Karp attempt with the reference's step control (Metric.h:253-297)
 // in FP32; an accepted step adds the FP32 increment to the FP64 state.
-__device__ __forceinline__ void attempt_m(const RayC& C, double* y, double& z, float& h, float inv_eps) {
-    float k1[5], k2[5], k3[5], k4[5], k5[5], k6[5];
+// Returns false, leaving y, z and h untouched, when the accept/reject
+// decision is within the noise margin of the threshold; nu receives the
+// noise bound of errmax and rho is updated as described above.
+__device__ __forceinline__ bool attempt_m(const RayC& C, double* y, double& z, float& h, float inv_eps,
+                                          float margin, float& rho, float& nu) {
+    float k1[5], k2[5], k3[5], k4[5], k5[5], k6[5], xn[5];
     const float pr0 = (float)y[3], pt0 = (float)y[4];
     const float hh = h;
-    derivs_m(C, y[0], y[1], pr0, pt0, k1);
+    derivs_m<true>(C, y[0], y[1], pr0, pt0, k1, xn);
     float d[5];
 #define STAGE_IN(expr)                                                                      \
     _Pragma("unroll") for (int i = 0; i < 5; i++) d[i] = hh * (expr);
@@ -127,39 +169,51 @@ __device__ __forceinline__ void attempt_m(const RayC& C, double* y, double& z, f
     STAGE_EVAL(k6);
 #undef STAGE_IN
 #undef STAGE_EVAL
-    float errmax = 0.f;
+    float errmax = 0.f, xs = 0.f;
 #pragma unroll
     for (int i = 0; i < 5; i++) {
         d[i] = hh * (fc1 * k1[i] + fc3 * k3[i] + fc4 * k4[i] + fc6 * k6[i]);
         float err = hh * (fdc1 * k1[i] + fdc3 * k3[i] + fdc4 * k4[i] + fdc5 * k5[i] + fdc6 * k6[i]);
-        float scal = fabsf((float)y[i]) + fabsf(k1[i] * hh) + (float)TINY;
-        errmax = fmaxf(errmax, fabsf(err) * rcp(scal));
+        float iscal = rcp(fabsf((float)y[i]) + fabsf(k1[i] * hh) + (float)TINY);
+        errmax = fmaxf(errmax, fabsf(err) * iscal);
+        xs = fmaxf(xs, xn[i] * iscal);
     }
     errmax *= inv_eps;
+    nu = kU32 * kSumAbsDc * fabsf(hh) * xs * inv_eps;
+    if (fabsf(errmax - 1.0f) < margin * (nu + 5.f * errmax * rho)) return false;
     if (errmax <= 1.0f) {
         z += (double)hh;
 #pragma unroll
         for (int i = 0; i < 5; i++) y[i] += (double)d[i];
-        h = errmax > (float)ERRCON ? (float)SAFETY * hh * powf_pos(errmax, (float)PGROW)
-                                   : (float)ADAPTIVE * hh;
+        if (errmax > (float)ERRCON) {
+            h = (float)SAFETY * hh * powf_pos(errmax, (float)PGROW);
+            rho = 0.2f * nu * rcp(errmax);
+        } else {
+            h = (float)ADAPTIVE * hh;
+        }
     } else {
         h = fminf((float)SAFETY * hh * powf_pos(errmax, (float)PSHRNK), 0.1f * hh);
+        rho = 0.25f * (rho + nu * rcp(errmax));
     }
+    return true;
 }
 
 // Continues the rays queued in Q (the warm-up queue, walked a second time
 // with its own head) from their warm-up state in cont, skipping rays that
 // ended in the warm-up; rays whose step collapses or that near the attempt
-// cap go to Q64.
+// cap go to Q64 (exact FP64 from the start), rays whose FP32 step control
+// loses touch with the FP64 one go to Qx (FP64 from the warm-up state).
 template <class Src>
-__global__ __launch_bounds__(256) void rk_mixed(Src S, ParamsK prm, QueueK Q, unsigned* head,
-                                                QueueK Q64, ContK cont, StatsK* st, int restart_at,
-                                                unsigned chunk) {
+__global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q, unsigned* head,
+                                                QueueK Q64, QueueK Qx, ContK cont, StatsK* st,
+                                                int restart_at, unsigned chunk) {
     const unsigned lane = lane_id();
     const unsigned qlen = Q.counters[0];
     unsigned res_next = 0, res_end = 0;   // this wave's reserved queue positions
     const float inv_eps = (float)(1.0 / prm.eps);
     const float hmin = (float)prm.hmin_fp64;
+    const float margin = prm.decision_margin;
+    const float budget = prm.noise_budget > 0 ? prm.noise_budget : __builtin_inff();
 
     int64_t slot = -1;
     int sg = 0;
@@ -177,6 +231,7 @@ __global__ __launch_bounds__(256) void rk_mixed(Src S, ParamsK prm, QueueK Q, un
         r_cap = P.r_cap;
     }
     int nstp = 0;
+    float rho = 0.f, nsum = 0.f;
     int cur_prio = 0;
     unsigned long long attempts = 0, restarts = 0;
 
@@ -214,7 +269,9 @@ __global__ __launch_bounds__(256) void rk_mixed(Src S, ParamsK prm, QueueK Q, un
                     C.partR = (float)(q + (pa - b) * (pa - b));
                     C.Poff = pasq - pa * b;
                     z = cont.y[5 * cont.stride + slot];
-                    h = cont.h[slot];
+                    h = (float)cont.h[slot];
+                    rho = 0.f;
+                    nsum = 0.f;
                 }
             }
         }
@@ -222,16 +279,19 @@ __global__ __launch_bounds__(256) void rk_mixed(Src S, ParamsK prm, QueueK Q, un
         // are not exhausted: they claim again next round
         if (__ballot(slot >= 0 || !exhausted) == 0) break;
         update_wave_prio(0, slot >= 0, nstp, cur_prio);
-        bool restart = false, done = false;
+        bool restart = false, retrace = false, done = false;
         if (slot >= 0) {
-            attempt_m(C, y, z, h, inv_eps);
+            float nu;
+            const bool decided = attempt_m(C, y, z, h, inv_eps, margin, rho, nu);
             attempts++;
+            nsum += nu;
+            retrace = !decided || !(nsum <= budget);
             const int step = nstp;
             nstp++;
-            bool esc = z <= prm.z_end || (prm.r_escape > 0 && y[0] > prm.r_escape);
-            bool cap = !esc && r_cap > 0 && y[0] < r_cap;
-            restart = !esc && !cap && (nstp >= restart_at || !(fabsf(h) >= hmin));
-            done = esc || cap || restart;
+            bool esc = !retrace && (z <= prm.z_end || (prm.r_escape > 0 && y[0] > prm.r_escape));
+            bool cap = !retrace && !esc && r_cap > 0 && y[0] < r_cap;
+            restart = !retrace && !esc && !cap && (nstp >= restart_at || !(fabsf(h) >= hmin));
+            done = esc || cap || restart || retrace;
             if (esc || cap) {
                 double th = -1, ph = -1;
                 if (esc) {
@@ -243,7 +303,8 @@ __global__ __launch_bounds__(256) void rk_mixed(Src S, ParamsK prm, QueueK Q, un
             }
         }
         queue_push(Q64, restart, (uint32_t)slot, lane);
-        if (restart) restarts++;
+        queue_push(Qx, retrace, (uint32_t)slot, lane);
+        if (restart || retrace) restarts++;
         if (done) slot = -1;
     }
     unsigned long long s = wave_sum(attempts);
@@ -299,28 +360,39 @@ int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, Sta
     // Qw: warm-up (FP64) then FP32 queue; Qc: near-critical rays, FP64 from
     // the start, traced on the auxiliary stream while the main stream runs
     // the warm-up and the FP32 bulk; Qr: rays the FP32 kernel hands back
-    // (step collapse, attempt cap), FP64 from the start on the main stream.
-    QueueK Qw, Qc, Qr;
+    // (step collapse, attempt cap), exact FP64 from the start on the main
+    // stream; Qx: rays whose FP32 step control it hands back (noise
+    // accounting), FP64 from their warm-up state.
+    QueueK Qw, Qc, Qr, Qx;
     int rc;
     if ((rc = queue_alloc(c, n, &Qw))) return rc;
     void* p;
-    Qc = Qr = Qw;
+    Qc = Qr = Qx = Qw;
     if ((rc = c->buffer("q_slots_crit", sizeof(uint32_t) * n, &p))) return rc;
     Qc.slots = (uint32_t*)p;
     Qc.counters = Qw.counters + 2;   // [2] = length, [3] = head
     if ((rc = c->buffer("q_slots_restart", sizeof(uint32_t) * n, &p))) return rc;
     Qr.slots = (uint32_t*)p;
     Qr.counters = Qw.counters + 6;   // [6] = length, [7] = head
+    if ((rc = c->buffer("q_slots_retrace", sizeof(uint32_t) * n, &p))) return rc;
+    Qx.slots = (uint32_t*)p;
+    Qx.counters = Qw.counters + 8;   // [8] = length, [9] = head
     unsigned* head32 = Qw.counters + 4;
     ContK cont;
     cont.stride = n;
     if ((rc = c->buffer("cont_y", sizeof(double) * 6 * n, &p))) return rc;
     cont.y = (double*)p;
-    if ((rc = c->buffer("cont_h", sizeof(float) * n, &p))) return rc;
-    cont.h = (float*)p;
+    if ((rc = c->buffer("cont_h", sizeof(double) * n, &p))) return rc;
+    cont.h = (double*)p;
     if ((rc = c->buffer("cont_nstp", sizeof(int32_t) * n, &p))) return rc;
     cont.nstp = (int32_t*)p;
 
+    // rays that near the attempt cap restart in exact FP64, so that a capped
+    // ray ends exactly as in the FP64 path
+    const int margin = prm.max_steps / 16 > 16 ? prm.max_steps / 16 : 16;
+    const int restart_at = prm.max_steps - margin > 1 ? prm.max_steps - margin : 1;
+    cont.rq = Qr;
+    cont.restart_at = restart_at;
     if ((rc = trace_setup(c, S, prm, n, Qw, &Qc, d_stats))) return rc;
     // fork: critical rays on the auxiliary stream, at most one workgroup per CU
     BT_HIP(hipEventRecord(c->ev_fork, c->stream));
@@ -350,16 +422,16 @@ int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, Sta
             (void)join();
             return rc;
         }
-        // rays that near the attempt cap restart in FP64, so that a capped
-        // ray ends exactly as in the FP64 path
-        int margin = prm.max_steps / 16 > 16 ? prm.max_steps / 16 : 16;
-        int restart_at = prm.max_steps - margin > 1 ? prm.max_steps - margin : 1;
-        rk_mixed<Src><<<L.blocks, L.threads, 0, c->stream>>>(S, prm, Qw, head32, Qr, cont, d_stats,
+        rk_mixed<Src><<<L.blocks, L.threads, 0, c->stream>>>(S, prm, Qw, head32, Qr, Qx, cont, d_stats,
                                                              restart_at, c->tune.chunk);
         if (hipGetLastError() != hipSuccess) {
             (void)join();
             return fail(BT_ERR_HIP, "rk_mixed launch failed");
         }
+    }
+    if ((rc = trace_f64_queue(c, S, prm, n, Qx, d_stats, "rk_f64_retrace", &cont, nullptr, 0, true))) {
+        (void)join();
+        return rc;
     }
     rc = trace_f64_queue(c, S, prm, n, Qr, d_stats, "rk_f64_fallback", nullptr);
     const hipError_t je = join();

@@ -908,6 +908,7 @@ struct StarQueue {
     unsigned* ctr;         // [0] queued pixels, [1] chunk indices taken, [2] odd pixels, [3] stored ranges
     float* partial;        // per chunk: FW*FW*3 partial taps
     unsigned cap;          // chunk capacity of `partial`
+    uint8_t* lit;          // per pixel: 1 when its taps are written this frame (0: all its taps are zero)
 };
 
 // All candidates of a pixel in order, into acc (the reference's loop),
@@ -1030,7 +1031,10 @@ __global__ __launch_bounds__(256) void star_map_light(const float* thphi, const 
     }
     wave_append(Q.odd, &Q.ctr[2], to_odd, (uint32_t)ij);
     if (seq) star_seq<STEP, false>(P, s, count, s_cand[threadIdx.y * blockDim.x + threadIdx.x], acc);
-    if (self) store_taps<STEP>(taps, NP, ij, acc);
+    // a pixel without candidates has all-zero taps: they are not stored, and
+    // star_sum skips them (the reference zeroes every tap of every pixel)
+    if (valid) Q.lit[ij] = (queue || to_odd || seq) ? 1 : 0;
+    if (seq) store_taps<STEP>(taps, NP, ij, acc);
 }
 
 // One lane per odd pixel (star_pixel_odd) of few candidates: all its
@@ -1309,8 +1313,8 @@ __device__ __forceinline__ void make_pix(float br, float bg, float bb, uint8_t* 
 // sumStarLight (MakeImageKernel.cu:753-773) + makePix; trail (or null): the
 // frame's star trails, added after the taps (factor 1).
 template <int STEP>
-__global__ __launch_bounds__(256) void star_sum(const float* taps, int W, int H, uint8_t* img, float* o_star,
-                                                const float* trail) {
+__global__ __launch_bounds__(256) void star_sum(const float* taps, const uint8_t* lit, int W, int H, uint8_t* img,
+                                                float* o_star, const float* trail) {
     constexpr int FW = 2 * STEP + 1;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y * blockDim.y + threadIdx.y;
@@ -1323,6 +1327,7 @@ __global__ __launch_bounds__(256) void star_sum(const float* taps, int W, int H,
     for (int u = start; u <= stop; u++)
         for (int v = 0; v <= 2 * STEP; v++) {
             size_t src = (size_t)(i + u - STEP) * W + ((j + v - STEP + W) % W);
+            if (!lit[src]) continue;   // its taps are zero (and not stored)
             int tap = FW * FW - (FW * u + v + 1);
             br += factor * taps[(size_t)(tap * 3 + 0) * NP + src];
             bg += factor * taps[(size_t)(tap * 3 + 1) * NP + src];
@@ -2103,8 +2108,8 @@ size_t frame_bytes(int w, int h, int step) {
     const size_t fw = 2 * (size_t)step + 1;
     // thphi, gap | bh, area x2, star and map images, per-tap star light,
     // star chunk partials, star queues (6), odd pixels and lensing (3), env queues (2),
-    // tree ranges (NP / 4 + 65536 of 8 B), BGRA output
-    return NC * (8 + 4) + NP * (1 + 4 + 4 + 4 + 4 + 12 * fw * fw + 12 * fw * fw + 28 + 12 + 8 + 2 + 4) + 4096 +
+    // lit-pixel flags, tree ranges (NP / 4 + 65536 of 8 B), BGRA output
+    return NC * (8 + 4) + NP * (1 + 4 + 4 + 4 + 4 + 12 * fw * fw + 12 * fw * fw + 28 + 12 + 8 + 2 + 4 + 1) + 4096 +
            65536 * 8;
 }
 
@@ -2259,6 +2264,8 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
         Q.ranges = (int2*)p;
         if ((rc = c->buffer("r_star_qn", 64, &p))) return rc;
         Q.ctr = (unsigned*)p;
+        if ((rc = c->buffer("r_star_lit", NP, &p))) return rc;
+        Q.lit = (uint8_t*)p;
         const int NT = fw * fw * 3;
         // chunk indices beyond cap are never used: a pixel whose chunks do
         // not fit is traced by its own lane in star_map_light
@@ -2340,11 +2347,11 @@ int render_frame(Context* c, const GridF& g, const SceneK& s, const ViewK& v, co
             float* ostar = diffract ? light : (dbg ? dbg->star : nullptr);
             const float* trail = tf ? tf->k.trail : nullptr;
             if (step == 1)
-                star_sum<1><<<grid2d(W, H), blk, 0, c->stream>>>(taps, W, H, img_star, ostar, trail);
+                star_sum<1><<<grid2d(W, H), blk, 0, c->stream>>>(taps, Q.lit, W, H, img_star, ostar, trail);
             else if (step == 2)
-                star_sum<2><<<grid2d(W, H), blk, 0, c->stream>>>(taps, W, H, img_star, ostar, trail);
+                star_sum<2><<<grid2d(W, H), blk, 0, c->stream>>>(taps, Q.lit, W, H, img_star, ostar, trail);
             else
-                star_sum<3><<<grid2d(W, H), blk, 0, c->stream>>>(taps, W, H, img_star, ostar, trail);
+                star_sum<3><<<grid2d(W, H), blk, 0, c->stream>>>(taps, Q.lit, W, H, img_star, ostar, trail);
         }
         BT_HIP(hipGetLastError());
         if (diffract) {

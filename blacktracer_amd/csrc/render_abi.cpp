@@ -239,6 +239,68 @@ int bt_star_tree(const float* stars4, int star_count, int tree_level, int32_t* t
     return star_tree(stars4, star_count, tree_level, tree, star_pos, star_mag);
 }
 
+// float -> int with truncation, saturating out of range, NaN -> 0.
+static int cvt_i32_host(float x) {
+    if (!(x == x)) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x <= -2147483648.0f) return -2147483647 - 1;
+    return (int)x;
+}
+
+// StarProcessor::addLowLightStarsToImage (StarProcessor.h:108-158), on the
+// host as in the reference (a one-time edit of the sky image before the
+// summed table is made).  The splat buffer starts at zero (the reference's
+// cv::Mat is uninitialised) and the colour index stops at the table's last
+// entry, 390 (the reference's clamp, 1172, reads past its table).
+int bt_sky_add_low_stars(uint8_t* sky_bgr, int rows, int cols, const float* stars4, int star_count,
+                         float magnitude_cut) {
+    static const int kRGB[1173] = {
+#include "blackbody_rgb.inc"
+    };
+    if (!sky_bgr || rows < 1 || cols < 1 || star_count < 0 || (star_count > 0 && !stars4))
+        return fail(BT_ERR_INVALID, "bt_sky_add_low_stars: bad arguments");
+    const size_t npix = (size_t)rows * cols;
+    std::vector<int> light(3 * npix, 0);
+    constexpr int kStep = 2;
+    constexpr float kReach = (kStep + .5f) * (kStep + .5f);
+    constexpr double kPi = 3.141592653589793238;   // Const.h:2
+    for (int s = 0; s < star_count; s++) {
+        const float* st = stars4 + 4 * (size_t)s;
+        if (!(st[2] > magnitude_cut)) continue;   // bright stars go to the star tree (StarProcessor.h:39-46)
+        const float pt = (float)(st[0] / kPi * 1.f * rows);          // double arithmetic, stored as float
+        const float pp = (float)(st[1] / (kPi * 2.) * 1.f * cols);
+        const int row0 = (int)pt, col0 = (int)pp;
+        const float bv = st[3];
+        const float tidx = 46.f * ((1.f / ((0.92f * bv) + 1.7f)) + (1.f / ((0.92f * bv) + 0.62f))) - 10.f;
+        const int k = std::max(0, std::min((int)std::floor(tidx), 390));
+        const float cb = (float)(kRGB[3 * k + 2] * kRGB[3 * k + 2]);
+        const float cg = (float)(kRGB[3 * k + 1] * kRGB[3 * k + 1]);
+        const float cr = (float)(kRGB[3 * k] * kRGB[3 * k]);
+        const int u_lo = std::max(0, kStep - row0);
+        const int u_hi = row0 + kStep >= rows ? 2 * kStep - (row0 + kStep - rows + 1) : 2 * kStep;
+        const float fy = pt - row0, fx = pp - col0;
+        for (int u = u_lo; u <= u_hi; u++)
+            for (int v = 0; v <= 2 * kStep; v++) {
+                const float dy = u - kStep + .5f - fy, dx = v - kStep + .5f - fx;
+                const float d = dy * dy + dx * dx;
+                if (d > kReach) continue;
+                // 400 * 10^(-0.4 (m - 2.5 log10 g(d))), g(d) = exp(-2 d) / pi
+                const float g = (float)std::exp((double)(-2.f * d)) * 0.318310f;   // StarProcessor.h:96-98
+                const float e = -.4f * (st[2] - 2.5f * (float)std::log10((double)g));
+                const float bright = 400.f * (float)std::pow(10.0, (double)e);
+                int* px = &light[3 * ((size_t)(row0 - kStep + u) * cols + (size_t)((col0 - kStep + v + cols) % cols))];
+                px[0] = cvt_i32_host(bright * cb);
+                px[1] = cvt_i32_host(bright * cg);
+                px[2] = cvt_i32_host(bright * cr);
+            }
+    }
+    for (size_t q = 0; q < 3 * npix; q++) {
+        const int o = sky_bgr[q];
+        sky_bgr[q] = (uint8_t)std::min(255, (int)std::sqrt((double)(light[q] + o * o)));
+    }
+    return BT_OK;
+}
+
 int bt_scene_create(bt_context* h, const float* stars4, int star_count, int tree_level, const uint8_t* sky_bgr,
                     int sky_rows, int sky_cols, bt_scene** out) {
     if (!h || !out) return fail(BT_ERR_INVALID, "bt_scene_create: null argument");

@@ -18,6 +18,15 @@
 //   * completion is an event on the slot's stream after the frame's copy to
 //     pinned staging memory; the dispatcher then copies it to the client's
 //     buffer and records the latency (arrival -> frame in host memory).
+// The HBM budget counts every device byte the server holds: cached grids
+// (with their FP32 render grids), views, each slot's work buffers at their
+// high-water mark (a slot context keeps its buffers between frames, so a
+// slot is charged for the largest frame it rendered until they are
+// trimmed), and a grid build's arrays and work buffers while it runs (the
+// builder context gives its work buffers back after every build, and freed
+// grids go back to the device, not to a pool).  Grids and slot buffers are
+// freed outside the server lock, after their bytes have been chosen under
+// it; their bytes stay counted until they are gone.
 // Every frame is rendered by the same kernels on the same inputs whatever
 // else is in flight, so a request's result is identical alone and in a mix.
 #include <algorithm>
@@ -81,7 +90,8 @@ struct Slot {
     hipEvent_t done = nullptr;
     uint8_t* staging = nullptr;   // pinned host memory
     size_t staging_bytes = 0;
-    size_t reserved = 0;
+    size_t charged = 0;           // bytes of the slot context's work buffers counted in `used`
+    bool trimming = false;        // its buffers are being freed: not admitted to meanwhile
     Clock::time_point admit;
 };
 
@@ -111,6 +121,7 @@ struct bt_server {
     int64_t submitted = 0, completed = 0, failed = 0, builds = 0, hits = 0, evictions = 0;
     int inflight = 0, peak_inflight = 0;
     size_t building = 0;                // bytes the builder has reserved for a grid in progress
+    size_t releasing = 0;               // bytes chosen for freeing, still counted in `used`
     uint64_t tick = 0;
     bool stop = false;
     bool have_first = false;
@@ -139,10 +150,39 @@ void complete(bt_server* s, const Pending& p, int status, Clock::time_point admi
     s->last_done = now;
 }
 
-// Frees idle cached grids, least recently used first, until `need` more
-// bytes fit the budget.  Caller holds s->mu.
-bool make_room(bt_server* s, size_t need) {
-    while (s->used + need > s->cfg.hbm_budget) {
+// What make_room chose to free: idle grids (taken out of the cache) and
+// idle slots whose work buffers go.  Their bytes stay in `used` (and in
+// `releasing`) until release() has freed them.
+struct Victims {
+    std::vector<std::pair<bt_grid*, size_t>> grids;
+    std::vector<Slot*> slots;
+    size_t bytes = 0;
+    bool empty() const { return grids.empty() && slots.empty(); }
+};
+
+// Device bytes a slot context holds after rendering a w x h frame: the
+// frame's work buffers and the server's output buffer, each named buffer
+// allocated with an eighth of slack and 256 bytes (Context::buffer).
+size_t slot_bytes(int step, int w, int h) {
+    const size_t f = frame_bytes(w, h, step) + (size_t)4 * w * h;
+    return f + f / 8 + (size_t)64 * 256;
+}
+
+// A grid build's device bytes while it runs: the grid's arrays with its
+// FP32 render grid, and the work buffers of the build (queues, continuation
+// state, block lists: at most 160 B per grid point, allocated with an
+// eighth of slack).
+size_t build_bytes(int N, int M) {
+    const size_t NM = (size_t)N * M;
+    return grid_device_bytes(N, M) + 180 * NM + (size_t)64 * 256;
+}
+
+// Chooses idle cached grids (least recently used first), then the work
+// buffers of idle slots other than `keep`, until `need` more bytes fit the
+// budget once they are freed.  Caller holds s->mu; the caller frees the
+// victims with release() before it allocates.
+bool make_room(bt_server* s, size_t need, Victims& v, const Slot* keep = nullptr) {
+    while (s->used - s->releasing + need > s->cfg.hbm_budget) {
         CachedGrid* victim = nullptr;
         GridKey vk{};
         for (auto& kv : s->grids) {
@@ -153,13 +193,59 @@ bool make_room(bt_server* s, size_t need) {
                 vk = kv.first;
             }
         }
-        if (!victim) return false;
-        s->used -= victim->bytes;
-        bt_grid_free(victim->g);
-        s->grids.erase(vk);
-        s->evictions++;
+        if (victim) {
+            v.grids.emplace_back(victim->g, victim->bytes);
+            v.bytes += victim->bytes;
+            s->releasing += victim->bytes;
+            s->grids.erase(vk);
+            s->evictions++;
+            continue;
+        }
+        Slot* idle = nullptr;
+        for (Slot& sl : s->slots)
+            if (&sl != keep && !sl.busy && !sl.trimming && sl.charged > 0) {
+                idle = &sl;
+                break;
+            }
+        if (!idle) return false;
+        idle->trimming = true;
+        v.slots.push_back(idle);
+        v.bytes += idle->charged;
+        s->releasing += idle->charged;
     }
     return true;
+}
+
+// Frees an idle cached grid: no frame holds it (refs == 0 since its last
+// frame retired) and its build has completed, so no stream reads it.  Its
+// blocks go back to the device, not to the builder context's pool, and the
+// builder context's lock (held for the whole of a build) is not taken.
+void free_idle_grid(bt_grid* g) {
+    g->ctx = nullptr;
+    grid_release(g);
+    delete g;
+}
+
+// Frees the victims with s->mu released (lk is its lock), then takes their
+// bytes out of `used`.  A slot being trimmed is not admitted to meanwhile.
+void release(bt_server* s, Victims& v, std::unique_lock<std::mutex>& lk) {
+    if (v.empty()) return;
+    lk.unlock();
+    for (auto& gb : v.grids) free_idle_grid(gb.first);
+    for (Slot* sl : v.slots) {
+        Context& c = sl->ctx->impl;
+        std::lock_guard<std::recursive_mutex> cl(c.mu);
+        c.release_buffers();
+    }
+    lk.lock();
+    for (Slot* sl : v.slots) {
+        sl->charged = 0;
+        sl->trimming = false;
+    }
+    s->used -= v.bytes;
+    s->releasing -= v.bytes;
+    v = Victims();
+    s->cv.notify_all();
 }
 
 void builder_main(bt_server* s) {
@@ -190,10 +276,23 @@ void builder_main(bt_server* s) {
         // reserve the grid's bytes first (Grid.h:612-615 sizes): wait while
         // frames in flight hold the memory, fail if nothing can free it
         const int N = (int)(uint32_t)std::round(std::pow(2, s->cfg.max_level) / (2 - 1) + 1);   // angle = 1
-        const size_t est = grid_device_bytes(N, 2 * (N - 1));
+        const size_t est = build_bytes(N, 2 * (N - 1));
         bool fits;
-        while (!(fits = make_room(s, est)) && s->inflight > 0 && !s->stop) s->cv.wait(lk);
-        if (s->stop) return;
+        while (true) {
+            Victims v;
+            fits = make_room(s, est, v);
+            const bool freed = !v.empty();
+            release(s, v, lk);
+            if (s->stop) return;
+            if (s->used + est <= s->cfg.hbm_budget) {
+                fits = true;
+                break;
+            }
+            if (freed) continue;   // the victims are gone: look again at once
+            // nothing in flight and nothing being freed: it can never fit
+            if (!fits && s->inflight == 0 && s->releasing == 0) break;
+            s->cv.wait(lk);        // frames retire, or another thread finishes freeing
+        }
         if (!fits) {
             CachedGrid& cg = s->grids[key];
             cg.queued = false;
@@ -211,10 +310,13 @@ void builder_main(bt_server* s) {
         bt_grid* g = nullptr;
         int rc = bt_grid_build(s->builder, &pose, &s->cfg.integ, s->cfg.max_level, s->cfg.start_level, 1, &gp, &g);
         GridF gf;
-        if (rc == BT_OK) {
+        {
             Context& bc = s->builder->impl;
             std::lock_guard<std::recursive_mutex> blk(bc.mu);
-            rc = grid_f32_of(bc, g, &gf);
+            if (rc == BT_OK) rc = grid_f32_of(bc, g, &gf);
+            // the build's work buffers go back to the device (the budget
+            // counted them only while the build ran)
+            bc.release_buffers();
         }
         size_t bytes = 0;
         if (rc == BT_OK) bytes = g->bytes + sizeof(float) * 2 * (size_t)g->gridf_rows * g->M;
@@ -233,7 +335,9 @@ void builder_main(bt_server* s) {
             s->peak = std::max(s->peak, s->used);
             s->builds++;
         } else if (g) {
-            bt_grid_free(g);
+            lk.unlock();
+            free_idle_grid(g);
+            lk.lock();
         }
         s->cv.notify_all();
     }
@@ -287,20 +391,29 @@ void dispatcher_main(bt_server* s) {
             Clock::time_point now = Clock::now();
             sl.grid->refs--;
             sl.grid->last_use = ++s->tick;
-            s->used -= sl.reserved;
-            s->inflight--;
+            s->inflight--;   // the slot keeps its buffers, and their charge
             sl.busy = false;
             complete(s, sl.req, BT_OK, sl.admit, now, sl.req.hit);
             progress = true;
         }
         // 2. admit pending requests, oldest first, while slots and memory allow
+        bool rescan = false;
         for (auto it = s->pending.begin(); it != s->pending.end();) {
+            // a free slot whose buffers already hold this frame (the smallest
+            // such), else the free slot holding the most
+            const size_t need_slot = slot_bytes(s->cfg.step, it->r.width, it->r.height);
             Slot* free_slot = nullptr;
-            for (Slot& sl : s->slots)
-                if (!sl.busy) {
+            for (Slot& sl : s->slots) {
+                if (sl.busy || sl.trimming) continue;
+                if (!free_slot) {
                     free_slot = &sl;
-                    break;
+                    continue;
                 }
+                const bool a_fits = sl.charged >= need_slot, b_fits = free_slot->charged >= need_slot;
+                if (a_fits != b_fits ? a_fits : (a_fits ? sl.charged < free_slot->charged
+                                                        : sl.charged > free_slot->charged))
+                    free_slot = &sl;
+            }
             if (!free_slot) break;
             GridKey key{it->r.a, it->r.r, it->r.theta};
             auto git = s->grids.find(key);
@@ -342,12 +455,23 @@ void dispatcher_main(bt_server* s) {
                 ++it;
                 continue;
             }
-            // HBM admission: frames in flight + cached grids within the budget
-            cg.refs++;   // pin it while evicting
-            bool fits = make_room(s, it->bytes);
+            // HBM admission: cached grids and views, every slot's buffers
+            // and a build in progress within the budget
+            const size_t extra = need_slot > free_slot->charged ? need_slot - free_slot->charged : 0;
+            Victims vic;
+            cg.refs++;   // pin it while choosing victims
+            bool fits = make_room(s, extra, vic, free_slot);
             cg.refs--;
+            if (!vic.empty()) {
+                // free them with the lock released, then scan again
+                release(s, vic, lk);
+                rescan = true;
+                progress = true;
+                break;
+            }
+            if (fits && s->used + extra > s->cfg.hbm_budget) break;   // another thread is still freeing
             if (!fits) {
-                if (s->inflight == 0 && s->building == 0) {
+                if (s->inflight == 0 && s->building == 0 && s->releasing == 0) {
                     // nothing in flight, no grid being built and every idle
                     // grid gone: this request (frame + its grid + the views)
                     // can never fit
@@ -361,16 +485,21 @@ void dispatcher_main(bt_server* s) {
             Slot& sl = *free_slot;
             sl.req = *it;
             sl.grid = &cg;
-            sl.reserved = it->bytes;
             sl.admit = Clock::now();
             cg.refs++;
             cg.last_use = ++s->tick;
-            s->used += sl.reserved;
-            s->peak = std::max(s->peak, s->used);
+            s->used += extra;
+            sl.charged += extra;
             int rc = launch(s, sl, cg, vit->second.v);
+            {
+                // charge what the slot's buffers hold now (at most the estimate)
+                const size_t held = sl.ctx->impl.buffer_bytes();
+                s->used = s->used - sl.charged + std::max(sl.charged, held);
+                sl.charged = std::max(sl.charged, held);
+            }
+            s->peak = std::max(s->peak, s->used);
             if (rc != BT_OK) {
                 cg.refs--;
-                s->used -= sl.reserved;
                 complete(s, *it, rc, sl.admit, Clock::now(), it->hit);
             } else {
                 sl.busy = true;
@@ -382,6 +511,7 @@ void dispatcher_main(bt_server* s) {
         }
         if (progress) s->cv.notify_all();
         if (s->stop && s->inflight == 0) return;
+        if (rescan) continue;
         if (!progress) s->cv.wait_for(lk, std::chrono::microseconds(s->inflight ? 50 : 2000));
     }
 }
@@ -401,6 +531,19 @@ void bt_server_config_default(bt_server_config* c) {
     c->view_angle = 3.141592653589793238 / 2;   // BlackTracer.cpp:119-120
     c->off_up = 0.0;
     c->off_right = 0.25 * 3.141592653589793238 / 4;
+}
+
+int bt_server_footprint(const bt_server_config* cfg, int width, int height, bt_server_bytes* out) {
+    if (!cfg || !out || width <= 0 || height <= 0 || cfg->max_level < 1 || cfg->max_level > 12 || cfg->step < 1 ||
+        cfg->step > 3)
+        return fail(BT_ERR_INVALID, "bt_server_footprint: bad arguments");
+    const int N = (int)(uint32_t)std::round(std::pow(2, cfg->max_level) + 1);   // full sky (angle = 1)
+    const int M = 2 * (N - 1);
+    out->slot = slot_bytes(cfg->step, width, height);
+    out->view = sizeof(float) * (2 * (size_t)(width + 1) * (height + 1) + (size_t)width * height);
+    out->grid = grid_device_bytes(N, M);
+    out->build = build_bytes(N, M);
+    return BT_OK;
 }
 
 int bt_server_create(bt_context* h, const bt_scene* scene, const bt_server_config* cfg, bt_server** out) {
@@ -425,6 +568,7 @@ int bt_server_create(bt_context* h, const bt_scene* scene, const bt_server_confi
         return rc;
     }
     s->builder->impl.tune = h->impl.tune;
+    s->builder->impl.pool_cap = 0;   // freed grids go back to the device: the budget sees them go
     s->slots.resize(c.slots);
     for (Slot& sl : s->slots) {
         if ((rc = bt_context_create(h->impl.device, &sl.ctx)) ||
@@ -539,7 +683,7 @@ void bt_server_destroy(bt_server* s) {
         if (sl.ctx) bt_context_destroy(sl.ctx);
     }
     for (auto& kv : s->grids)
-        if (kv.second.g) bt_grid_free(kv.second.g);
+        if (kv.second.g) free_idle_grid(kv.second.g);
     for (auto& kv : s->views)
         if (kv.second.v) bt_view_free(kv.second.v);
     if (s->builder) bt_context_destroy(s->builder);

@@ -1,4 +1,5 @@
-"""Seeded synthetic scene inputs for the render pass (stars and sky map).
+"""Seeded synthetic inputs: the render pass's stars and sky map, and the
+camera poses of the 64-pose batched grid build.
 
 Stars of the 1080p full-frame workload: uniform on the sphere, magnitude uniform in [0, 8],
 temperature log-uniform in [3000, 30000] K.  The reference stores a B-V
@@ -79,3 +80,17 @@ def synthetic_sky(cols=4096, seed=99):
     g = np.clip(25 + 150 * base, 0, 255)
     r = np.clip(20 + 190 * base * (0.6 + 0.4 * n), 0, 255)
     return np.stack([b, g, r], axis=-1).astype(np.uint8)
+
+
+def batch_poses(n=64, seed=2024):
+    """n camera poses of the batched grid build: (a, r, theta, max_level) with
+    a ~ U[0, 0.999] (at least 1e-3: Metric.h:133-143 divides by a), r ~ U[3, 50] M,
+    theta ~ U[0.1, pi - 0.1], max_level ~ U{6..11} (start level 1)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        a = max(1e-3, float(rng.uniform(0, 0.999)))
+        r = float(rng.uniform(3, 50))
+        th = float(rng.uniform(0.1, np.pi - 0.1))
+        out.append((a, r, th, int(rng.integers(6, 12))))
+    return out

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define BT_ABI_VERSION 1
+#define BT_ABI_VERSION 2
 
 /* ---- status codes ---- */
 #define BT_OK              0
@@ -96,6 +96,10 @@ typedef struct {
     double crit_eps;          /* mixed: FP64 for rays whose (b, q) lie within crit_eps of the
                                  critical curve (distance in the (b, q) plane; 0.03) */
     double hmin_fp64;         /* mixed: switch a ray to FP64 once |h| < hmin_fp64 */
+    double noise_budget;      /* mixed: FP64 re-trace of a ray once the summed rounding-noise
+                                 bound of its FP32 error estimates exceeds this (0.13; 0: off) */
+    double decision_margin;   /* mixed: FP64 re-trace of a ray whose FP32 accept/reject decision
+                                 lies within this many noise bounds of the threshold (4; 0: off) */
 } bt_integ_params;
 
 void bt_integ_params_default(bt_integ_params* p);
@@ -220,6 +224,15 @@ int bt_viewer_matrix(double view_angle, double off_up, double off_right, int wid
  * ints; star_pos / star_mag 2*star_count floats in leaf order. */
 int bt_star_tree(const float* stars4, int star_count, int tree_level, int32_t* tree,
                  float* star_pos, float* star_mag);
+/* StarProcessor(...)'s split of a catalogue at the magnitude cut
+ * (StarProcessor.h:36-52) with addLowLightStarsToImage
+ * (StarProcessor.h:108-158): every star of stars4 fainter than
+ * magnitude_cut is painted into the BGR8 sky image (rows x cols, in place)
+ * as a 5 x 5 blackbody splat; the brighter ones belong in the star tree
+ * (bt_star_catalog_parse with the same cut).  Call before building the
+ * scene from the sky. */
+int bt_sky_add_low_stars(uint8_t* sky_bgr, int rows, int cols, const float* stars4, int star_count,
+                         float magnitude_cut);
 
 /* ---- device-resident render objects (no counterpart in the reference,
  * which re-uploads everything per makeImage call; used by the serving
@@ -339,7 +352,9 @@ int  bt_grid_file_read(const char* path, int* max_level, int* N, int* M, int* ha
  * admitted frames overlap on the device.  A request's result does not
  * depend on what else is in flight. */
 typedef struct {
-    size_t hbm_budget;        /* bytes for cached grids + views + frames in flight (16 GiB) */
+    size_t hbm_budget;        /* device bytes the server may hold (16 GiB): cached grids,
+                                 views, every slot's work buffers at their high-water mark,
+                                 a running build (bt_server_footprint) */
     int    slots;             /* frames in flight at most, 1..16 (4) */
     int    max_level, start_level;   /* grids of new poses (10, 1) */
     bt_integ_params integ;    /* integrator settings of those grids (defaults) */
@@ -369,7 +384,7 @@ typedef struct {
     int64_t submitted, completed, failed;
     int64_t grid_builds, grid_hits, evictions;
     int     peak_inflight;
-    size_t  peak_bytes;       /* peak of cached grids + views + reserved frame bytes */
+    size_t  peak_bytes;       /* peak of the device bytes counted against the budget */
     double  p50_ms, p99_ms, mean_ms;   /* latency of completed requests */
     double  frames_per_s;     /* completed / (last completion - first arrival) */
     double  wall_s;
@@ -377,7 +392,16 @@ typedef struct {
 
 typedef struct bt_server bt_server;
 
+/* What a server counts against its budget, per item. */
+typedef struct {
+    size_t slot;    /* a slot's work buffers after a width x height frame */
+    size_t view;    /* the view of one resolution */
+    size_t grid;    /* a cached grid (its arrays and FP32 render grid) */
+    size_t build;   /* a grid build while it runs (arrays, render grid, work buffers) */
+} bt_server_bytes;
+
 void bt_server_config_default(bt_server_config* cfg);
+int  bt_server_footprint(const bt_server_config* cfg, int width, int height, bt_server_bytes* out);
 /* The scene must outlive the server. */
 int  bt_server_create(bt_context* ctx, const bt_scene* scene, const bt_server_config* cfg,
                       bt_server** out);

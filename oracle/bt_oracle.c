@@ -242,23 +242,32 @@ void bto_wrap_to_pi(double* thetaW, double* phiW) {
     *phiW = fmod(*phiW, PI2);
 }
 
+/* Test knob (default off): scales every sin/cos of derivs by (1 + k 2^-53),
+ * about k ulps, so that tests can find the rays whose result a last-ulp
+ * difference of the trigonometric functions moves (the device's sin/cos
+ * need not round as the host libm does). */
+static int g_trig_ulp = 0;
+void bto_set_trig_ulp(int k) { g_trig_ulp = k; }
+static double d_sin(double x) { return g_trig_ulp ? sin(x) * (1.0 + g_trig_ulp * 0x1p-53) : sin(x); }
+static double d_cos(double x) { return g_trig_ulp ? cos(x) * (1.0 + g_trig_ulp * 0x1p-53) : cos(x); }
+
 /* Metric.h:198-226.  var = (r, theta, phi, pR, pTheta). */
 void bto_derivs(double a, const double* var, double* out, double b, double q) {
     const double asq = a * a;
     const double r = var[0], th = var[1], pr = var[3], pt = var[4];
-    double cosv = cos(th);
-    double sinv = sin(th);
+    double cosv = d_cos(th);
+    double sinv = d_sin(th);
     double cossq = sq(cosv);
     double sinsq = sq(sinv);
     double bsq = sq(b);
     double delta = sq(r) - 2. * r + asq;                 /* _Delta */
-    double rosq = sq(r) + asq * sq(cos(th));              /* _rosq */
+    double rosq = sq(r) + asq * sq(d_cos(th));            /* _rosq */
     double P = sq(r) + asq - a * b;                       /* _P */
     double prsq = sq(pr);
     double pthetasq = sq(pt);
     double R = k_R(a, r, b, q);                           /* _R */
     double partR = (q + sq(a - b));
-    double btheta = q - sq(cos(th)) * (sq(b) / sq(sin(th)) - asq); /* _BigTheta */
+    double btheta = q - sq(d_cos(th)) * (sq(b) / sq(d_sin(th)) - asq); /* _BigTheta */
     double rosqsq = sq(2 * rosq);
     double sqrosqdel = (sq(rosq) * delta);
     double asqcossin = asq * cosv * sinv;

@@ -77,6 +77,8 @@ void   bto_derivs(double a, const double* var, double* out, double b, double q);
 void   bto_rkck(double a, const double* var, const double* dvdz, double h,
                 double* out, double* err, double b, double q);
 void   bto_wrap_to_pi(double* theta, double* phi);
+/* Test knob: scale derivs' sin/cos by (1 + k 2^-53) (0: off, the default). */
+void   bto_set_trig_ulp(int k);
 
 /* Initial conditions of one camera-sky direction (Grid.h:524-547):
  * out[0..3] = pR, pTheta, b, q. */
@@ -133,6 +135,10 @@ int    btor_make_tree(const float* stars4, int S, int tree_level, int32_t* tree,
                       float* pos, float* mag);
 /* Summed-area sky table (MakeImageKernel.cu:1312-1331): rows*cols*4 floats. */
 void   btor_sum_table(const uint8_t* sky_bgr, int rows, int cols, float* table);
+/* addLowLightStarsToImage (StarProcessor.h:108-158): stars4 (theta, phi,
+ * magnitude, B-V) of the stars below the catalogue cut painted into the
+ * BGR8 sky in place. */
+void   btor_sky_low_stars(uint8_t* sky_bgr, int rows, int cols, const float* stars4, int n);
 /* makeGrid from a perfect spatial hash (MakeImageKernel.cu:369-384). */
 void   btor_make_grid(const float* hash_table, const int32_t* offset_table,
                       const int32_t* hash_pos_tag, int hw, int ow, int GN, int GM, float* grid);

@@ -55,10 +55,35 @@
  * stays in the frame, gradient reads past the corner array count as
  * invalid, and shadow corners get a zero gradient (unwritten there).
  *
- * Pinning: stage outputs are checked by tests against closed forms
- * (Hermite identity on grid points, spherical-triangle areas summing to
- * 4 pi, summed-table identity, HSP round trip, star-tree leaf membership by
- * brute force, point-in-quad by brute force).
+ * Pinning (each function, the check that would fail on a plausible
+ * mistake in it; tests/test_oracle_render_pins.py (P),
+ * tests/test_oracle_render_closed_forms.py (C), test_oracle_blend_pins.py,
+ * test_oracle_trail_pins.py, test_low_stars_cpu.py):
+ *   camera / sphere views     closed-form pinhole and equirect directions (C)
+ *   make_tree, search_tree    brute-force leaf membership and cover (P)
+ *   calc_area3/4              Van Oosterom-Strackee; quads tile 4 pi (P)
+ *   pi_check(_tot), wrap_to_pi_f   seam unwrapping; brute force (C)
+ *   hermite_f, intersection   exact on quadratics; line crossing (C)
+ *   interpolate_* / find_*    quadratic fields reproduced, through coarse
+ *                             blocks and next to captured points (C, P)
+ *   rgb_to_hsp / hsp_to_rgb   round trip (P)
+ *   redshift                  static observer; E_inf / E_cam; flat-space
+ *                             special-relativistic Doppler (P, C)
+ *   gaussian, interpolate_star, the star taps and the x20 sum
+ *                             normalised profile; affine sub-pixel position;
+ *                             taps = 100 frac g 10^(-0.4 (m + 4 log10 z))
+ *                             times the blackbody colour; sums over stars (C)
+ *   make_pix, composite       saturation / hue preservation; root sum of
+ *                             squares (C)
+ *   smooth_area               constant fields kept, a delta spread as
+ *                             1/(2r+1) per axis (C)
+ *   the sky pass              constant sky; footprint mean of a known sky;
+ *                             lensing and redshift in HSP (C)
+ *   sum_table, make_grid      summed-area identity (P); hash round trip (C)
+ *   blend_*, trails           border walk on a disk, self-blend, endpoints;
+ *                             trails along the motion, sprites (blend/trail)
+ *   btor_sky_low_stars        5 x 5 splat closed form, overwrite order,
+ *                             seam and poles (low stars)
  */
 #include "bt_oracle.h"
 
@@ -996,6 +1021,61 @@ int btor_search_tree(const int32_t* tree, const float* mn, const float* mx, int 
 /* ------------------------------------------------------------ summed table */
 
 /* MakeImageKernel.cu:1312-1331: table[(p * cols + q) * 4 + c] (BGR + area) */
+/* StarProcessor::addLowLightStarsToImage (StarProcessor.h:108-158): the
+ * stars fainter than the catalogue cut are painted into the sky image (BGR8,
+ * rows x cols, in place), each as a 5 x 5 splat (taps within 2.5 pixels)
+ * of brightness 400 * 10^(-0.4 (m - 2.5 log10 g(d))) with
+ * g(d) = 0.318310 exp(-2 d) of the squared tap distance d, times the star's
+ * squared blackbody colour; a tap overwrites what an earlier star left
+ * there (catalogue order); then every pixel becomes
+ * min(255, (int) sqrt(star + original^2)) per channel.  Divergences on
+ * undefined behaviour: the splat buffer starts at zero (the reference's is
+ * an uninitialised cv::Mat) and the colour index is clamped to 390 (the
+ * reference clamps to 1172 and reads past its 1173-entry table). */
+void btor_sky_low_stars(uint8_t* sky_bgr, int rows, int cols, const float* stars4, int n) {
+    int* buf = (int*)calloc((size_t)rows * cols * 3, sizeof(int));
+    const int step = 2;
+    const float maxDistSq = (step + .5f) * (step + .5f);
+    for (int s = 0; s < n; s++) {
+        const float thetastar = stars4[4 * s], phistar = stars4[4 * s + 1];
+        const float magnitude = stars4[4 * s + 2], bvcolor = stars4[4 * s + 3];
+        /* the float angle over the double PI: evaluated in double, stored as float */
+        const float pixPosTheta = (float)(thetastar / DPI * 1.f * rows);
+        const float pixPosPhi = (float)(phistar / DPI2 * 1.f * cols);
+        const int pixHor = (int)pixPosPhi, pixVert = (int)pixPosTheta;
+        const float temp = 46.f * ((1.f / ((0.92f * bvcolor) + 1.7f)) + (1.f / ((0.92f * bvcolor) + 0.62f))) - 10.f;
+        int index = (int)floorf(temp);
+        index = index < 0 ? 0 : index > 390 ? 390 : index;
+        const float rx = (float)(kTempRGB[3 * index] * kTempRGB[3 * index]);
+        const float ry = (float)(kTempRGB[3 * index + 1] * kTempRGB[3 * index + 1]);
+        const float rz = (float)(kTempRGB[3 * index + 2] * kTempRGB[3 * index + 2]);
+        const int start = 0 > -(pixVert - step) ? 0 : -(pixVert - step);
+        const int diff = pixVert + step - rows + 1;
+        const int stop = (pixVert + step >= rows) ? 2 * step - diff : 2 * step;
+        for (int u = start; u <= stop; u++) {
+            for (int v = 0; v <= 2 * step; v++) {
+                const float a = -step + u + .5f - (pixPosTheta - pixVert);
+                const float b = -step + v + .5f - (pixPosPhi - pixHor);
+                const float dist = a * a + b * b;
+                if (dist > maxDistSq) continue;
+                const float gauss = f_exp(-2.f * dist) * 0.318310f;
+                const float appMag = -.4f * (magnitude - 2.5f * f_log10(gauss));
+                const float brightness = 400.f * f_pow(10.f, appMag);
+                const size_t o = 3 * ((size_t)(-step + u + pixVert) * cols + (size_t)((-step + v + pixHor + cols) % cols));
+                buf[o] = cvt_i32(brightness * rz);
+                buf[o + 1] = cvt_i32(brightness * ry);
+                buf[o + 2] = cvt_i32(brightness * rx);
+            }
+        }
+    }
+    for (size_t q = 0; q < (size_t)rows * cols * 3; q++) {
+        const int orig = sky_bgr[q];
+        const int v = (int)sqrt((double)(buf[q] + orig * orig));
+        sky_bgr[q] = (uint8_t)(v < 255 ? v : 255);
+    }
+    free(buf);
+}
+
 void btor_sum_table(const uint8_t* sky_bgr, int rows, int cols, float* table) {
     float pxsz = FPI2 / (1.f * cols);
 #ifdef _OPENMP

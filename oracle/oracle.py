@@ -212,6 +212,29 @@ def trace_rays(ps, prm, theta, phi, nthreads=0):
     return th, ph, st, s
 
 
+def set_trig_ulp(k):
+    """Test knob: scale the integrator's sin/cos by (1 + k 2^-53); 0 turns it off."""
+    lib().bto_set_trig_ulp(int(k))
+
+
+def ulp_sensitive(ps, prm, theta, phi, tol=1e-10):
+    """Rays whose traced result moves by more than tol rad, or whose step
+    count changes, when the integrator's sin/cos move by about one ulp either
+    way: the rays a last-ulp difference between two libms can move."""
+    t0, p0, s0, _ = trace_rays(ps, prm, theta, phi)
+    out = np.zeros(len(theta), bool)
+    for k in (1, -1):
+        set_trig_ulp(k)
+        try:
+            t1, p1, s1, _ = trace_rays(ps, prm, theta, phi)
+        finally:
+            set_trig_ulp(0)
+        dp = np.abs(p1 - p0)
+        dp = np.minimum(dp, 2 * np.pi - dp)
+        out |= (np.abs(t1 - t0) > tol) | (dp > tol) | (s1 != s0)
+    return out
+
+
 def initial_conditions(ps, theta, phi):
     out = np.zeros(4)
     lib().bto_initial_conditions(C.byref(ps), theta, phi, _ptr(out, DP))
@@ -283,6 +306,15 @@ def make_tree(stars4, tree_level):
     mag = np.zeros((S, 2), dtype=np.float32)
     lib().btor_make_tree(_ptr(s4, FP), S, tree_level, _ptr(tree, IP), _ptr(pos, FP), _ptr(mag, FP))
     return tree, pos, mag
+
+
+def sky_low_stars(sky_bgr, stars4):
+    """addLowLightStarsToImage of every star in stars4 (the caller applies the
+    magnitude cut): a painted copy of the BGR8 sky."""
+    sky = np.ascontiguousarray(sky_bgr, dtype=np.uint8).copy()
+    s4 = f32(stars4).reshape(-1, 4)
+    lib().btor_sky_low_stars(_ptr(sky, U8P), sky.shape[0], sky.shape[1], _ptr(s4, FP), s4.shape[0])
+    return sky
 
 
 def sum_table(sky_bgr):

@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_version_and_defaults():
     L = _lib.load()
-    assert L.bt_abi_version() == 1
+    assert L.bt_abi_version() == 2
     p = _lib.BtIntegParams()
     L.bt_integ_params_default(C.byref(p))
     assert (p.eps, p.h1, p.z_end, p.max_steps, p.apriori_capture) == (1e-5, 0.01, -1e7, 1000, 1)

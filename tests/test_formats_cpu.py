@@ -4,8 +4,12 @@ Pins: the catalogue conversion keeps each star's direction (equatorial
 (ra, dec) and the wrapped (theta, phi) give the same unit vector), the
 magnitude cut and file order; the .star layout is checked byte by byte
 against cereal's binary archive rules (a 32-bit int raw, a vector as a
-64-bit count then its raw elements, little endian)."""
+64-bit count then its raw elements, little endian) and against a file that
+cereal itself writes (the reference's vendored cereal 1.2.2, built into
+oracle/_ref/libbtcereal.so by oracle/Makefile where the checkout exists)."""
+import ctypes as C
 import math
+import os
 import struct
 
 import numpy as np
@@ -120,3 +124,30 @@ def test_grid_file_header_rules(tmp_path):
     p.write_bytes(raw[:-8] + struct.pack("<ii", hw + 1, ow))
     with pytest.raises(bt.BtError):
         bt.read_grid_file(p)
+
+
+CEREAL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                      "libbtcereal.so")
+
+
+@pytest.mark.skipif(not os.path.exists(CEREAL), reason="oracle/_ref/libbtcereal.so not built (no reference checkout)")
+@pytest.mark.parametrize("S,L", [(0, 1), (1, 2), (2500, 8)])
+def test_star_file_equals_cereal_archive(tmp_path, S, L):
+    """StarProcessor's cache (StarProcessor.h:19-23) written by cereal's own
+    BinaryOutputArchive and by bt_star_file_write: the same bytes."""
+    rng = np.random.default_rng(S + L)
+    stars = np.stack([np.arccos(1 - 2 * rng.random(S)), 2 * math.pi * rng.random(S),
+                      rng.uniform(0, 8, S), rng.uniform(0, 1.5, S)], -1).astype(np.float32).reshape(S, 4)
+    tree, pos, mag = bt.star_tree(stars, L)
+    ours = tmp_path / "ours.star"
+    bt.write_star_file(ours, tree, L, pos, mag)
+    lib = C.CDLL(CEREAL)
+    theirs = tmp_path / "cereal.star"
+    tree = np.ascontiguousarray(tree, np.int32)
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1)
+    mag = np.ascontiguousarray(mag, np.float32).reshape(-1)
+    fp = C.POINTER(C.c_float)
+    rc = lib.btcereal_star_write(str(theirs).encode(), tree.ctypes.data_as(C.POINTER(C.c_int)), C.c_int(tree.size),
+                                 pos.ctypes.data_as(fp), mag.ctypes.data_as(fp), C.c_int(S), C.c_int(L))
+    assert rc == 0
+    assert ours.read_bytes() == theirs.read_bytes()

@@ -40,7 +40,7 @@ def test_batch_equals_single_builds(ctx):
         assert g.rays_traced == single.rays_traced
         assert np.array_equal(g.cel, single.cel)
         assert np.array_equal(g.levels, single.levels)
-        assert np.mean(g.steps != single.steps) <= 0.001
+        assert np.array_equal(g.steps, single.steps)
         total += g.rays_traced
     assert st["rays"] == total
 

@@ -1,8 +1,14 @@
 """The reference's caches through the device path: a grid written to a
 .grid file and read back renders through makeImage exactly as the resident
 grid; a scene made from a .star file renders exactly as the scene built from
-the same stars."""
+the same stars; a .grid file has the bytes cereal itself writes for the
+same tables (oracle/_ref/libbtcereal.so, built from the reference's vendored
+cereal headers where the checkout exists).  The element layout of CUDA's
+float2/int2 in those tables (x then y) is the one assumption: the
+reference's tree has no serializer for them."""
+import ctypes as C
 import math
+import os
 
 import numpy as np
 import pytest
@@ -57,3 +63,33 @@ def test_star_file_scene_renders_as_built_scene(ctx, tmp_path):
     assert np.array_equal(bt.render_frame(g, a, v), bt.render_frame(g, b, v))
     with pytest.raises(bt.BtError):   # a tree that does not index the stars
         bt.Scene.from_tree(t2, l2, p2[:-1], m2[:-1], sky, ctx=ctx)
+
+
+CEREAL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                      "libbtcereal.so")
+
+
+@pytest.mark.skipif(not os.path.exists(CEREAL), reason="oracle/_ref/libbtcereal.so not built (no reference checkout)")
+def test_grid_file_equals_cereal_archive(ctx, tmp_path):
+    """Grid's cache (Grid.h:34-39 with PSHOffsetTable.h:44-48) written by
+    cereal's BinaryOutputArchive from the grid's exported tables and by
+    bt_grid_file_write: the same bytes, adaptive and uniform grids."""
+    lib = C.CDLL(CEREAL)
+    bh = bt.BlackHole(0.7)
+    cam = bt.Camera(1.1, 0.0, 12.0, black_hole=bh)
+    for lv, start in ((8, 1), (5, 5)):
+        g = bt.Grid(lv, start, True, cam, bh, ctx=ctx)
+        ours = tmp_path / f"ours{lv}.grid"
+        g.write(ours)
+        ht, ot, tag, hw, ow = g.psh() if start != lv else (np.zeros(0, np.float32), np.zeros(0, np.int32),
+                                                           np.zeros(0, np.int32), 0, 0)
+        ht = np.ascontiguousarray(ht, np.float32).reshape(-1)
+        ot = np.ascontiguousarray(ot, np.int32).reshape(-1)
+        tag = np.ascontiguousarray(tag, np.int32).reshape(-1)
+        theirs = tmp_path / f"cereal{lv}.grid"
+        ip = C.POINTER(C.c_int)
+        rc = lib.btcereal_grid_write(str(theirs).encode(), C.c_int(g.MAXLEVEL), C.c_int(g.N), C.c_int(g.M),
+                                     ht.ctypes.data_as(C.POINTER(C.c_float)), ot.ctypes.data_as(ip),
+                                     tag.ctypes.data_as(ip), C.c_int(hw), C.c_int(ow))
+        assert rc == 0
+        assert ours.read_bytes() == theirs.read_bytes()

@@ -63,10 +63,11 @@ def test_alone_equals_in_mix_equals_direct(ctx, scene):
 
 
 def test_budget_admission_and_eviction(ctx, scene):
-    # room for one 640x360 frame, its view (641 x 361 corners of 8 B and
-    # 640 x 360 pixel solid angles) and about two level-7 grids (about 1 MiB
-    # each): frames run one at a time and the three poses evict each other
-    budget = bt.frame_bytes(640, 360) + 641 * 361 * 8 + 640 * 360 * 4 + (5 << 19)
+    # room for one 640x360 slot's buffers, its view, a grid build and about
+    # one and a half level-7 grids: frames run one at a time and the three
+    # poses evict each other
+    fp = bt.server_footprint(640, 360, max_level=LEVEL)
+    budget = fp["slot"] + fp["view"] + fp["build"] + 3 * fp["grid"] // 2
     srv = bt.Server(scene, hbm_budget=budget, slots=4, max_level=LEVEL, ctx=ctx)
     reqs = [(POSES[k % 3], 0.1 * k, (640, 360)) for k in range(9)]
     outs, comps = _run(srv, reqs)
@@ -93,12 +94,59 @@ def test_budget_admission_and_eviction(ctx, scene):
 def test_request_that_can_never_fit_fails(ctx, scene):
     """Frame + view + grid above the budget with nothing in flight: the
     request completes with BT_ERR_BUDGET instead of waiting forever."""
-    # the frame alone fits; frame + view does not
-    srv = bt.Server(scene, hbm_budget=bt.frame_bytes(640, 360) + 641 * 361 * 4, slots=2, max_level=LEVEL,
-                    ctx=ctx)
+    # the frame alone fits; its slot's buffers, its view and its grid do not
+    fp = bt.server_footprint(640, 360, max_level=LEVEL)
+    srv = bt.Server(scene, hbm_budget=fp["slot"] + fp["view"] // 2, slots=2, max_level=LEVEL, ctx=ctx)
     out = np.zeros((360, 640, 4), dtype=np.uint8)
     srv.submit(*POSES[0], 0.0, out, 7)
     srv.drain(60)
     comps = srv.poll(8)
     assert len(comps) == 1 and comps[0]["status"] == bt.BT_ERR_BUDGET
+    srv.close()
+
+
+def _hip_free_bytes():
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_device_memory_stays_within_the_budget(ctx, scene):
+    """The budget counts what the device really holds: the drop in free
+    device memory (hipMemGetInfo) over a budget-limited run, sampled while
+    frames run and after they drained, stays within hbm_budget.  The budget
+    holds two 640 x 360 slots' work buffers, not four, so slots are trimmed
+    and grids evicted."""
+    import time
+    fp = bt.server_footprint(640, 360, max_level=LEVEL)
+    budget = 2 * fp["slot"] + 2 * fp["view"] + fp["build"] + 2 * fp["grid"]
+    srv = bt.Server(scene, hbm_budget=budget, slots=4, max_level=LEVEL, ctx=ctx)
+    ctx.sync()
+    time.sleep(0.05)
+    free0 = _hip_free_bytes()
+    poses = POSES + [(0.3, 6.0, 1.0)]
+    reqs = [(poses[k % 4], 0.07 * k, [(640, 360), (320, 180)][k % 2]) for k in range(16)]
+    outs = []
+    for tag, (pose, off, (w, h)) in enumerate(reqs):
+        out = np.zeros((h, w, 4), dtype=np.uint8)
+        outs.append(out)
+        srv.submit(*pose, off, out, tag)
+    low = free0
+    t0 = time.perf_counter()
+    done = 0
+    while done < len(reqs) and time.perf_counter() - t0 < 120:
+        low = min(low, _hip_free_bytes())
+        done += len(srv.poll(64))
+        time.sleep(0.002)
+    srv.drain(60)
+    low = min(low, _hip_free_bytes())
+    st = srv.stats()
+    drop = free0 - low
+    print(f"budget {budget >> 20} MiB: free-memory drop {drop >> 20} MiB, counted peak {st['peak_bytes'] >> 20} MiB, "
+          f"evictions {st['evictions']}, completed {st['completed']}", flush=True)
+    assert st["completed"] == len(reqs)
+    assert st["peak_bytes"] <= budget
+    assert drop <= budget
     srv.close()

@@ -1,9 +1,14 @@
 """Parity of the device ray path (bt_trace_rays, FP64) with the CPU oracle.
 
-Bar: capture flags bit-exact; celestial (theta', phi') within 1e-9 rad;
-step counts equal (the device sin/cos/pow may differ from the host libm in
-the last ulp, which can move a step count only for rays whose step control
-is at a decision boundary -- counted and bounded here)."""
+Bar: capture flags bit-exact; celestial (theta', phi') within 1e-9 rad and
+step counts equal on every ray.  The one exemption: in the radius-stop
+variant (rel tol 1e-8, thousands of steps per ray) the rays that the oracle
+itself shows to be ill-conditioned at the last ulp -- whose result moves by
+more than 1e-10 rad, or whose step count changes, when the oracle's sin/cos
+move by one ulp either way (oracle.ulp_sensitive) -- may differ as much as
+the device's sin/cos and the host libm do; they are listed and reported,
+and must still agree in capture flag and within 1e-7 rad.  With the
+reference settings no ray of these tests is ulp-sensitive."""
 import math
 
 import numpy as np
@@ -22,18 +27,24 @@ def _dirs(n, seed):
     return np.arccos(1 - 2 * rng.random(n)), 2 * math.pi * rng.random(n)
 
 
-def _compare(th, ph, st, oth, oph, ost, max_step_diff_frac=0.001):
+def _compare(th, ph, st, oth, oph, ost, exempt=None):
     cap, ocap = th == -1, oth == -1
     assert np.array_equal(cap, ocap), f"{np.sum(cap != ocap)} capture flags differ"
     m = ~cap
-    dth = np.abs(th[m] - oth[m])
-    dph = np.abs(ph[m] - oph[m])
+    ex = np.zeros(th.size, bool) if exempt is None else exempt
+    dth = np.abs(th - oth)
+    dph = np.abs(ph - oph)
     dph = np.minimum(dph, 2 * math.pi - dph)
-    bad = (dth > TOL) | (dph > TOL)
-    assert bad.sum() <= max(1, int(0.0005 * m.sum())), \
-        f"{bad.sum()} rays beyond {TOL} rad (max {max(dth.max(), dph.max()):.3g})"
-    assert np.mean(st != ost) <= max_step_diff_frac
-    return max(dth.max(initial=0), dph.max(initial=0))
+    d = np.where(m, np.maximum(dth, dph), 0.0)
+    bad = (d > TOL) & ~ex
+    assert bad.sum() == 0, f"{bad.sum()} rays beyond {TOL} rad (max {d[~ex].max():.3g})"
+    assert np.array_equal(st[~ex], ost[~ex]), f"{np.sum(st[~ex] != ost[~ex])} step counts differ"
+    if ex.any():
+        print(f"  exempt (ulp-sensitive) rays: {ex.sum()}, beyond {TOL} rad: {np.sum(d[ex] > TOL)}, "
+              f"max {d[ex].max():.3g} rad, steps differ {np.sum(st[ex] != ost[ex])}; indices "
+              f"{np.flatnonzero(ex & (d > TOL)).tolist()}", flush=True)
+        assert d[ex].max() < 1e-7
+    return d[~ex].max(initial=0)
 
 
 @pytest.mark.parametrize("pose", [(0.999, math.pi / 2, 0.0, 5.0), (0.5, 1.2, 0.3, 3.0),
@@ -50,7 +61,7 @@ def test_trace_rays_fp64_matches_oracle(ctx, pose):
     assert stats["rays"] == t.size
     assert stats["integrated"] == ostats.integrated
     assert stats["newton"] == ostats.newton and stats["golden"] == ostats.golden
-    assert abs(stats["attempts"] - ostats.attempts) <= 0.001 * ostats.attempts
+    assert stats["attempts"] == ostats.attempts
     print(f"  max |d angle| = {err:.3g} rad", flush=True)
 
 
@@ -122,7 +133,7 @@ def test_trace_rays_edge_cases(ctx):
     th, ph, st = bt.trace_rays(cam, t, p, ctx=ctx)
     ps = O.pose(0.999, math.pi / 2, 0.0, 5.0)
     oth, oph, ost, _ = O.trace_rays(ps, O.params(), t, p)
-    _compare(th, ph, st, oth, oph, ost, max_step_diff_frac=0.0)
+    _compare(th, ph, st, oth, oph, ost)
 
 
 def test_trace_rays_max_steps_sentinel(ctx):
@@ -134,7 +145,7 @@ def test_trace_rays_max_steps_sentinel(ctx):
     th, ph, st = bt.trace_rays(cam, t, p, params=prm, ctx=ctx)
     ps = O.pose(0.9, 1.0, 0.5, 5.0)
     oth, oph, ost, _ = O.trace_rays(ps, O.params(eps=1e-14, max_steps=20), t, p)
-    _compare(th, ph, st, oth, oph, ost, max_step_diff_frac=0.0)
+    _compare(th, ph, st, oth, oph, ost)
     esc = th != -1
     assert (st[esc] == 19).all()
     assert np.allclose(th[esc], 1.0) and np.allclose(ph[esc], 0.5)
@@ -147,4 +158,6 @@ def test_trace_rays_radius_stop_variant(ctx):
     th, ph, st = bt.trace_rays(cam, t, p, params=bt.radius_stop_params(), ctx=ctx)
     ps = O.pose(0.999, math.pi / 2, 0.0, 5.0)
     oth, oph, ost, _ = O.trace_rays(ps, O.radius_stop_params(), t, p)
-    _compare(th, ph, st, oth, oph, ost, max_step_diff_frac=0.002)
+    sensitive = O.ulp_sensitive(ps, O.radius_stop_params(), t, p)
+    assert sensitive.sum() <= 0.01 * t.size
+    _compare(th, ph, st, oth, oph, ost, exempt=sensitive)
