@@ -133,6 +133,8 @@ SIGNATURES = {
     "bt_trace_rays_device": (C.c_int, [C.c_void_p, C.POINTER(BtPose), C.POINTER(BtIntegParams),
                                        C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.POINTER(BtTraceStats)]),
+    "bt_trace_rays_device_noalloc": (C.c_int, [C.c_void_p, C.POINTER(BtPose), C.POINTER(BtIntegParams),
+                                               C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
     "bt_grid_params_default": (None, [C.POINTER(BtGridParams)]),
     "bt_grid_build": (C.c_int, [C.c_void_p, C.POINTER(BtPose), C.POINTER(BtIntegParams), C.c_int,
                                 C.c_int, C.c_int, C.POINTER(BtGridParams), C.POINTER(C.c_void_p)]),

@@ -67,6 +67,9 @@ struct StatsK {
 };
 
 int make_pose(const bt_pose* in, PoseK* out);
+// noalloc.hip: looks up the device's copy of the no-allocation entry
+// point's scratch array (once per device, with that device current).
+int noalloc_setup(int device);
 int make_params(const bt_integ_params* in, ParamsK* out);
 double host_calc_speed(double a, double r, double theta);
 

@@ -323,6 +323,9 @@ int bt_context_create(int device, bt_context** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c.num_cus = prop.multiProcessorCount;
+    // the no-allocation entry point's scratch address on this device (once
+    // per device; a failure is reported by that entry point)
+    (void)noalloc_setup(device);
     // two streams at the default priority: the main one and an auxiliary one
     // for work that overlaps it (joined back before a call returns)
     if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||

@@ -126,6 +126,19 @@ int bt_trace_rays(bt_context* ctx, const bt_pose* pose, const bt_integ_params* p
 int bt_trace_rays_device(bt_context* ctx, const bt_pose* pose, const bt_integ_params* prm,
                          int64_t n, double* d_theta, double* d_phi, int32_t* d_step,
                          bt_trace_stats* stats);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): bt_trace_rays_device on the FP64 path (prm null or
+ * precision BT_PREC_FP64; BT_ERR_INVALID otherwise) whose work queue lives in
+ * a fixed device array of the library (about 36 MiB on each GPU it runs
+ * on, held until the process ends) instead of the context's buffers, so the
+ * call allocates nothing.  n <= BT_NOALLOC_MAX_RAYS, else BT_ERR_INVALID
+ * and nothing runs.  No statistics.  Calls that overlap on one GPU (through
+ * different contexts) share the array and race; calls through one context
+ * are serialised by it. */
+#define BT_NOALLOC_MAX_RAYS (1 << 20)
+int bt_trace_rays_device_noalloc(bt_context* ctx, const bt_pose* pose, const bt_integ_params* prm,
+                                 int64_t n, double* d_theta, double* d_phi, int32_t* d_step);
 
 /* ---- launch-shape tuning of the persistent integrators (no reference
  * counterpart: the reference launches one thread per ray with a fixed block,
