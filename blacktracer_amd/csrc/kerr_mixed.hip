@@ -139,8 +139,22 @@ __device__ __forceinline__ void derivs_m(const RayC& C, double r_d, double th_d,
 //  * the noise summed over a ray's attempts bounds how far its FP32 state
 //    and step sizes may have wandered from the FP64 ones.
 // Either condition hands the ray to an FP64 re-trace from its warm-up state.
+// Besides nu, errmax carries the plain FP32 rounding of its own evaluation
+// (the scale |y| + |h k1| from the rounded state, its reciprocal, the
+// products: kErrRound relative), and a new h that of its FP32 storage and
+// of the hardware exp2/log2 power (kHRound relative); both enter the margin
+// and rho, so that a decision within FP32's own rounding of the threshold
+// is not taken in FP32 either.
 constexpr float kU32 = 5.9604645e-8f;   // 2^-24
 constexpr float kSumAbsDc = 0.11554158f;
+constexpr float kErrRound = 4.f * kU32;
+constexpr float kHRound = 3.f * kU32;
+
+// Rays confined to the equatorial plane (Carter constant q = 0 up to
+// rounding): theta sits at a double root of Theta, a turning point it never
+// leaves in FP64, while FP32 rounding of the theta terms kicks it off the
+// plane.  They are few and short, so they restart in the exact FP64 path.
+constexpr double kPlanarQ = 1e-9;
 
 // One Cash-Karp attempt with the reference's step control (Metric.h:253-297)
 // in FP32; an accepted step adds the FP32 increment to the FP64 state.
@@ -180,20 +194,21 @@ __device__ __forceinline__ bool attempt_m(const RayC& C, double* y, double& z, f
     }
     errmax *= inv_eps;
     nu = kU32 * kSumAbsDc * fabsf(hh) * xs * inv_eps;
-    if (fabsf(errmax - 1.0f) < margin * (nu + 5.f * errmax * rho)) return false;
+    if (fabsf(errmax - 1.0f) < margin * (nu + errmax * (5.f * rho + kErrRound))) return false;
     if (errmax <= 1.0f) {
         z += (double)hh;
 #pragma unroll
         for (int i = 0; i < 5; i++) y[i] += (double)d[i];
         if (errmax > (float)ERRCON) {
             h = (float)SAFETY * hh * powf_pos(errmax, (float)PGROW);
-            rho = 0.2f * nu * rcp(errmax);
+            rho = 0.2f * (nu * rcp(errmax) + kErrRound) + kHRound;
         } else {
             h = (float)ADAPTIVE * hh;
+            rho += kU32;
         }
     } else {
         h = fminf((float)SAFETY * hh * powf_pos(errmax, (float)PSHRNK), 0.1f * hh);
-        rho = 0.25f * (rho + nu * rcp(errmax));
+        rho = 0.25f * (rho + nu * rcp(errmax) + kErrRound) + kHRound;
     }
     return true;
 }
@@ -231,6 +246,7 @@ __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q,
         r_cap = P.r_cap;
     }
     int nstp = 0;
+    bool planar = false;
     float rho = 0.f, nsum = 0.f;
     int cur_prio = 0;
     unsigned long long attempts = 0, restarts = 0;
@@ -263,6 +279,7 @@ __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q,
 #pragma unroll
                     for (int i = 0; i < 5; i++) y[i] = cont.y[i * cont.stride + slot];
                     double b = Q.b[slot], q = Q.q[slot];
+                    planar = !(fabs(q) > kPlanarQ);
                     C.b = (float)b;
                     C.bsq = (float)(b * b);
                     C.q = (float)q;
@@ -270,7 +287,7 @@ __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q,
                     C.Poff = pasq - pa * b;
                     z = cont.y[5 * cont.stride + slot];
                     h = (float)cont.h[slot];
-                    rho = 0.f;
+                    rho = kU32;   // h rounded to FP32
                     nsum = 0.f;
                 }
             }
@@ -280,7 +297,9 @@ __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q,
         if (__ballot(slot >= 0 || !exhausted) == 0) break;
         update_wave_prio(0, slot >= 0, nstp, cur_prio);
         bool restart = false, retrace = false, done = false;
-        if (slot >= 0) {
+        if (slot >= 0 && planar) {
+            restart = done = true;
+        } else if (slot >= 0) {
             float nu;
             const bool decided = attempt_m(C, y, z, h, inv_eps, margin, rho, nu);
             attempts++;

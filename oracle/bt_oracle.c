@@ -242,14 +242,17 @@ void bto_wrap_to_pi(double* thetaW, double* phiW) {
     *phiW = fmod(*phiW, PI2);
 }
 
-/* Test knob (default off): scales every sin/cos of derivs by (1 + k 2^-53),
- * about k ulps, so that tests can find the rays whose result a last-ulp
- * difference of the trigonometric functions moves (the device's sin/cos
- * need not round as the host libm does). */
-static int g_trig_ulp = 0;
+/* Test knobs (default off): move every sin/cos of derivs, or every pow of
+ * the step-size control, one ulp up (k = 1) or down (k = -1), so that tests
+ * can find the rays whose result a last-ulp difference of these functions
+ * moves (the device's libm need not round as the host's does). */
+static int g_trig_ulp = 0, g_pow_ulp = 0;
 void bto_set_trig_ulp(int k) { g_trig_ulp = k; }
-static double d_sin(double x) { return g_trig_ulp ? sin(x) * (1.0 + g_trig_ulp * 0x1p-53) : sin(x); }
-static double d_cos(double x) { return g_trig_ulp ? cos(x) * (1.0 + g_trig_ulp * 0x1p-53) : cos(x); }
+void bto_set_pow_ulp(int k) { g_pow_ulp = k; }
+static double ulp_move(double v, int k) { return k ? nextafter(v, k > 0 ? HUGE_VAL : -HUGE_VAL) : v; }
+static double d_sin(double x) { return ulp_move(sin(x), g_trig_ulp); }
+static double d_cos(double x) { return ulp_move(cos(x), g_trig_ulp); }
+static double d_pow(double x, double y) { return ulp_move(pow(x, y), g_pow_ulp); }
 
 /* Metric.h:198-226.  var = (r, theta, phi, pR, pTheta). */
 void bto_derivs(double a, const double* var, double* out, double b, double q) {
@@ -322,10 +325,10 @@ static void rkqs(double a, double* var, double* dvdz, double* z, double* h, doub
     if (errmax <= 1.0) {
         *z += *h;
         for (int i = 0; i < 5; i++) var[i] = varTemp[i];
-        if (errmax > ERRCON) *h = SAFETY * *h * pow(errmax, PGROW);
+        if (errmax > ERRCON) *h = SAFETY * *h * d_pow(errmax, PGROW);
         else *h = ADAPTIVE * *h;
     } else {
-        *h = fmin(SAFETY * *h * pow(errmax, PSHRNK), 0.1 * *h);
+        *h = fmin(SAFETY * *h * d_pow(errmax, PSHRNK), 0.1 * *h);
     }
 }
 

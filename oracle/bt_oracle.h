@@ -79,6 +79,7 @@ void   bto_rkck(double a, const double* var, const double* dvdz, double h,
 void   bto_wrap_to_pi(double* theta, double* phi);
 /* Test knob: scale derivs' sin/cos by (1 + k 2^-53) (0: off, the default). */
 void   bto_set_trig_ulp(int k);
+void   bto_set_pow_ulp(int k);
 
 /* Initial conditions of one camera-sky direction (Grid.h:524-547):
  * out[0..3] = pR, pTheta, b, q. */

@@ -213,25 +213,57 @@ def trace_rays(ps, prm, theta, phi, nthreads=0):
 
 
 def set_trig_ulp(k):
-    """Test knob: scale the integrator's sin/cos by (1 + k 2^-53); 0 turns it off."""
+    """Test knob: move the integrator's sin/cos one ulp up (k = 1) or down
+    (k = -1); 0 turns it off."""
     lib().bto_set_trig_ulp(int(k))
+
+
+def set_pow_ulp(k):
+    """Test knob: move the step-size control's pow one ulp up (k = 1) or down
+    (k = -1); 0 turns it off."""
+    lib().bto_set_pow_ulp(int(k))
+
+
+# The one-ulp perturbations of the libm calls on the integration path:
+# sin/cos of the equations of motion and pow of the step-size control, each
+# one ulp up and one down.
+_ULP_KNOBS = [(set_trig_ulp, 1), (set_trig_ulp, -1), (set_pow_ulp, 1), (set_pow_ulp, -1)]
 
 
 def ulp_sensitive(ps, prm, theta, phi, tol=1e-10):
     """Rays whose traced result moves by more than tol rad, or whose step
-    count changes, when the integrator's sin/cos move by about one ulp either
-    way: the rays a last-ulp difference between two libms can move."""
+    count changes, when the integrator's sin/cos or its step control's pow
+    move by one ulp either way: the rays a last-ulp difference between two
+    libms can move."""
     t0, p0, s0, _ = trace_rays(ps, prm, theta, phi)
     out = np.zeros(len(theta), bool)
-    for k in (1, -1):
-        set_trig_ulp(k)
+    for knob, k in _ULP_KNOBS:
+        knob(k)
         try:
             t1, p1, s1, _ = trace_rays(ps, prm, theta, phi)
         finally:
-            set_trig_ulp(0)
+            knob(0)
         dp = np.abs(p1 - p0)
         dp = np.minimum(dp, 2 * np.pi - dp)
         out |= (np.abs(t1 - t0) > tol) | (dp > tol) | (s1 != s0)
+    return out
+
+
+def ulp_sensitive_grid(ps, prm, max_level, start_level, angle=True, tol=1e-10):
+    """(N, M) mask of the grid points whose traced value moves by more than
+    tol rad, or whose step count changes, under the same one-ulp
+    perturbations as ulp_sensitive."""
+    g0 = OracleGrid(ps, prm, max_level, start_level, angle)
+    out = np.zeros(g0.steps.shape, bool)
+    for knob, k in _ULP_KNOBS:
+        knob(k)
+        try:
+            g1 = OracleGrid(ps, prm, max_level, start_level, angle)
+        finally:
+            knob(0)
+        d = np.abs(g1.cel - g0.cel).max(-1)
+        d = np.minimum(d, 2 * np.pi - d)
+        out |= (d > tol) | (g1.steps != g0.steps)
     return out
 
 
