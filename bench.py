@@ -105,11 +105,26 @@ def gc_err(t1, p1, t2, p2):
     return 2 * np.arcsin(np.sqrt(np.clip(h, 0, 1)))
 
 
+_T0 = time.perf_counter()
+
+
+def progress(msg):
+    """A progress line on stderr (stdout carries only the result line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 # A ray is ill-conditioned at the rounding level when its FP64 answer moves
 # by more than this (arcsec) under a regrouping of the FP64 arithmetic (the
 # mixed mode with every ray in FP64, crit_eps 1e30): no arithmetic but the
 # FP64 path's own reproduces it.
 ILL_ARCSEC = 0.5
+
+
+def g_rays(level):
+    """Rays of the uniform Grid(level, level) sky: the autotuner calibrates at
+    the size it tunes for."""
+    n = (1 << level) + 1
+    return n * 2 * (n - 1)
 
 
 def err_stats(cel_a, cel_b, cel_w=None):
@@ -605,7 +620,7 @@ def precision_sweep(bt, ctx):
         mk = (lambda prec: bt.radius_stop_params(prec)) if kind == "radius-stop" else bt.IntegParams
         res, shapes = {}, {}
         for name, prec in (("fp64", bt.BT_PREC_FP64), ("mixed", bt.BT_PREC_MIXED)):
-            shapes[name] = ctx.autotune(cam, params=mk(prec), n_rays=1 << 21)
+            shapes[name] = ctx.autotune(cam, params=mk(prec), n_rays=g_rays(level))
             g, t = timed(level, cam, bh, mk(prec))
             res[name] = (g, t)
         g64, t64 = res["fp64"]
@@ -655,8 +670,10 @@ def main():
     # ---- headline: mixed-precision build of the 2049 x 4096 sky, at the
     # launch shape the autotuner picks for this pose and precision
     shapes = {}
+    progress("autotune (mixed)")
     if not args.no_tune:
-        shapes["mixed"] = ctx.autotune(cam, params=p_mixed, n_rays=1 << 21)
+        shapes["mixed"] = ctx.autotune(cam, params=p_mixed, n_rays=g_rays(LEVEL))
+    progress(f"timed mixed builds: {args.steps} steps after {args.warmup} warm-up")
     for _ in range(args.warmup):
         build(p_mixed).close()
     ctx.sync()
@@ -683,8 +700,9 @@ def main():
     achieved = FLOPS_PER_ATTEMPT * attempts / (rk_ms * 1e-3) / 1e12 if rk_ms > 0 else 0.0
 
     # ---- the plain FP64 build of the same sky (tuned for FP64 in turn)
+    progress("FP64 builds")
     if not args.no_tune:
-        shapes["fp64"] = ctx.autotune(cam, params=p_fp64, n_rays=1 << 21)
+        shapes["fp64"] = ctx.autotune(cam, params=p_fp64, n_rays=g_rays(LEVEL))
     build(p_fp64).close()
     ctx.sync()
     ctx.timing(True)
@@ -705,6 +723,7 @@ def main():
     if "mixed" in shapes:
         ctx.set_tune(*(shapes["mixed"][k] for k in ("chunk", "block", "waves")))
     # ---- precision: mixed vs FP64 ray by ray; FP64 vs converged on the sub-lattice
+    progress("precision comparison")
     gm, g64 = build(p_mixed), build(p_fp64)
     prec = err_stats(g64.cel, gm.cel)
     stm = gm.stats()
@@ -722,6 +741,7 @@ def main():
     prec["fp64_vs_converged"]["note"] = "FP64 path (eps 1e-5) against FP64 at eps 1e-10, 513x1024 sub-lattice"
     conv.close()
 
+    progress("1080p frames")
     fr, render_roof, frame_check = frames(bt, S, ctx, args.frames)
     # the metric's gate: the mixed build counts only when every ray of the
     # sky is within 1 arcsec of the FP64 build; otherwise the headline is the
@@ -769,12 +789,16 @@ def main():
         "render_roofline": render_roof,
     }
     out["roofline"]["fp64_path"]["frac"] = out["roofline"]["fp64_path"]["achieved"] / PEAK_FP64_TFLOPS
+    progress("nine precision poses")
     out["precision_poses"] = precision_poses(bt, ctx)
     if not args.no_batch:
+        progress("64-pose batch")
         out["batch"] = batch(bt, ctx)
     if not args.no_serve:
+        progress("serving stream")
         out["serving"] = serve(bt, S, ctx, args.serve_seconds)
     if not args.no_cpu:
+        progress("CPU baseline")
         kept = []
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, kept)
         if kept:

@@ -61,8 +61,9 @@ class Context:
         check(_lib.load().bt_tune_set(self.handle, C.byref(t)))
 
     def autotune(self, camera, params=None, n_rays=1 << 20):
-        """Times seeded rays of the camera's pose over launch shapes, keeps the
-        fastest; returns {chunk, block, waves, rays_per_lane, ms}."""
+        """Times about n_rays directions of the camera's equal-angle sky over
+        launch shapes, keeps the fastest; returns {chunk, block, waves,
+        rays_per_lane, ms}.  Calibrate at the size that will be traced."""
         t = _lib.BtTune()
         ms = C.c_double()
         params = params or IntegParams()

@@ -1,6 +1,8 @@
 // bt_grid.h -- the device-resident adaptive grid (one camera pose).
 #pragma once
 
+#include <vector>
+
 #include "bt_internal.h"
 
 namespace bt {
@@ -25,6 +27,25 @@ int grid_fix_tvertices(Context* c, const GridK& G);
 int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_params* gp,
                      StatsK* h_stats);
 void grid_release(bt_grid* g);
+
+// One grid of a dataflow build (grid_flow.hip), set up by the batch build:
+// its start ray keys [north | south | lattice], the start level's block
+// list [pole-row blocks | lattice blocks] and a second list of the same
+// size.
+struct FlowGridHost {
+    GridK G;
+    PoseK P;
+    uint32_t* keys;
+    uint32_t* list0;
+    uint32_t* list1;
+    unsigned nkeys, nb0, npole;
+    int gap0;
+};
+// Builds the grids' adaptive levels (raytrace's pole rows included) in one
+// persistent kernel on the FP64 path; rays (nullable) receives each grid's
+// traced-ray count.  The grids' arrays and start lattices are set up.
+int grid_flow_run(Context* c, const std::vector<FlowGridHost>& gh, const ParamsK& prm, double thresh,
+                  StatsK* d_stats, std::vector<int64_t>* rays);
 
 }  // namespace bt
 

@@ -171,6 +171,13 @@ struct Context {
     size_t pool_bytes = 0;
     static constexpr size_t kPoolCap = size_t(32) << 30;
     size_t pool_cap = kPoolCap;   // pooled bytes kept at most (0: every freed block is freed)
+    // Dataflow grid builds (grid_flow.hip): the epoch that tags their queue
+    // entries, and the queue buffers as last cleared (a buffer that is new,
+    // regrown or reallocated is cleared before its first use).
+    unsigned flow_epoch = 0;
+    void* flow_rq_seen = nullptr;
+    void* flow_tq_seen = nullptr;
+    uint64_t flow_rcap = 0, flow_tcap = 0;
 
     // A block of at least `bytes` (reused when one of up to 2x that size is
     // pooled); *got receives its actual size.

@@ -52,6 +52,7 @@ void Context::release_buffers() {
     for (auto& kv : scratch)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     scratch.clear();
+    flow_rq_seen = flow_tq_seen = nullptr;
 }
 
 int Context::stage(size_t bytes, void** out) {

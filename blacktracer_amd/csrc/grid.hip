@@ -33,23 +33,13 @@
 #include <cstring>
 #include <vector>
 
-#include "bt_grid.h"
+#include "grid_dev.h"
 
 namespace bt {
 namespace {
 
 constexpr double PI = 3.141592653589793238;
 constexpr double PI2 = 3.141592653589793238 * 2.;
-
-__device__ __forceinline__ bool present_at(const GridK& G, uint32_t i, uint32_t j) {
-    if (i >= (uint32_t)G.N || j >= (uint32_t)G.M) return false;
-    uint32_t idx = i * (uint32_t)G.M + j;
-    return (G.present[idx >> 5] >> (idx & 31)) & 1u;
-}
-
-__device__ __forceinline__ void set_present(const GridK& G, uint32_t idx) {
-    atomicOr(&G.present[idx >> 5], 1u << (idx & 31));
-}
 
 __global__ void init_grid(GridK G, int64_t nm, int64_t nwords) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,25 +70,6 @@ __global__ void pole_keys(uint32_t* keys, uint32_t south) {
     if (threadIdx.x == 0) {
         keys[0] = 0u;
         keys[1] = south;
-    }
-}
-
-// raytrace(), Grid.h:303-315: copy the pole values along the pole rows at
-// the start-level spacing and queue the pole-row blocks (column t < M / gap).
-__device__ __forceinline__ void pole_row_point(const GridK& G, int gap, uint32_t* blocks, uint32_t t) {
-    uint32_t j = t * (uint32_t)gap;
-    uint32_t M = (uint32_t)G.M;
-    G.cel[2 * j] = G.cel[0];
-    G.cel[2 * j + 1] = G.cel[1];
-    G.steps[j] = G.steps[0];
-    set_present(G, j);
-    blocks[t] = j;
-    if (G.eq) {
-        uint32_t s = (uint32_t)(G.N - 1) * M;
-        G.cel[2 * (s + j)] = G.cel[2 * s];
-        G.cel[2 * (s + j) + 1] = G.cel[2 * s + 1];
-        G.steps[s + j] = G.steps[0];   // the north pole's count (Grid.h:312)
-        set_present(G, s + j);
     }
 }
 
@@ -276,39 +247,14 @@ __global__ __launch_bounds__(256) void refine_level_multi(const RefineJob* jobs,
 
 __device__ __forceinline__ void refine_block(const GridK& G, uint32_t key, int level, double thresh,
                                              uint32_t* next_blocks, uint32_t* trace, unsigned* counters) {
+    BlockRefine R;
+    if (!refine_test(G, key, level, thresh, R)) return;
     const uint32_t M = (uint32_t)G.M;
-    uint32_t gap = 1u << (G.max_level - level);
-    uint32_t i = key / M;
-    uint32_t j = key % M;
-    uint32_t k = i + gap / 2;
-    uint32_t l = j + gap / 2;
-    j = j % M;
-    // refineCheck(i, j, gap, level)
-    uint32_t kk = i + gap;
-    uint32_t ll = (j + gap) % M;
-    const double* c = G.cel;
-    double th1 = c[2 * (i * M + j)], th2 = c[2 * (kk * M + j)];
-    double th3 = c[2 * (i * M + ll)], th4 = c[2 * (kk * M + ll)];
-    double ph1 = c[2 * (i * M + j) + 1], ph2 = c[2 * (kk * M + j) + 1];
-    double ph3 = c[2 * (i * M + ll) + 1], ph4 = c[2 * (kk * M + ll) + 1];
-    double diag = (th1 - th4) * (th1 - th4) + (ph1 - ph4) * (ph1 - ph4);
-    double diag2 = (th2 - th3) * (th2 - th3) + (ph2 - ph3) * (ph2 - ph3);
-    double mx = (diag < diag2) ? diag2 : diag;   // std::max
-    bool refine = (level < 6 && mx > 1E-10) || (mx > thresh);
-    if (!refine) {
-        G.level[i * M + j] = (int8_t)level;
-        return;
-    }
-    claim(G, k, j, trace, &counters[0]);
-    claim(G, k, l, trace, &counters[0]);
-    claim(G, i, l, trace, &counters[0]);
-    claim(G, i + gap, l, trace, &counters[0]);
-    claim(G, k, (j + gap) % M, trace, &counters[0]);
+#pragma unroll
+    for (int m = 0; m < 5; m++) claim(G, R.pts[m] / M, R.pts[m] % M, trace, &counters[0]);
     unsigned base = atomicAdd(&counters[1], 4u);
-    next_blocks[base + 0] = i * M + j;
-    next_blocks[base + 1] = k * M + j;
-    next_blocks[base + 2] = k * M + l;
-    next_blocks[base + 3] = i * M + l;
+#pragma unroll
+    for (int m = 0; m < 4; m++) next_blocks[base + m] = R.kids[m];
 }
 
 // refine_block for a whole wave (every lane calls it; `live` lanes hold a
@@ -324,39 +270,16 @@ __device__ __forceinline__ void refine_block_wave(const GridK& G, bool live, uin
         return;
     }
     const unsigned lane = __lane_id();
-    const uint32_t M = (uint32_t)G.M;
+    BlockRefine R;
     bool refine = false;
-    uint32_t pts[5];
     unsigned newmask = 0;
-    uint32_t i = 0, j = 0, k = 0, l = 0;
     if (live) {
-        const uint32_t gap = 1u << (G.max_level - level);
-        i = key / M;
-        j = key % M;
-        k = i + gap / 2;
-        l = j + gap / 2;
-        const uint32_t kk = i + gap, ll = (j + gap) % M;
-        const double* c = G.cel;
-        double th1 = c[2 * (i * M + j)], th2 = c[2 * (kk * M + j)];
-        double th3 = c[2 * (i * M + ll)], th4 = c[2 * (kk * M + ll)];
-        double ph1 = c[2 * (i * M + j) + 1], ph2 = c[2 * (kk * M + j) + 1];
-        double ph3 = c[2 * (i * M + ll) + 1], ph4 = c[2 * (kk * M + ll) + 1];
-        double diag = (th1 - th4) * (th1 - th4) + (ph1 - ph4) * (ph1 - ph4);
-        double diag2 = (th2 - th3) * (th2 - th3) + (ph2 - ph3) * (ph2 - ph3);
-        double mx = (diag < diag2) ? diag2 : diag;   // std::max
-        refine = (level < 6 && mx > 1E-10) || (mx > thresh);
-        if (!refine) {
-            G.level[i * M + j] = (int8_t)level;
-        } else {
-            pts[0] = k * M + j;
-            pts[1] = k * M + l;
-            pts[2] = i * M + l;
-            pts[3] = (i + gap) * M + l;
-            pts[4] = k * M + (j + gap) % M;
+        refine = refine_test(G, key, level, thresh, R);
+        if (refine) {
 #pragma unroll
             for (int m = 0; m < 5; m++) {
-                const uint32_t bit = 1u << (pts[m] & 31);
-                if (!(atomicOr(&G.present[pts[m] >> 5], bit) & bit)) newmask |= 1u << m;
+                const uint32_t bit = 1u << (R.pts[m] & 31);
+                if (!(atomicOr(&G.present[R.pts[m] >> 5], bit) & bit)) newmask |= 1u << m;
             }
         }
     }
@@ -374,17 +297,15 @@ __device__ __forceinline__ void refine_block_wave(const GridK& G, bool live, uin
     base = __shfl(base, 0) + inc - nnew;
 #pragma unroll
     for (int m = 0; m < 5; m++)
-        if ((newmask >> m) & 1u) trace[base++] = pts[m];
+        if ((newmask >> m) & 1u) trace[base++] = R.pts[m];
     // the refined blocks' four children
     const unsigned long long rm = __ballot(refine);
     unsigned nbase = 0;
     if (lane == 0 && rm) nbase = atomicAdd(&counters[1], 4u * (unsigned)__popcll(rm));
     nbase = __shfl(nbase, 0) + 4u * (unsigned)__popcll(rm & ((1ull << lane) - 1));
     if (refine) {
-        next_blocks[nbase + 0] = i * M + j;
-        next_blocks[nbase + 1] = k * M + j;
-        next_blocks[nbase + 2] = k * M + l;
-        next_blocks[nbase + 3] = i * M + l;
+#pragma unroll
+        for (int m = 0; m < 4; m++) next_blocks[nbase + m] = R.kids[m];
     }
 }
 
@@ -905,6 +826,31 @@ int grid_build_batch(Context* c, const std::vector<bt_grid*>& gs, const bt_grid_
     BT_HIP(hipGetLastError());
     start_lattice_multi<<<blocks_for(n_lat, 256), 256, 0, c->stream>>>(d_sj, G, n_lat);
     BT_HIP(hipGetLastError());
+    if (prm.precision == BT_PREC_FP64) {
+        // the FP64 path: every level of every grid in one dataflow kernel
+        std::vector<FlowGridHost> fh(G);
+        for (int k = 0; k < G; k++) {
+            FlowGridHost& f = fh[k];
+            f.G = gs[k]->k;
+            f.P = gs[k]->pose;
+            f.keys = per[k].keys;
+            f.list0 = per[k].blocks;
+            f.list1 = per[k].next;
+            f.nkeys = (unsigned)counts[k];
+            f.nb0 = per[k].nb;
+            f.npole = sj[k].npole;
+            f.gap0 = sj[k].gap;
+        }
+        std::vector<int64_t> rays;
+        if ((rc = grid_flow_run(c, fh, prm, thresh, d_stats, &rays))) return rc;
+        for (int k = 0; k < G; k++) gs[k]->rays = rays[k];
+        if (fix && (rc = grid_fix_tvertices_multi(c, gs))) return rc;
+        StatsK s;
+        BT_HIP(hipMemcpyAsync(&s, d_stats, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+        BT_HIP(hipStreamSynchronize(c->stream));
+        if (h_stats) *h_stats = s;
+        return BT_OK;
+    }
     auto run_batch = [&](const std::vector<int64_t>& counts) -> int {
         BatchHost b;
         for (int k = 0; k < G; k++) {

@@ -222,35 +222,56 @@ __device__ __forceinline__ void derivs(double a, double asq, const double* v, do
 
 // odeint1's loop body (Metric.h:292-297): derivs, scale, rkqs (Metric.h:253-270)
 // with rkck (Metric.h:228-251).  Updates var/z/h in place; returns errmax.
+// Each stage sum of rkck is accumulated as its stages arrive, in rkck's
+// left-to-right order (b41 k1 + b42 k2, then + b43 k3, ...): with no FMA
+// contraction every partial sum rounds exactly as in the one-expression
+// form, and the stage derivatives need not stay live to the end.
 __device__ __forceinline__ double attempt(double a, double asq, double* var, double& z, double& h,
                                           double eps, double b, double q) {
-    double dvdz[5], scal[5], k2[5], k3[5], k4[5], k5[5], k6[5], t[5];
+    double dvdz[5], scal[5], k[5], t[5], s4[5], s5[5], s6[5], pc[5], pd[5];
     derivs(a, asq, var, dvdz, b, q);
 #pragma unroll
     for (int i = 0; i < 5; i++) scal[i] = fabs(var[i]) + fabs(dvdz[i] * h) + TINY;
     const double hh = h;
 #pragma unroll
     for (int i = 0; i < 5; i++) t[i] = var[i] + b21 * hh * dvdz[i];
-    derivs(a, asq, t, k2, b, q);
+    derivs(a, asq, t, k, b, q);   // k2
 #pragma unroll
-    for (int i = 0; i < 5; i++) t[i] = var[i] + hh * (b31 * dvdz[i] + b32 * k2[i]);
-    derivs(a, asq, t, k3, b, q);
+    for (int i = 0; i < 5; i++) {
+        t[i] = var[i] + hh * (b31 * dvdz[i] + b32 * k[i]);
+        s4[i] = b41 * dvdz[i] + b42 * k[i];
+        s5[i] = b51 * dvdz[i] + b52 * k[i];
+        s6[i] = b61 * dvdz[i] + b62 * k[i];
+    }
+    derivs(a, asq, t, k, b, q);   // k3
 #pragma unroll
-    for (int i = 0; i < 5; i++) t[i] = var[i] + hh * (b41 * dvdz[i] + b42 * k2[i] + b43 * k3[i]);
-    derivs(a, asq, t, k4, b, q);
+    for (int i = 0; i < 5; i++) {
+        t[i] = var[i] + hh * (s4[i] + b43 * k[i]);
+        s5[i] = s5[i] + b53 * k[i];
+        s6[i] = s6[i] + b63 * k[i];
+        pc[i] = c1 * dvdz[i] + c3 * k[i];
+        pd[i] = dc1 * dvdz[i] + dc3 * k[i];
+    }
+    derivs(a, asq, t, k, b, q);   // k4
 #pragma unroll
-    for (int i = 0; i < 5; i++)
-        t[i] = var[i] + hh * (b51 * dvdz[i] + b52 * k2[i] + b53 * k3[i] + b54 * k4[i]);
-    derivs(a, asq, t, k5, b, q);
+    for (int i = 0; i < 5; i++) {
+        t[i] = var[i] + hh * (s5[i] + b54 * k[i]);
+        s6[i] = s6[i] + b64 * k[i];
+        pc[i] = pc[i] + c4 * k[i];
+        pd[i] = pd[i] + dc4 * k[i];
+    }
+    derivs(a, asq, t, k, b, q);   // k5
 #pragma unroll
-    for (int i = 0; i < 5; i++)
-        t[i] = var[i] + hh * (b61 * dvdz[i] + b62 * k2[i] + b63 * k3[i] + b64 * k4[i] + b65 * k5[i]);
-    derivs(a, asq, t, k6, b, q);
+    for (int i = 0; i < 5; i++) {
+        t[i] = var[i] + hh * (s6[i] + b65 * k[i]);
+        pd[i] = pd[i] + dc5 * k[i];
+    }
+    derivs(a, asq, t, k, b, q);   // k6
     double errmax = 0.0;
 #pragma unroll
     for (int i = 0; i < 5; i++) {
-        t[i] = var[i] + hh * (c1 * dvdz[i] + c3 * k3[i] + c4 * k4[i] + c6 * k6[i]);
-        double err = hh * (dc1 * dvdz[i] + dc3 * k3[i] + dc4 * k4[i] + dc5 * k5[i] + dc6 * k6[i]);
+        t[i] = var[i] + hh * (pc[i] + c6 * k[i]);
+        double err = hh * (pd[i] + dc6 * k[i]);
         errmax = fmax(errmax, fabs(err / scal[i]));
     }
     errmax /= eps;

@@ -3,10 +3,14 @@
 // Three knobs (Context::tune): threads per workgroup, resident waves per
 // SIMD (which, for a given batch, sets how many rays each lane traces in
 // turn) and the queue chunk a wave reserves per atomic.  bt_autotune times
-// a calibration batch of the caller's pose -- directions uniform on the
-// sphere from a fixed seed, traced with the caller's integrator settings --
-// for candidate shapes by coordinate descent (chunk, then block, then
-// waves), keeps the fastest in the context and reports it.  Results do not
+// a calibration batch of the caller's pose -- the directions of the
+// camera's equal-angle sky (the lattice every grid build traces: theta =
+// i pi / (N - 1), phi = j 2 pi / M, M = 2 (N - 1)) at about n_rays points,
+// traced with the caller's integrator settings -- for candidate shapes by
+// coordinate descent (chunk, then block, then waves), keeps the fastest in
+// the context and reports it.  The best shape depends on the batch size
+// (small batches are bound by their slowest rays, large ones by
+// throughput), so callers calibrate at the size they trace.  Results do not
 // depend on the launch shape: every ray's arithmetic is the same whichever
 // lane traces it.
 #include <cmath>
@@ -18,16 +22,6 @@
 using namespace bt;
 
 namespace {
-
-// splitmix64: a fixed-seed stream of 64-bit values.
-uint64_t mix64(uint64_t& s) {
-    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-double unit(uint64_t& s) { return (double)(mix64(s) >> 11) * (1.0 / 9007199254740992.0); }
 
 int valid(const bt_tune* t) {
     if (t->block != 64 && t->block != 128 && t->block != 256)
@@ -71,13 +65,18 @@ int bt_autotune(bt_context* h, const bt_pose* pose, const bt_integ_params* prm, 
     Context& c = h->impl;
     std::lock_guard<std::recursive_mutex> lk(c.mu);
     BT_HIP(hipSetDevice(c.device));
-    const size_t n = (size_t)n_rays;
+    // the equal-angle sky lattice with N rows and M = 2 (N - 1) columns
+    // closest to n_rays points
+    int64_t rows = 2;
+    while ((rows + 1) * 2 * rows <= n_rays) rows++;
+    const int64_t cols = 2 * (rows - 1);
+    const size_t n = (size_t)(rows * cols);
     std::vector<double> th(n), ph(n);
-    uint64_t seed = 0x5EEDBA11ull;
-    for (size_t k = 0; k < n; k++) {
-        th[k] = std::acos(1.0 - 2.0 * unit(seed));
-        ph[k] = 2.0 * 3.141592653589793238 * unit(seed);
-    }
+    for (int64_t i = 0; i < rows; i++)
+        for (int64_t j = 0; j < cols; j++) {
+            th[(size_t)(i * cols + j)] = (double)i / (double)(rows - 1) * 3.141592653589793238;
+            ph[(size_t)(i * cols + j)] = (double)j / (double)cols * 2.0 * 3.141592653589793238;
+        }
     // pristine inputs (the trace works in place) and the working arrays
     void *d_in, *d_th, *d_ph, *d_st;
     int rc;

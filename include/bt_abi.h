@@ -154,10 +154,12 @@ int bt_tune_get(bt_context* ctx, bt_tune* out);
 /* BT_ERR_INVALID for a block other than 64/128/256, chunk outside
  * [64, 2^20] or waves outside 0..8.  Results never depend on the shape. */
 int bt_tune_set(bt_context* ctx, const bt_tune* t);
-/* Times n_rays (<= 0: 2^20; at most 2^24) seeded uniform directions of the
- * pose under prm for candidate shapes (coordinate descent over chunk
- * {64..1024}, block {64,128,256}, waves {0..4}) and keeps the fastest in
- * the context; out and best_ms (nullable) receive it and its time. */
+/* Times about n_rays (<= 0: 2^20; at most 2^24) directions of the pose's
+ * equal-angle camera sky (the lattice grid builds trace) under prm for
+ * candidate shapes (coordinate descent over chunk {64..1024}, block
+ * {64,128,256}, waves {0..4}) and keeps the fastest in the context; out and
+ * best_ms (nullable) receive it and its time.  The best shape depends on
+ * the batch size: calibrate at the size that will be traced. */
 int bt_autotune(bt_context* ctx, const bt_pose* pose, const bt_integ_params* prm, int64_t n_rays,
                 bt_tune* out, double* best_ms);
 
