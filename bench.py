@@ -113,42 +113,20 @@ def progress(msg):
     print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
-# A ray is ill-conditioned at the rounding level when its FP64 answer moves
-# by more than this (arcsec) under a regrouping of the FP64 arithmetic (the
-# mixed mode with every ray in FP64, crit_eps 1e30): no arithmetic but the
-# FP64 path's own reproduces it.
-ILL_ARCSEC = 0.5
-
-
-def g_rays(level):
-    """Rays of the uniform Grid(level, level) sky: the autotuner calibrates at
-    the size it tunes for."""
-    n = (1 << level) + 1
-    return n * 2 * (n - 1)
-
-
-def err_stats(cel_a, cel_b, cel_w=None):
+def err_stats(cel_a, cel_b):
     """Great-circle errors of grid b against grid a over the rays both trace
-    to the sky; with cel_w (the same sky in regrouped FP64) also the rays
-    above 1" outside the ill-conditioned class."""
+    to the sky."""
     ta, pa = cel_a[..., 0].ravel(), cel_a[..., 1].ravel()
     tb, pb = cel_b[..., 0].ravel(), cel_b[..., 1].ravel()
     traced = ta != -2
     cap_a, cap_b = ta == -1, tb == -1
     ok = traced & ~cap_a & ~cap_b
     e = gc_err(ta[ok], pa[ok], tb[ok], pb[ok]) / ARCSEC
-    out = {"rays_compared": int(ok.sum()),
-           "capture_flags_equal": bool(np.array_equal(cap_a[traced], cap_b[traced])),
-           "mean_arcsec": float(e.mean()), "p99_arcsec": float(np.quantile(e, 0.99)),
-           "p999_arcsec": float(np.quantile(e, 0.999)), "max_arcsec": float(e.max()),
-           "frac_above_1arcsec": float(np.mean(e > 1.0))}
-    if cel_w is not None:
-        tw, pw = cel_w[..., 0].ravel()[ok], cel_w[..., 1].ravel()[ok]
-        ill = gc_err(ta[ok], pa[ok], tw, pw) / ARCSEC > ILL_ARCSEC
-        out["ill_conditioned_rays"] = int(ill.sum())
-        out["above_1arcsec_not_ill"] = int(np.sum((e > 1.0) & ~ill))
-        out["max_arcsec_not_ill"] = float(e[~ill].max())
-    return out
+    return {"rays_compared": int(ok.sum()),
+            "capture_flags_equal": bool(np.array_equal(cap_a[traced], cap_b[traced])),
+            "mean_arcsec": float(e.mean()), "p99_arcsec": float(np.quantile(e, 0.99)),
+            "p999_arcsec": float(np.quantile(e, 0.999)), "max_arcsec": float(e.max()),
+            "above_1arcsec": int(np.sum(e > 1.0)), "frac_above_1arcsec": float(np.mean(e > 1.0))}
 
 
 def _cpu_model():
@@ -563,8 +541,6 @@ def precision_poses(bt, ctx):
             ctx.sync()
             t64 = time.perf_counter() - t0
             row = {"a": a, "r": r, "fp64_ms": 1e3 * t64}
-            gw = bt.Grid(LEVEL, LEVEL, True, cam, bh, params=bt.IntegParams(bt.BT_PREC_MIXED, crit_eps=1e30),
-                         ctx=ctx)
             for name, ce in (("default", None), ("crit_eps_1e-4", 1e-4)):
                 prm = bt.IntegParams(bt.BT_PREC_MIXED) if ce is None else bt.IntegParams(bt.BT_PREC_MIXED,
                                                                                          crit_eps=ce)
@@ -573,20 +549,17 @@ def precision_poses(bt, ctx):
                 gm = bt.Grid(LEVEL, LEVEL, True, cam, bh, params=prm, ctx=ctx)
                 ctx.sync()
                 tm = time.perf_counter() - t0
-                e = err_stats(g64.cel, gm.cel, gw.cel)
+                e = err_stats(g64.cel, gm.cel)
                 st = gm.stats()
                 row[name] = {"max_arcsec": e["max_arcsec"], "p99_arcsec": e["p99_arcsec"],
-                             "mean_arcsec": e["mean_arcsec"], "frac_above_1arcsec": e["frac_above_1arcsec"],
-                             "ill_conditioned_rays": e["ill_conditioned_rays"],
-                             "above_1arcsec_not_ill": e["above_1arcsec_not_ill"],
-                             "max_arcsec_not_ill": e["max_arcsec_not_ill"],
+                             "mean_arcsec": e["mean_arcsec"], "above_1arcsec": e["above_1arcsec"],
+                             "frac_above_1arcsec": e["frac_above_1arcsec"],
                              "capture_flags_equal": e["capture_flags_equal"],
                              "fp64_attempt_share": st["attempts_fp64"] / max(1, st["attempts"]),
                              "fp64_finished_rays": st["fallback_rays"] / max(1, st["integrated"]),
                              "mixed_ms": 1e3 * tm, "mixed_over_fp64": t64 / tm}
                 gm.close()
             g64.close()
-            gw.close()
             rows.append(row)
     return rows
 
@@ -626,10 +599,7 @@ def precision_sweep(bt, ctx):
         g64, t64 = res["fp64"]
         gm, tm = res["mixed"]
         row = {"a": a, "r": r, "grid": f"Grid({level},{level})", "params": kind, "rays": gm.rays_traced}
-        gw = bt.Grid(level, level, True, cam, bh, params=mk(bt.BT_PREC_MIXED), ctx=ctx) if kind != "reference" \
-            else bt.Grid(level, level, True, cam, bh, params=bt.IntegParams(bt.BT_PREC_MIXED, crit_eps=1e30), ctx=ctx)
-        row.update(err_stats(g64.cel, gm.cel, gw.cel))
-        gw.close()
+        row.update(err_stats(g64.cel, gm.cel))
         st = gm.stats()
         row["fp64_attempt_share"] = st["attempts_fp64"] / max(1, st["attempts"])
         row["fallback_rays"] = st["fallback_rays"]

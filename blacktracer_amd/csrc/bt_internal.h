@@ -238,8 +238,6 @@ struct ContK {
     double* h;
     int32_t* nstp;
     int64_t stride;
-    QueueK rq;          // continued rays that near the attempt cap restart here
-    int restart_at;     // ... once they have made this many attempts
 };
 
 // ---------------------------------------------------------------- launchers
@@ -267,7 +265,7 @@ int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const Q
 template <class Src>
 int trace_f64_queue(Context* c, const Src& S, const ParamsK& prm, int64_t n_max, const QueueK& Q,
                     StatsK* d_stats, const char* timer, const ContK* cont,
-                    hipStream_t stream = nullptr, int prio = 0, bool fast = false);
+                    hipStream_t stream = nullptr, int prio = 0);
 template <class Src>
 int trace_f64_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, StatsK* d_stats);
 // kerr_mixed.hip: FP32 bulk with FP64 set-up, warm-up, fallback and accumulation.

@@ -19,11 +19,10 @@ namespace {
 
 using namespace kerr;
 
-// rk_f64 modes: the exact attempt run to the end (the plain FP64 path);
-// attempt_w through the step-size growth only (the mixed path's warm-up);
-// attempt_w run to the end; attempt_w run to the end from the state the
-// warm-up left in cont (the mixed path's re-traced rays).
-constexpr int kExact = 0, kWarm = 1, kFast = 2, kCont = 3;
+// rk_f64 modes: the exact attempt run to the end (the plain FP64 path, and
+// every ray the mixed path sends to FP64); attempt_w through the step-size
+// growth only (the mixed path's warm-up).
+constexpr int kExact = 0, kWarm = 1;
 
 __global__ void zero_counters(unsigned* c, int n) {
     int i = threadIdx.x;
@@ -164,22 +163,14 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
                     }
                     b = Q.b[slot];
                     q = Q.q[slot];
-                    if (MODE == kCont) {
-#pragma unroll
-                        for (int i = 0; i < 5; i++) var[i] = cont.y[i * cont.stride + slot];
-                        z = cont.y[5 * cont.stride + slot];
-                        h = cont.h[slot];
-                        nstp = cont.nstp[slot];
-                    } else {
-                        var[0] = lp.r;
-                        var[1] = lp.theta;
-                        var[2] = lp.phi;
-                        var[3] = Q.pr[slot];
-                        var[4] = Q.pt[slot];
-                        z = 0.0;
-                        h = prm.h0;
-                        nstp = 0;
-                    }
+                    var[0] = lp.r;
+                    var[1] = lp.theta;
+                    var[2] = lp.phi;
+                    var[3] = Q.pr[slot];
+                    var[4] = Q.pt[slot];
+                    z = 0.0;
+                    h = prm.h0;
+                    nstp = 0;
                 } else {
                     exhausted = true;
                 }
@@ -187,11 +178,9 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
         }
         if (__ballot(slot >= 0) == 0) break;
         update_wave_prio(prio, slot >= 0, nstp, cur_prio);
-        bool restart = false;
-        const int64_t my_slot = slot;
         if (slot >= 0) {
             double em;
-            if (MODE != kExact) {
+            if (WARM) {
                 RayW C;
                 C.set(lp.a, lp.asq, b, q);
                 em = attempt_w(C, var, z, h, prm.eps);
@@ -204,12 +193,7 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
             bool esc = z <= prm.z_end || (prm.r_escape > 0 && var[0] > prm.r_escape);
             bool cap = !esc && lp.r_cap > 0 && var[0] < lp.r_cap;
             bool out = !esc && !cap && nstp >= prm.max_steps;
-            // a continued ray that nears the attempt cap restarts in the exact
-            // path, so that a capped ray ends exactly as there
-            restart = MODE == kCont && !esc && !cap && !out && nstp >= cont.restart_at;
-            if (restart) {
-                slot = -1;
-            } else if (esc || cap || out) {
+            if (esc || cap || out) {
                 double th, ph;
                 if (cap) {
                     th = -1;
@@ -232,7 +216,6 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
                 slot = -1;
             }
         }
-        if (MODE == kCont) queue_push(cont.rq, restart, (uint32_t)my_slot, lane);
     }
     unsigned long long s = wave_sum(attempts);
     if (lane == 0 && s) {
@@ -286,21 +269,13 @@ int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const Q
 
 template <class Src>
 int trace_f64_queue(Context* c, const Src& S, const ParamsK& prm, int64_t n_max, const QueueK& Q,
-                    StatsK* d_stats, const char* timer, const ContK* cont, hipStream_t stream, int prio,
-                    bool fast) {
+                    StatsK* d_stats, const char* timer, const ContK* cont, hipStream_t stream, int prio) {
     if (!stream) stream = c->stream;
     KTimer t(c, timer, stream);
     Launch L;
-    if (cont && fast) {
-        if (int rc = persistent_launch(c, rk_f64<kCont, Src>, n_max, &L)) return rc;
-        rk_f64<kCont, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, *cont, prio);
-    } else if (cont) {
+    if (cont) {
         if (int rc = persistent_launch(c, rk_f64<kWarm, Src>, n_max, &L)) return rc;
         rk_f64<kWarm, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, *cont, prio);
-    } else if (fast) {
-        if (int rc = persistent_launch(c, rk_f64<kFast, Src>, n_max, &L)) return rc;
-        rk_f64<kFast, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, ContK{},
-                                                               prio);
     } else {
         if (int rc = persistent_launch(c, rk_f64<kExact, Src>, n_max, &L)) return rc;
         rk_f64<kExact, Src><<<L.blocks, L.threads, 0, stream>>>(S, prm, Q, d_stats, c->tune.chunk, ContK{},
@@ -315,9 +290,9 @@ template int trace_setup<SingleSrc>(Context*, const SingleSrc&, const ParamsK&, 
 template int trace_setup<MultiSrc>(Context*, const MultiSrc&, const ParamsK&, int64_t, const QueueK&,
                                    const QueueK*, StatsK*);
 template int trace_f64_queue<SingleSrc>(Context*, const SingleSrc&, const ParamsK&, int64_t, const QueueK&,
-                                        StatsK*, const char*, const ContK*, hipStream_t, int, bool);
+                                        StatsK*, const char*, const ContK*, hipStream_t, int);
 template int trace_f64_queue<MultiSrc>(Context*, const MultiSrc&, const ParamsK&, int64_t, const QueueK&,
-                                       StatsK*, const char*, const ContK*, hipStream_t, int, bool);
+                                       StatsK*, const char*, const ContK*, hipStream_t, int);
 
 template <class Src>
 int trace_f64_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, StatsK* d_stats) {

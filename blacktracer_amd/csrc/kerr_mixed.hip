@@ -4,22 +4,23 @@
 //   FP64  initial conditions, constants of motion (b, q), the capture test
 //         and the critical-curve test (ic_capture<true>, kerr_f64.hip);
 //         the initial step-size growth of every ray, until its first
-//         attempt with errmax > ERRCON (rk_f64<true>): there the FP64 error
-//         estimate is far below ERRCON while an FP32 one is rounding noise
-//         above it, and the two step sequences would part from the start;
-//         the state (r, theta, phi, pR, pTheta) and the affine parameter z,
-//         to which every accepted FP32 increment is added; at each stage
-//         point Delta = (r-1)^2 - (1-a^2), P = r^2 + a^2 - ab and the range
-//         reduction of theta, so that no FP32 cancellation happens near the
-//         horizon or the poles; every ray near the critical curve, and
-//         every ray whose step collapses (|h| < hmin_fp64) or which nears
-//         the attempt cap, is restarted from its initial state in FP64
-//         (rk_f64, the plain path's kernel) and so ends exactly as there;
-//         every ray whose FP32 step control may have parted from the FP64
-//         one (an accept/reject decision within the rounding noise of the
-//         threshold, or too much noise summed over its attempts; see the
-//         noise accounting below) is traced again in FP64 from the state
-//         the warm-up left (rk_f64<kCont>).
+//         attempt with errmax > ERRCON (rk_f64's warm-up mode): there the
+//         FP64 error estimate is far below ERRCON while an FP32 one is
+//         rounding noise above it, and the two step sequences would part
+//         from the start; the state (r, theta, phi, pR, pTheta) and the
+//         affine parameter z, to which every accepted FP32 increment is
+//         added; at each stage point Delta = (r-1)^2 - (1-a^2), P = r^2 +
+//         a^2 - ab and the range reduction of theta, so that no FP32
+//         cancellation happens near the horizon or the poles.  Every ray
+//         that leaves the FP32 bulk runs the exact FP64 path (rk_f64's exact
+//         mode) from its initial state and so ends exactly as there: the
+//         rays near the critical curve (from the start, on the auxiliary
+//         stream), the equatorial rays, the rays whose step collapses
+//         (|h| < hmin_fp64) or which near the attempt cap, and the rays
+//         whose FP32 step control may part from the FP64 one (an
+//         accept/reject decision within the rounding noise of the
+//         threshold, or too much noise summed over the ray's attempts; see
+//         the noise accounting below).
 //   FP32  the six right-hand-side evaluations, stage combinations, error
 //         estimate and step-size control.
 //
@@ -138,7 +139,8 @@ __device__ __forceinline__ void derivs_m(const RayC& C, double r_d, double th_d,
 //    the old deviation; errmax scales as h^5);
 //  * the noise summed over a ray's attempts bounds how far its FP32 state
 //    and step sizes may have wandered from the FP64 ones.
-// Either condition hands the ray to an FP64 re-trace from its warm-up state.
+// Either condition hands the ray to the exact FP64 path from its initial
+// state.
 // Besides nu, errmax carries the plain FP32 rounding of its own evaluation
 // (the scale |y| + |h k1| from the rounded state, its reciprocal, the
 // products: kErrRound relative), and a new h that of its FP32 storage and
@@ -215,13 +217,13 @@ __device__ __forceinline__ bool attempt_m(const RayC& C, double* y, double& z, f
 
 // Continues the rays queued in Q (the warm-up queue, walked a second time
 // with its own head) from their warm-up state in cont, skipping rays that
-// ended in the warm-up; rays whose step collapses or that near the attempt
-// cap go to Q64 (exact FP64 from the start), rays whose FP32 step control
-// loses touch with the FP64 one go to Qx (FP64 from the warm-up state).
+// ended in the warm-up; rays whose step collapses, that near the attempt
+// cap, that lie in the equatorial plane or whose FP32 step control may
+// part from the FP64 one go to Q64 (the exact FP64 path from the start).
 template <class Src>
 __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q, unsigned* head,
-                                                QueueK Q64, QueueK Qx, ContK cont, StatsK* st,
-                                                int restart_at, unsigned chunk) {
+                                                QueueK Q64, ContK cont, StatsK* st, int restart_at,
+                                                unsigned chunk) {
     const unsigned lane = lane_id();
     const unsigned qlen = Q.counters[0];
     unsigned res_next = 0, res_end = 0;   // this wave's reserved queue positions
@@ -296,7 +298,7 @@ __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q,
         // are not exhausted: they claim again next round
         if (__ballot(slot >= 0 || !exhausted) == 0) break;
         update_wave_prio(0, slot >= 0, nstp, cur_prio);
-        bool restart = false, retrace = false, done = false;
+        bool restart = false, flagged = false, done = false;
         if (slot >= 0 && planar) {
             restart = done = true;
         } else if (slot >= 0) {
@@ -304,13 +306,13 @@ __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q,
             const bool decided = attempt_m(C, y, z, h, inv_eps, margin, rho, nu);
             attempts++;
             nsum += nu;
-            retrace = !decided || !(nsum <= budget);
+            flagged = !decided || !(nsum <= budget);
             const int step = nstp;
             nstp++;
-            bool esc = !retrace && (z <= prm.z_end || (prm.r_escape > 0 && y[0] > prm.r_escape));
-            bool cap = !retrace && !esc && r_cap > 0 && y[0] < r_cap;
-            restart = !retrace && !esc && !cap && (nstp >= restart_at || !(fabsf(h) >= hmin));
-            done = esc || cap || restart || retrace;
+            bool esc = !flagged && (z <= prm.z_end || (prm.r_escape > 0 && y[0] > prm.r_escape));
+            bool cap = !flagged && !esc && r_cap > 0 && y[0] < r_cap;
+            restart = !flagged && !esc && !cap && (nstp >= restart_at || !(fabsf(h) >= hmin));
+            done = esc || cap || restart || flagged;
             if (esc || cap) {
                 double th = -1, ph = -1;
                 if (esc) {
@@ -321,9 +323,8 @@ __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q,
                 ray_store(S.rio(sg), S.local(sg, slot), th, ph, step);
             }
         }
-        queue_push(Q64, restart, (uint32_t)slot, lane);
-        queue_push(Qx, retrace, (uint32_t)slot, lane);
-        if (restart || retrace) restarts++;
+        queue_push(Q64, restart || flagged, (uint32_t)slot, lane);
+        if (restart || flagged) restarts++;
         if (done) slot = -1;
     }
     unsigned long long s = wave_sum(attempts);
@@ -376,26 +377,22 @@ int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, Sta
         }
         return BT_OK;
     }
-    // Qw: warm-up (FP64) then FP32 queue; Qc: near-critical rays, FP64 from
-    // the start, traced on the auxiliary stream while the main stream runs
-    // the warm-up and the FP32 bulk; Qr: rays the FP32 kernel hands back
-    // (step collapse, attempt cap), exact FP64 from the start on the main
-    // stream; Qx: rays whose FP32 step control it hands back (noise
-    // accounting), FP64 from their warm-up state.
-    QueueK Qw, Qc, Qr, Qx;
+    // Qw: warm-up (FP64) then FP32 queue; Qc: near-critical rays, the exact
+    // FP64 path from the start, traced on the auxiliary stream while the
+    // main stream runs the warm-up and the FP32 bulk; Qr: rays the FP32
+    // kernel hands back (equatorial, step collapse, attempt cap, noise
+    // accounting), the exact FP64 path from the start on the main stream.
+    QueueK Qw, Qc, Qr;
     int rc;
     if ((rc = queue_alloc(c, n, &Qw))) return rc;
     void* p;
-    Qc = Qr = Qx = Qw;
+    Qc = Qr = Qw;
     if ((rc = c->buffer("q_slots_crit", sizeof(uint32_t) * n, &p))) return rc;
     Qc.slots = (uint32_t*)p;
     Qc.counters = Qw.counters + 2;   // [2] = length, [3] = head
     if ((rc = c->buffer("q_slots_restart", sizeof(uint32_t) * n, &p))) return rc;
     Qr.slots = (uint32_t*)p;
     Qr.counters = Qw.counters + 6;   // [6] = length, [7] = head
-    if ((rc = c->buffer("q_slots_retrace", sizeof(uint32_t) * n, &p))) return rc;
-    Qx.slots = (uint32_t*)p;
-    Qx.counters = Qw.counters + 8;   // [8] = length, [9] = head
     unsigned* head32 = Qw.counters + 4;
     ContK cont;
     cont.stride = n;
@@ -410,8 +407,6 @@ int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, Sta
     // ray ends exactly as in the FP64 path
     const int margin = prm.max_steps / 16 > 16 ? prm.max_steps / 16 : 16;
     const int restart_at = prm.max_steps - margin > 1 ? prm.max_steps - margin : 1;
-    cont.rq = Qr;
-    cont.restart_at = restart_at;
     if ((rc = trace_setup(c, S, prm, n, Qw, &Qc, d_stats))) return rc;
     // fork: critical rays on the auxiliary stream, at most one workgroup per CU
     BT_HIP(hipEventRecord(c->ev_fork, c->stream));
@@ -419,7 +414,7 @@ int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, Sta
     const int64_t crit_lanes = (int64_t)c->num_cus * c->tune.block;
     // the critical rays' waves issue ahead of the bulk's (base priority 1)
     rc = trace_f64_queue(c, S, prm, n < crit_lanes ? n : crit_lanes, Qc, d_stats, "rk_f64_critical",
-                         nullptr, c->aux, 1, true);
+                         nullptr, c->aux, 1);
     // join (also on error, so that nothing is left running on the aux stream)
     auto join = [&]() -> hipError_t {
         const hipError_t e = hipEventRecord(c->ev_join, c->aux);
@@ -441,16 +436,12 @@ int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, Sta
             (void)join();
             return rc;
         }
-        rk_mixed<Src><<<L.blocks, L.threads, 0, c->stream>>>(S, prm, Qw, head32, Qr, Qx, cont, d_stats,
-                                                             restart_at, c->tune.chunk);
+        rk_mixed<Src><<<L.blocks, L.threads, 0, c->stream>>>(S, prm, Qw, head32, Qr, cont, d_stats, restart_at,
+                                                             c->tune.chunk);
         if (hipGetLastError() != hipSuccess) {
             (void)join();
             return fail(BT_ERR_HIP, "rk_mixed launch failed");
         }
-    }
-    if ((rc = trace_f64_queue(c, S, prm, n, Qx, d_stats, "rk_f64_retrace", &cont, nullptr, 0, true))) {
-        (void)join();
-        return rc;
     }
     rc = trace_f64_queue(c, S, prm, n, Qr, d_stats, "rk_f64_fallback", nullptr);
     const hipError_t je = join();

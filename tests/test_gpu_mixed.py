@@ -1,8 +1,9 @@
 """Mixed-precision ray path (BT_PREC_MIXED) against the plain FP64 path.
 
 What is exact: capture flags (the a-priori test runs in FP64 in both
-paths); every ray sent to FP64 -- near the critical curve, step collapse,
-attempt cap -- ends exactly as in the FP64 path.  What is not: rays
+paths); every ray sent to FP64 -- near the critical curve, equatorial,
+step collapse, attempt cap, flagged by the rounding-noise accounting --
+runs the exact FP64 path from its start and ends exactly as there.  What is not: rays
 traced in FP32 take a step sequence of their own, so their celestial angle
 differs from the FP64 path's by up to that path's own truncation error; the
 bound tested here is statistical (mean, p99) with the maximum reported.
@@ -59,30 +60,42 @@ def test_mixed_vs_fp64_error_and_flags(ctx, a, r):
 
 
 def test_mixed_all_critical_is_fp64(ctx):
-    """crit_eps larger than any distance: every ray is traced in FP64 (the
-    regrouped FP64 attempt), so the results are the FP64 path's to FP64
-    accuracy: flags equal, angles within 1e-9 rad except for a few rays that
-    graze a photon orbit, where any rounding difference is amplified."""
+    """crit_eps larger than any distance: every ray is near-critical and
+    runs the exact FP64 path, so the mixed mode returns the FP64 path's
+    results bit for bit."""
     cam = bt.Camera(math.pi / 2, 0.0, 5.0, black_hole=bt.BlackHole(0.999))
     t, p = _lattice(6)
     t64, p64, s64 = bt.trace_rays(cam, t, p, ctx=ctx)
     tm, pm, sm, st = bt.trace_rays(cam, t, p, params=bt.IntegParams(bt.BT_PREC_MIXED, crit_eps=1e30),
                                    ctx=ctx, return_stats=True)
-    assert np.array_equal(t64 == -1, tm == -1)
-    far = (np.abs(t64 - tm) > 1e-9) | (np.abs(p64 - pm) > 1e-9)
-    print(f"  rays beyond 1e-9 rad: {far.sum()} of {t.size}; steps differ: {(s64 != sm).sum()}", flush=True)
-    assert far.mean() <= 0.005
-    assert np.mean(s64 != sm) <= 0.005
+    assert np.array_equal(t64, tm) and np.array_equal(p64, pm) and np.array_equal(s64, sm)
     assert st["attempts_fp64"] == st["attempts"]
     assert st["fallback_rays"] == st["integrated"]
+
+
+def test_mixed_flagged_rays_run_the_fp64_path(ctx):
+    """A noise budget below any attempt's noise flags every ray of the FP32
+    bulk at its first FP32 attempt, and each then runs the exact FP64 path
+    from its start: step counts and flags equal the FP64 path's, and so do
+    the angles, bit for bit on all but the rays that end during the FP64
+    warm-up (far from the hole, in the regrouped FP64 form), which agree to
+    1e-9 rad."""
+    cam = bt.Camera(math.pi / 2, 0.0, 5.0, black_hole=bt.BlackHole(0.999))
+    t, p = _lattice(6)
+    t64, p64, s64 = bt.trace_rays(cam, t, p, ctx=ctx)
+    tm, pm, sm, st = bt.trace_rays(cam, t, p, params=bt.IntegParams(bt.BT_PREC_MIXED, noise_budget=1e-30),
+                                   ctx=ctx, return_stats=True)
+    assert np.array_equal(s64, sm) and np.array_equal(t64 == -1, tm == -1)
+    d = np.maximum(np.abs(t64 - tm), np.abs(p64 - pm))
+    print(f"  bit-equal rays {np.mean(d == 0):.4f}, max |d angle| {d.max():.3g} rad", flush=True)
+    assert d.max() < 1e-9
+    assert st["fallback_rays"] > 0
 
 
 def test_mixed_attempt_cap_restarts_in_fp64(ctx):
     """Rays of the FP32 bulk that near the attempt cap restart in FP64, so a
     capped ray returns the FP64 path's sentinel (the camera's own angles).
-    crit_eps = 0 sends every ray through the FP32 bulk (near-critical rays
-    run the regrouped FP64 form, which has FP64 accuracy but not the exact
-    path's roundings, and are not restarted)."""
+    crit_eps = 0 sends every ray through the FP32 bulk."""
     cam = bt.Camera(math.pi / 2, 0.0, 5.0, black_hole=bt.BlackHole(0.999))
     t, p = _lattice(6)
     prm64 = bt.IntegParams(bt.BT_PREC_FP64, max_steps=40)
