@@ -42,7 +42,8 @@ class BtIntegParams(C.Structure):
                 ("r_escape", C.c_double), ("r_capture_factor", C.c_double),
                 ("newton_cap", C.c_int), ("precision", C.c_int),
                 ("crit_eps", C.c_double), ("hmin_fp64", C.c_double),
-                ("noise_budget", C.c_double), ("decision_margin", C.c_double)]
+                ("noise_budget", C.c_double), ("decision_margin", C.c_double),
+                ("drift_budget", C.c_double)]
 
 
 class BtTraceStats(C.Structure):

@@ -58,7 +58,7 @@ struct ParamsK {
     int    max_steps, apriori, newton_cap, precision;
     double crit_eps, hmin_fp64;
     double h0;         // h1 * sgn((float)z_end)                      (Metric.h:287)
-    float  noise_budget, decision_margin;
+    float  noise_budget, decision_margin, drift_budget;
 };
 
 // Device counters, one set per launch.

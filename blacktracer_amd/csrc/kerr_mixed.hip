@@ -138,8 +138,14 @@ __device__ __forceinline__ void derivs_m(const RayC& C, double r_d, double th_d,
 //    new h moves by 0.2 nu / errmax, a rejected one's by 0.25 of it and of
 //    the old deviation; errmax scales as h^5);
 //  * the noise summed over a ray's attempts bounds how far its FP32 state
-//    and step sizes may have wandered from the FP64 ones.
-// Either condition hands the ray to the exact FP64 path from its initial
+//    and step sizes may have wandered from the FP64 ones;
+//  * the rounding the accepted theta and phi increments carry, u |h|
+//    (xn_theta + xn_phi) per step, summed over the ray (its drift), bounds
+//    the FP32 part of the final angles' error: a ray whose decisions all
+//    agree with FP64 can still end an arcsecond off when its path winds
+//    far in phi (measured: near-equatorial rays with 2.5e-6 rad of drift
+//    ended 5e-6 rad off, rays below 1e-6 well under an arcsecond).
+// Any of these hands the ray to the exact FP64 path from its initial
 // state.
 // Besides nu, errmax carries the plain FP32 rounding of its own evaluation
 // (the scale |y| + |h k1| from the rounded state, its reciprocal, the
@@ -162,9 +168,10 @@ constexpr double kPlanarQ = 1e-9;
 // in FP32; an accepted step adds the FP32 increment to the FP64 state.
 // Returns false, leaving y, z and h untouched, when the accept/reject
 // decision is within the noise margin of the threshold; nu receives the
-// noise bound of errmax and rho is updated as described above.
+// noise bound of errmax, rho is updated as described above, and an accepted
+// step adds its angle increments' rounding to drift.
 __device__ __forceinline__ bool attempt_m(const RayC& C, double* y, double& z, float& h, float inv_eps,
-                                          float margin, float& rho, float& nu) {
+                                          float margin, float& rho, float& nu, float& drift) {
     float k1[5], k2[5], k3[5], k4[5], k5[5], k6[5], xn[5];
     const float pr0 = (float)y[3], pt0 = (float)y[4];
     const float hh = h;
@@ -198,6 +205,7 @@ __device__ __forceinline__ bool attempt_m(const RayC& C, double* y, double& z, f
     nu = kU32 * kSumAbsDc * fabsf(hh) * xs * inv_eps;
     if (fabsf(errmax - 1.0f) < margin * (nu + errmax * (5.f * rho + kErrRound))) return false;
     if (errmax <= 1.0f) {
+        drift += kU32 * fabsf(hh) * (xn[1] + xn[2]);
         z += (double)hh;
 #pragma unroll
         for (int i = 0; i < 5; i++) y[i] += (double)d[i];
@@ -231,6 +239,7 @@ __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q,
     const float hmin = (float)prm.hmin_fp64;
     const float margin = prm.decision_margin;
     const float budget = prm.noise_budget > 0 ? prm.noise_budget : __builtin_inff();
+    const float dbudget = prm.drift_budget > 0 ? prm.drift_budget : __builtin_inff();
 
     int64_t slot = -1;
     int sg = 0;
@@ -249,7 +258,7 @@ __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q,
     }
     int nstp = 0;
     bool planar = false;
-    float rho = 0.f, nsum = 0.f;
+    float rho = 0.f, nsum = 0.f, drift = 0.f;
     int cur_prio = 0;
     unsigned long long attempts = 0, restarts = 0;
 
@@ -291,6 +300,7 @@ __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q,
                     h = (float)cont.h[slot];
                     rho = kU32;   // h rounded to FP32
                     nsum = 0.f;
+                    drift = 0.f;
                 }
             }
         }
@@ -303,10 +313,10 @@ __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q,
             restart = done = true;
         } else if (slot >= 0) {
             float nu;
-            const bool decided = attempt_m(C, y, z, h, inv_eps, margin, rho, nu);
+            const bool decided = attempt_m(C, y, z, h, inv_eps, margin, rho, nu, drift);
             attempts++;
             nsum += nu;
-            flagged = !decided || !(nsum <= budget);
+            flagged = !decided || !(nsum <= budget) || !(drift <= dbudget);
             const int step = nstp;
             nstp++;
             bool esc = !flagged && (z <= prm.z_end || (prm.r_escape > 0 && y[0] > prm.r_escape));

@@ -96,10 +96,14 @@ typedef struct {
     double crit_eps;          /* mixed: FP64 for rays whose (b, q) lie within crit_eps of the
                                  critical curve (distance in the (b, q) plane; 0.03) */
     double hmin_fp64;         /* mixed: switch a ray to FP64 once |h| < hmin_fp64 */
-    double noise_budget;      /* mixed: FP64 re-trace of a ray once the summed rounding-noise
+    double noise_budget;      /* mixed: the exact FP64 path for a ray once the summed rounding-noise
                                  bound of its FP32 error estimates exceeds this (0.13; 0: off) */
-    double decision_margin;   /* mixed: FP64 re-trace of a ray whose FP32 accept/reject decision
-                                 lies within this many noise bounds of the threshold (4; 0: off) */
+    double decision_margin;   /* mixed: the exact FP64 path for a ray whose FP32 accept/reject
+                                 decision lies within this many noise bounds of the threshold
+                                 (4; 0: off) */
+    double drift_budget;      /* mixed: the exact FP64 path for a ray once the FP32 rounding its
+                                 accepted theta and phi increments carry sums past this many
+                                 radians (1.2e-6; 0: off) */
 } bt_integ_params;
 
 void bt_integ_params_default(bt_integ_params* p);
