@@ -113,6 +113,13 @@ def progress(msg):
     print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
+def g_rays(level):
+    """Rays of the uniform Grid(level, level) sky: the autotuner calibrates at
+    the size it tunes for."""
+    n = (1 << level) + 1
+    return n * 2 * (n - 1)
+
+
 def err_stats(cel_a, cel_b):
     """Great-circle errors of grid b against grid a over the rays both trace
     to the sky."""
