@@ -104,6 +104,7 @@ SIGNATURES = {
     "bt_star_file_read": (C.c_int, [C.c_char_p, I32P, FP, FP, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "bt_grid_file_write": (C.c_int, [C.c_char_p, C.c_void_p]),
     "bt_grid_file_read": (C.c_int, [C.c_char_p] + [C.POINTER(C.c_int)] * 5 + [FP, I32P, I32P]),
+    "bt_selfcheck_division": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.POINTER(C.c_int64)]),
     "bt_tune_get": (C.c_int, [C.c_void_p, C.POINTER(BtTune)]),
     "bt_tune_set": (C.c_int, [C.c_void_p, C.POINTER(BtTune)]),
     "bt_autotune": (C.c_int, [C.c_void_p, C.POINTER(BtPose), C.POINTER(BtIntegParams), C.c_int64,

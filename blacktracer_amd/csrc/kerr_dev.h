@@ -37,6 +37,38 @@ constexpr double dc1 = 37.0 / 378.0 - 2825.0 / 27648.0, dc3 = 250.0 / 621.0 - 18
 __device__ __forceinline__ double sq(double x) { return x * x; }
 __device__ __forceinline__ double sq3(double x) { return x * x * x; }
 
+// Division by a shared denominator d, bit-identical to the compiler's FP64
+// division x / d when |x| (unless zero), |d| and |x / d| lie between about
+// 2^-900 and 2^900: the same steps -- the hardware reciprocal estimate, two
+// Newton-Raphson refinements, the quotient, its remainder by FMA and one
+// correction (Markstein's scheme) -- with the reciprocal refined once per
+// denominator instead of once per division.  In that range the division
+// sequence's pre-scaling and final fix-up leave every value unchanged; they
+// act only on zero, denormal, huge and non-finite operands.
+struct SharedDiv {
+    double d, y;
+    __device__ __forceinline__ explicit SharedDiv(double den) : d(den) {
+        double r = __builtin_amdgcn_rcp(den);
+        double e = fma(-den, r, 1.0);
+        r = fma(r, e, r);
+        e = fma(-den, r, 1.0);
+        y = fma(r, e, r);
+    }
+    // d > 0.  The remainder is taken as d q - x (the negated x - d q, exact),
+    // which gives a zero x's quotient the sign x / d has.
+    __device__ __forceinline__ double pos(double x) const {
+        const double q = x * y;
+        const double rem = fma(d, q, -x);
+        return fma(-rem, y, q);
+    }
+    // Any sign of d: a zero quotient takes the sign of x * y, that of x / d
+    // (a nonzero one has it already).
+    __device__ __forceinline__ double any(double x) const {
+        const double q = x * y;
+        return copysign(fma(fma(-d, q, x), y, q), q);
+    }
+};
+
 // R(r) of Metric.h:96-98 with P and Delta of Metric.h:64-66,92-94.
 __device__ __forceinline__ double Rfun(double a, double asq, double r, double b, double q) {
     double P = sq(r) + asq - a * b;
@@ -65,9 +97,10 @@ __device__ __forceinline__ void initial_conditions(const PoseK& P, double theta,
     double xCam = st * cp;
     double yCam = st * sp;
     double zCam = ct;
-    double yFido = (-yCam + P.speed) / (1 - P.speed * yCam);
-    double xFido = P.neg_s * xCam / (1 - P.speed * yCam);
-    double zFido = P.neg_s * zCam / (1 - P.speed * yCam);
+    const SharedDiv by_dop(1 - P.speed * yCam);
+    double yFido = by_dop.any(-yCam + P.speed);
+    double xFido = by_dop.any(P.neg_s * xCam);
+    double zFido = by_dop.any(P.neg_s * zCam);
     double rFido = xFido * P.bphi / P.k + P.br * yFido + P.brbt_k * zFido;
     double thetaFido = P.btheta * yFido - P.k * zFido;
     double phiFido = -xFido * P.br / P.k + P.bphi * yFido + P.bpbt_k * zFido;
@@ -159,17 +192,17 @@ __device__ __forceinline__ bool check_rup(const PoseK& P, double rV, double bV, 
     if (P.a_zero) return false;
     double ax = rV, bb = 4 * rV;
     const double tol = 0.00001;
-    const double gr = P.gr;
-    double c = bb - (bb - ax) / gr;
-    double d = ax + (bb - ax) / gr;
+    const SharedDiv by_gr(P.gr);   // the golden ratio; (bb - ax) / gr below
+    double c = bb - by_gr.pos(bb - ax);
+    double d = ax + by_gr.pos(bb - ax);
     int it = 0;
     while (fabs(c - d) > tol) {
         if (Rfun(P.a, P.asq, c, bV, qV) < Rfun(P.a, P.asq, d, bV, qV))
             bb = d;
         else
             ax = c;
-        c = bb - (bb - ax) / gr;
-        d = ax + (bb - ax) / gr;
+        c = bb - by_gr.pos(bb - ax);
+        d = ax + by_gr.pos(bb - ax);
         it++;
     }
     iters += it;
@@ -187,7 +220,11 @@ __device__ __forceinline__ bool check_celest(const PoseK& P, double pR, double b
     return check2 || check3;
 }
 
-// Metric.h:198-226.  v = (r, theta, phi, pR, pTheta).
+// Metric.h:198-226.  v = (r, theta, phi, pR, pTheta).  Every operation is
+// the reference's, in its order; the sixteen divisions share five
+// reciprocals (rosq, sinsq, rosq 2 delta, rosqsq and sqrosqdel; the
+// denominators rosq * 2 * delta and rosq * (2 * delta) are the same double,
+// and x / (2 rosq) is half of x / rosq exactly) through SharedDiv.
 __device__ __forceinline__ void derivs(double a, double asq, const double* v, double* o,
                                        double b, double q) {
     const double r = v[0], pr = v[3], pt = v[4];
@@ -203,21 +240,25 @@ __device__ __forceinline__ void derivs(double a, double asq, const double* v, do
     double pthetasq = sq(pt);
     double R = sq(P) - delta * (sq((b - a)) + q);
     double partR = (q + sq(a - b));
-    double btheta = q - cossq * (bsq / sinsq - asq);
+    const SharedDiv by_rosq(rosq), by_sinsq(sinsq);
+    const double bsq_sinsq = by_sinsq.pos(bsq);   // bsq / sinsq
+    double btheta = q - cossq * (bsq_sinsq - asq);
     double rosqsq = sq(2 * rosq);
     double sqrosqdel = (sq(rosq) * delta);
     double asqcossin = asq * cosv * sinv;
     double rtwo = 2 * r - 2;
+    const SharedDiv by_rd2(rosq * 2 * delta), by_rosqsq(rosqsq), by_srd(sqrosqdel);
+    const double bdr = btheta * delta + R;
 
-    o[0] = delta / rosq * pr;
-    o[1] = 1. / rosq * pt;
-    o[2] = (2 * a * P - (2 * a - 2 * b) * delta + (2 * b * cossq * delta) / sinsq) / (rosq * 2 * delta);
-    o[3] = (rtwo * btheta - rtwo * partR + 4 * r * P) / (rosq * (2 * delta)) - (prsq * rtwo) / (2 * rosq)
-        + (4 * pthetasq * r) / rosqsq - ((4 * r - 4) * (btheta * (delta) + R)) / (rosq * sq(2 * delta))
-        + (4 * prsq * r * (delta)) / rosqsq - (r * (btheta * delta + R)) / sqrosqdel;
-    o[4] = ((2 * cosv * sinv * (bsq / sinsq - asq) + (2 * bsq * sq3(cosv)) / sq3(sinv)) * delta) /
-        (rosq * 2 * delta) - (4 * asqcossin * pthetasq) / rosqsq - (4 * asqcossin * prsq * delta) /
-        rosqsq + (asqcossin * (btheta * delta + R)) / sqrosqdel;
+    o[0] = by_rosq.pos(delta) * pr;
+    o[1] = by_rosq.pos(1.) * pt;
+    o[2] = by_rd2.any(2 * a * P - (2 * a - 2 * b) * delta + by_sinsq.pos(2 * b * cossq * delta));
+    o[3] = by_rd2.any(rtwo * btheta - rtwo * partR + 4 * r * P) - by_rosq.pos(prsq * rtwo) * 0.5
+        + by_rosqsq.pos(4 * pthetasq * r) - ((4 * r - 4) * bdr) / (rosq * sq(2 * delta))
+        + by_rosqsq.pos(4 * prsq * r * (delta)) - by_srd.any(r * bdr);
+    o[4] = by_rd2.any((2 * cosv * sinv * (bsq_sinsq - asq) + (2 * bsq * sq3(cosv)) / sq3(sinv)) * delta)
+        - by_rosqsq.pos(4 * asqcossin * pthetasq) - by_rosqsq.pos(4 * asqcossin * prsq * delta)
+        + by_srd.any(asqcossin * bdr);
 }
 
 // odeint1's loop body (Metric.h:292-297): derivs, scale, rkqs (Metric.h:253-270)

@@ -224,6 +224,45 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
     }
 }
 
+// Operand pair i of the shared-division self-check: signs, exponents
+// (2^-200 .. 2^200) and significands from a counter hash, with one class in
+// four special: denominators or numerators whose significand is all ones
+// but for a few low bits, zero numerators of either sign, and denominators
+// that are powers of two.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double operand(unsigned long long h, unsigned long long g) {
+    const unsigned long long sign = h >> 63;
+    const unsigned long long expo = 1023 - 200 + (g % 401);
+    return __longlong_as_double((long long)((sign << 63) | (expo << 52) | (h & 0x000fffffffffffffull)));
+}
+
+__global__ void selfcheck_div(int64_t n, unsigned long long seed, unsigned long long* bad) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    bool miss = false;
+    if (i < n) {
+        const unsigned long long h1 = mix64(seed + 4 * (unsigned long long)i), h2 = mix64(h1 + 1),
+                                 h3 = mix64(h1 + 2), h4 = mix64(h1 + 3);
+        double x = operand(h1, h3), d = operand(h2, h4);
+        const unsigned cls = (unsigned)(h4 >> 60);
+        const unsigned long long ones = 0x000fffffffffffffull - (h3 & 0xff);
+        if (cls == 12) d = __longlong_as_double((__double_as_longlong(d) & ~0x000fffffffffffffll) | (long long)ones);
+        if (cls == 13) x = __longlong_as_double((__double_as_longlong(x) & ~0x000fffffffffffffll) | (long long)ones);
+        if (cls == 14) x = (h3 & 1) ? -0.0 : 0.0;
+        if (cls == 15) d = __longlong_as_double(__double_as_longlong(d) & ~0x000fffffffffffffll);
+        const double ref = x / d;
+        const SharedDiv by(d);
+        miss = __double_as_longlong(by.any(x)) != __double_as_longlong(ref);
+        const SharedDiv byp(fabs(d));
+        miss = miss || __double_as_longlong(byp.pos(x)) != __double_as_longlong(x / fabs(d));
+    }
+    const unsigned long long m = __ballot(miss);
+    if (lane_id() == 0 && m) atomicAdd(bad, (unsigned long long)__popcll(m));
+}
+
 }  // namespace
 
 int zero_words(unsigned* p, int n, hipStream_t st) {
@@ -314,3 +353,25 @@ int trace_f64(Context* c, const PoseK& pose, const ParamsK& prm, int64_t n, cons
 }
 
 }  // namespace bt
+
+extern "C" int bt_selfcheck_division(bt_context* h, int64_t n, uint64_t seed, int64_t* mismatches) {
+    using namespace bt;
+    if (!h || !mismatches || n < 0 || n > (int64_t(1) << 30))
+        return fail(BT_ERR_INVALID, "bt_selfcheck_division: null argument or n outside [0, 2^30]");
+    std::lock_guard<std::recursive_mutex> lk(h->impl.mu);
+    Context& c = h->impl;
+    BT_HIP(hipSetDevice(c.device));
+    void* p;
+    if (int rc = c.buffer("selfcheck_div", sizeof(unsigned long long), &p)) return rc;
+    unsigned long long* bad = (unsigned long long*)p;
+    if (int rc = zero_words((unsigned*)bad, 2, c.stream)) return rc;
+    if (n > 0) {
+        selfcheck_div<<<(unsigned)((n + 255) / 256), 256, 0, c.stream>>>(n, seed, bad);
+        BT_HIP(hipGetLastError());
+    }
+    unsigned long long out = 0;
+    BT_HIP(hipMemcpyAsync(&out, bad, sizeof(out), hipMemcpyDeviceToHost, c.stream));
+    BT_HIP(hipStreamSynchronize(c.stream));
+    *mismatches = (int64_t)out;
+    return BT_OK;
+}

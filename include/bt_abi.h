@@ -144,6 +144,15 @@ int bt_trace_rays_device(bt_context* ctx, const bt_pose* pose, const bt_integ_pa
 int bt_trace_rays_device_noalloc(bt_context* ctx, const bt_pose* pose, const bt_integ_params* prm,
                                  int64_t n, double* d_theta, double* d_phi, int32_t* d_step);
 
+/* Self-check (no reference counterpart): the FP64 integrator divides by a
+ * shared reciprocal where several quotients have one denominator (the
+ * division sequence's own steps, the reciprocal refined once).  This runs
+ * that form against the device's IEEE division on n generated operand pairs
+ * (exponents within 2^+-200, with all-ones significands, zero numerators
+ * and power-of-two denominators among them) and stores in *mismatches the
+ * pairs whose quotients differ in any bit.  n at most 2^30. */
+int bt_selfcheck_division(bt_context* ctx, int64_t n, uint64_t seed, int64_t* mismatches);
+
 /* ---- launch-shape tuning of the persistent integrators (no reference
  * counterpart: the reference launches one thread per ray with a fixed block,
  * integration_wrapper Grid.h:514-556) ---- */
