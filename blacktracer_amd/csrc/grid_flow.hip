@@ -341,9 +341,9 @@ __device__ void run_task(const FlowArgs& A, unsigned long long e, unsigned lane,
     complete_item(A, g, lane);
 }
 
-// Two waves per SIMD: the exact attempt's registers plus the queue state
-// come to a few more than 256, which the bound spills (20 bytes per lane)
-// rather than drop to one wave.
+// Two waves per SIMD: the exact attempt's registers plus the queue and
+// task state come to more than 256, which the bound spills (120 bytes per
+// lane) rather than drop to one wave.
 __global__ __launch_bounds__(256, 2) void grid_flow(FlowArgs A) {
     __shared__ uint32_t s_pts[4][320];
     const unsigned lane = lane_id();

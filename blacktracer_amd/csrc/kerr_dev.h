@@ -221,10 +221,11 @@ __device__ __forceinline__ bool check_celest(const PoseK& P, double pR, double b
 }
 
 // Metric.h:198-226.  v = (r, theta, phi, pR, pTheta).  Every operation is
-// the reference's, in its order; the sixteen divisions share five
-// reciprocals (rosq, sinsq, rosq 2 delta, rosqsq and sqrosqdel; the
-// denominators rosq * 2 * delta and rosq * (2 * delta) are the same double,
-// and x / (2 rosq) is half of x / rosq exactly) through SharedDiv.
+// the reference's, in its order; the sixteen divisions run through
+// SharedDiv, fourteen of them sharing five reciprocals (rosq, sinsq,
+// rosq 2 delta, rosqsq and sqrosqdel; the denominators rosq * 2 * delta and
+// rosq * (2 * delta) are the same double, and x / (2 rosq) is half of
+// x / rosq exactly).
 __device__ __forceinline__ void derivs(double a, double asq, const double* v, double* o,
                                        double b, double q) {
     const double r = v[0], pr = v[3], pt = v[4];
@@ -254,9 +255,10 @@ __device__ __forceinline__ void derivs(double a, double asq, const double* v, do
     o[1] = by_rosq.pos(1.) * pt;
     o[2] = by_rd2.any(2 * a * P - (2 * a - 2 * b) * delta + by_sinsq.pos(2 * b * cossq * delta));
     o[3] = by_rd2.any(rtwo * btheta - rtwo * partR + 4 * r * P) - by_rosq.pos(prsq * rtwo) * 0.5
-        + by_rosqsq.pos(4 * pthetasq * r) - ((4 * r - 4) * bdr) / (rosq * sq(2 * delta))
+        + by_rosqsq.pos(4 * pthetasq * r) - SharedDiv(rosq * sq(2 * delta)).pos((4 * r - 4) * bdr)
         + by_rosqsq.pos(4 * prsq * r * (delta)) - by_srd.any(r * bdr);
-    o[4] = by_rd2.any((2 * cosv * sinv * (bsq_sinsq - asq) + (2 * bsq * sq3(cosv)) / sq3(sinv)) * delta)
+    o[4] = by_rd2.any((2 * cosv * sinv * (bsq_sinsq - asq) + SharedDiv(sq3(sinv)).any(2 * bsq * sq3(cosv))) *
+                      delta)
         - by_rosqsq.pos(4 * asqcossin * pthetasq) - by_rosqsq.pos(4 * asqcossin * prsq * delta)
         + by_srd.any(asqcossin * bdr);
 }
@@ -313,9 +315,9 @@ __device__ __forceinline__ double attempt(double a, double asq, double* var, dou
     for (int i = 0; i < 5; i++) {
         t[i] = var[i] + hh * (pc[i] + c6 * k[i]);
         double err = hh * (pd[i] + dc6 * k[i]);
-        errmax = fmax(errmax, fabs(err / scal[i]));
+        errmax = fmax(errmax, fabs(SharedDiv(scal[i]).pos(err)));   // scal > 0
     }
-    errmax /= eps;
+    errmax = SharedDiv(eps).pos(errmax);   // eps > 0
     if (errmax <= 1.0) {
         z += h;
 #pragma unroll

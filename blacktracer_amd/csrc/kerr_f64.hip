@@ -128,9 +128,10 @@ __global__ __launch_bounds__(256) void ic_capture(Src S, ParamsK prm, int64_t n,
 // FP32 estimate would only see rounding noise; after its first attempt with
 // errmax > ERRCON its state goes to cont.  cont.nstp[ray] = -1 marks a ray
 // that ended here.
+// The exact attempt fits 2 waves per SIMD (248-254 VGPRs).
 template <int MODE, class Src>
-__global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, StatsK* st,
-                                              unsigned chunk, ContK cont, int prio) {
+__global__ __launch_bounds__(256, 2) void rk_f64(Src S, ParamsK prm, QueueK Q, StatsK* st,
+                                                 unsigned chunk, ContK cont, int prio) {
     constexpr bool WARM = MODE == kWarm;
     const unsigned lane = lane_id();
     int cur_prio = 0;
@@ -163,9 +164,12 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
                     }
                     b = Q.b[slot];
                     q = Q.q[slot];
-                    var[0] = lp.r;
-                    var[1] = lp.theta;
-                    var[2] = lp.phi;
+                    // the exact mode reads the camera position from the pose
+                    // rather than holding it (registers)
+                    const PoseK& P0 = S.pose(sg);
+                    var[0] = WARM ? lp.r : P0.r;
+                    var[1] = WARM ? lp.theta : P0.theta;
+                    var[2] = WARM ? lp.phi : P0.phi;
                     var[3] = Q.pr[slot];
                     var[4] = Q.pt[slot];
                     z = 0.0;
@@ -200,8 +204,8 @@ __global__ __launch_bounds__(256) void rk_f64(Src S, ParamsK prm, QueueK Q, Stat
                     ph = -1;
                 } else {
                     // out of attempts: varStart was never overwritten (Metric.h:298-302)
-                    th = esc ? var[1] : lp.theta;
-                    ph = esc ? var[2] : lp.phi;
+                    th = esc ? var[1] : (WARM ? lp.theta : S.pose(sg).theta);
+                    ph = esc ? var[2] : (WARM ? lp.phi : S.pose(sg).phi);
                     wrap_to_pi(th, ph);
                 }
                 ray_store(S.rio(sg), S.local(sg, slot), th, ph, step);
