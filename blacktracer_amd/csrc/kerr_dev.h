@@ -315,7 +315,10 @@ __device__ __forceinline__ double attempt(double a, double asq, double* var, dou
     for (int i = 0; i < 5; i++) {
         t[i] = var[i] + hh * (pc[i] + c6 * k[i]);
         double err = hh * (pd[i] + dc6 * k[i]);
-        errmax = fmax(errmax, fabs(SharedDiv(scal[i]).pos(err)));   // scal > 0
+        // scal >= TINY > 0.  An err below the division's range (2^-900)
+        // gives a ratio far below ERRCON, which decides nothing: errmax's
+        // value is used (pow) only above ERRCON.
+        errmax = fmax(errmax, fabs(SharedDiv(scal[i]).pos(err)));
     }
     errmax = SharedDiv(eps).pos(errmax);   // eps > 0
     if (errmax <= 1.0) {

@@ -164,9 +164,10 @@ def test_trace_rays_radius_stop_variant(ctx):
 
 
 def test_shared_division_is_ieee_division():
-    """The integrator's divisions by a shared reciprocal (the right-hand
-    side's sixteen quotients over five denominators) equal the device's
-    IEEE division bit for bit, on 2^24 generated operand pairs with
+    """The FP64 attempt's divisions through the shared-reciprocal form (the
+    right-hand side's sixteen quotients, five reciprocals among them, and
+    the error ratios) equal the device's IEEE division bit for bit, on 2^24
+    generated operand pairs with
     exponents within 2^+-200 (the right-hand side's values stay far inside:
     r below 1e8, sin^2 theta above 1e-40) and the hard classes among them:
     all-ones significands, zero numerators of either sign, power-of-two
