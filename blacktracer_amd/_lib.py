@@ -43,7 +43,7 @@ class BtIntegParams(C.Structure):
                 ("newton_cap", C.c_int), ("precision", C.c_int),
                 ("crit_eps", C.c_double), ("hmin_fp64", C.c_double),
                 ("noise_budget", C.c_double), ("decision_margin", C.c_double),
-                ("drift_budget", C.c_double)]
+                ("drift_budget", C.c_double), ("polar_b", C.c_double)]
 
 
 class BtTraceStats(C.Structure):

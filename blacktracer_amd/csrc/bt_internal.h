@@ -59,6 +59,7 @@ struct ParamsK {
     double crit_eps, hmin_fp64;
     double h0;         // h1 * sgn((float)z_end)                      (Metric.h:287)
     float  noise_budget, decision_margin, drift_budget;
+    double polar_b;
 };
 
 // Device counters, one set per launch.

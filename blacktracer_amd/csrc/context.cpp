@@ -282,11 +282,14 @@ int make_params(const bt_integ_params* in, ParamsK* out) {
     out->precision = in->precision;
     out->crit_eps = in->crit_eps;
     out->hmin_fp64 = in->hmin_fp64;
-    if (!(in->noise_budget >= 0) || !(in->decision_margin >= 0) || !(in->drift_budget >= 0))
-        return fail(BT_ERR_INVALID, "integ params: noise_budget, decision_margin and drift_budget must be >= 0");
+    if (!(in->noise_budget >= 0) || !(in->decision_margin >= 0) || !(in->drift_budget >= 0) ||
+        !(in->polar_b >= 0))
+        return fail(BT_ERR_INVALID,
+                    "integ params: noise_budget, decision_margin, drift_budget and polar_b must be >= 0");
     out->noise_budget = (float)in->noise_budget;
     out->decision_margin = (float)in->decision_margin;
     out->drift_budget = (float)in->drift_budget;
+    out->polar_b = in->polar_b;
     out->h0 = in->h1 * sgnf((float)in->z_end);   // Metric.h:287
     return BT_OK;
 }
@@ -445,6 +448,7 @@ void bt_integ_params_default(bt_integ_params* p) {
     p->noise_budget = 0.13;
     p->decision_margin = 4.0;
     p->drift_budget = 1.2e-6;
+    p->polar_b = 1e-3;
 }
 
 static int trace_common(bt_context* h, const bt_pose* pose, const bt_integ_params* prm,

@@ -15,7 +15,8 @@
 //         that leaves the FP32 bulk runs the exact FP64 path (rk_f64's exact
 //         mode) from its initial state and so ends exactly as there: the
 //         rays near the critical curve (from the start, on the auxiliary
-//         stream), the equatorial rays, the rays whose step collapses
+//         stream), the equatorial rays and those with b near 0 (which pass
+//         close to a pole), the rays whose step collapses
 //         (|h| < hmin_fp64) or which near the attempt cap, and the rays
 //         whose FP32 step control may part from the FP64 one (an
 //         accept/reject decision within the rounding noise of the
@@ -163,6 +164,15 @@ constexpr float kHRound = 3.f * kU32;
 // leaves in FP64, while FP32 rounding of the theta terms kicks it off the
 // plane.  They are few and short, so they restart in the exact FP64 path.
 constexpr double kPlanarQ = 1e-9;
+// Rays with b near 0 (|b| < prm.polar_b) pass within about |b| / sqrt(q)
+// of a pole, and the b / sin^2 and b^2 cos / sin^3 terms make the error
+// estimate of a step whose stage points straddle the pole hypersensitive to
+// the step size: with |b| of 7e-6 to 7e-5, an FP32 step size 8e-5 off the
+// FP64 one (within the rho bound) moved errmax by 4% instead of the 0.04%
+// that errmax ~ h^5 allows, the next step sizes parted, and the rays ended
+// 2-145 arcseconds off (measured attempt by attempt).  They are few (|b|
+// below 1e-3 is a thin band of the sky), so they restart in the exact FP64
+// path as well.
 
 // One Cash-Karp attempt with the reference's step control (Metric.h:253-297)
 // in FP32; an accepted step adds the FP32 increment to the FP64 state.
@@ -226,7 +236,7 @@ __device__ __forceinline__ bool attempt_m(const RayC& C, double* y, double& z, f
 // Continues the rays queued in Q (the warm-up queue, walked a second time
 // with its own head) from their warm-up state in cont, skipping rays that
 // ended in the warm-up; rays whose step collapses, that near the attempt
-// cap, that lie in the equatorial plane or whose FP32 step control may
+// cap, that lie in the equatorial plane, have b near 0 or whose FP32 step control may
 // part from the FP64 one go to Q64 (the exact FP64 path from the start).
 template <class Src>
 __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q, unsigned* head,
@@ -290,7 +300,7 @@ __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q,
 #pragma unroll
                     for (int i = 0; i < 5; i++) y[i] = cont.y[i * cont.stride + slot];
                     double b = Q.b[slot], q = Q.q[slot];
-                    planar = !(fabs(q) > kPlanarQ);
+                    planar = !(fabs(q) > kPlanarQ) || fabs(b) < prm.polar_b;
                     C.b = (float)b;
                     C.bsq = (float)(b * b);
                     C.q = (float)q;

@@ -104,6 +104,9 @@ typedef struct {
     double drift_budget;      /* mixed: the exact FP64 path for a ray once the FP32 rounding its
                                  accepted theta and phi increments carry sums past this many
                                  radians (1.2e-6; 0: off) */
+    double polar_b;           /* mixed: the exact FP64 path for rays with |b| below this: they pass
+                                 within about |b| / sqrt(q) of a pole, where the FP32 error
+                                 estimate turns hypersensitive to the step size (1e-3; 0: off) */
 } bt_integ_params;
 
 void bt_integ_params_default(bt_integ_params* p);

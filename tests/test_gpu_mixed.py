@@ -92,6 +92,57 @@ def test_mixed_flagged_rays_run_the_fp64_path(ctx):
     assert st["fallback_rays"] > 0
 
 
+def _polar_directions(ps, targets):
+    """Camera-sky directions whose b is each of targets (and its negative):
+    the zero of b along phi at several polar angles (bisection on the
+    oracle's initial conditions), then a first-order step to each target."""
+    from oracle import oracle as O
+
+    def b_of(th, ph):
+        return O.initial_conditions_batch(ps, np.array([th]), np.array([ph]))[0, 2]
+
+    th_out, ph_out = [], []
+    for thd in np.linspace(0.6, 2.6, 8):
+        grid = np.linspace(0.05, math.pi - 0.05, 121)
+        bs = O.initial_conditions_batch(ps, np.full(grid.size, thd), grid)[:, 2]
+        k = int(np.nonzero(np.diff(np.sign(bs)) != 0)[0][0])
+        lo, hi = grid[k], grid[k + 1]
+        for _ in range(60):
+            mid = 0.5 * (lo + hi)
+            if np.sign(b_of(thd, mid)) == np.sign(b_of(thd, lo)):
+                lo = mid
+            else:
+                hi = mid
+        ph0 = 0.5 * (lo + hi)
+        slope = (b_of(thd, ph0 + 1e-6) - b_of(thd, ph0 - 1e-6)) / 2e-6
+        for t in targets:
+            for sg in (1, -1):
+                th_out.append(thd)
+                ph_out.append(ph0 + sg * t / slope)
+    return np.array(th_out), np.array(ph_out)
+
+
+def test_mixed_polar_rays_run_the_fp64_path(ctx):
+    """Rays with |b| below polar_b pass within about |b| / sqrt(q) of a pole,
+    where the FP32 error estimate is hypersensitive to the step size; they
+    restart in the exact FP64 path and end as there (bit for bit, or within
+    1e-9 rad for rays that end in the FP64 warm-up)."""
+    from oracle import oracle as O
+    cam = bt.Camera(math.pi / 2, 0.0, 5.0, black_hole=bt.BlackHole(0.999))
+    ps = O.pose(0.999, math.pi / 2, 0.0, 5.0)
+    t, p = _polar_directions(ps, (1e-7, 1e-5, 3e-4))
+    b = O.initial_conditions_batch(ps, t, p)[:, 2]
+    assert np.all(np.abs(b) < 1e-3) and np.all(np.abs(b) > 0)
+    t64, p64, s64 = bt.trace_rays(cam, t, p, ctx=ctx)
+    tm, pm, sm, st = bt.trace_rays(cam, t, p, params=bt.IntegParams(bt.BT_PREC_MIXED), ctx=ctx,
+                                   return_stats=True)
+    assert np.array_equal(s64, sm) and np.array_equal(t64 == -1, tm == -1)
+    d = np.maximum(np.abs(t64 - tm), np.abs(p64 - pm))
+    print(f"  {t.size} polar rays (|b| 1e-7 .. 3e-4): bit-equal {np.mean(d == 0):.3f}, max |d angle| "
+          f"{d.max():.3g} rad, fallback {st['fallback_rays']}", flush=True)
+    assert d.max() < 1e-9
+
+
 def test_mixed_attempt_cap_restarts_in_fp64(ctx):
     """Rays of the FP32 bulk that near the attempt cap restart in FP64, so a
     capped ray returns the FP64 path's sentinel (the camera's own angles).
