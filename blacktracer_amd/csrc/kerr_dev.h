@@ -321,16 +321,21 @@ __device__ __forceinline__ double attempt(double a, double asq, double* var, dou
         errmax = fmax(errmax, fabs(SharedDiv(scal[i]).pos(err)));
     }
     errmax = SharedDiv(eps).pos(errmax);   // eps > 0
-    if (errmax <= 1.0) {
+    // One pow call serves both branches (the same arguments as the
+    // reference's two calls), so a wave whose lanes split between accept
+    // and reject evaluates it once rather than twice.
+    const bool accept = errmax <= 1.0;
+    const double pw = pow(errmax, accept ? PGROW : PSHRNK);
+    if (accept) {
         z += h;
 #pragma unroll
         for (int i = 0; i < 5; i++) var[i] = t[i];
         if (errmax > ERRCON)
-            h = SAFETY * h * pow(errmax, PGROW);
+            h = SAFETY * h * pw;
         else
             h = ADAPTIVE * h;
     } else {
-        h = fmin(SAFETY * h * pow(errmax, PSHRNK), 0.1 * h);
+        h = fmin(SAFETY * h * pw, 0.1 * h);
     }
     return errmax;
 }
