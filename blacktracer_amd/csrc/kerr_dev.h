@@ -467,16 +467,18 @@ __device__ __forceinline__ double attempt_w(const RayW& C, double* var, double& 
         errmax = fmax(errmax, fabs(err) * rcp64(scal[i]));
     }
     errmax /= eps;
-    if (errmax <= 1.0) {
+    const bool accept = errmax <= 1.0;
+    const double pw = pow(errmax, accept ? PGROW : PSHRNK);   // one call for both branches
+    if (accept) {
         z += h;
 #pragma unroll
         for (int i = 0; i < 5; i++) var[i] = t[i];
         if (errmax > ERRCON)
-            h = SAFETY * h * pow(errmax, PGROW);
+            h = SAFETY * h * pw;
         else
             h = ADAPTIVE * h;
     } else {
-        h = fmin(SAFETY * h * pow(errmax, PSHRNK), 0.1 * h);
+        h = fmin(SAFETY * h * pw, 0.1 * h);
     }
     return errmax;
 }

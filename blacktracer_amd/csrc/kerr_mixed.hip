@@ -214,20 +214,22 @@ __device__ __forceinline__ bool attempt_m(const RayC& C, double* y, double& z, f
     errmax *= inv_eps;
     nu = kU32 * kSumAbsDc * fabsf(hh) * xs * inv_eps;
     if (fabsf(errmax - 1.0f) < margin * (nu + errmax * (5.f * rho + kErrRound))) return false;
-    if (errmax <= 1.0f) {
+    const bool accept = errmax <= 1.0f;
+    const float pw = powf_pos(errmax, accept ? (float)PGROW : (float)PSHRNK);   // one call for both branches
+    if (accept) {
         drift += kU32 * fabsf(hh) * (xn[1] + xn[2]);
         z += (double)hh;
 #pragma unroll
         for (int i = 0; i < 5; i++) y[i] += (double)d[i];
         if (errmax > (float)ERRCON) {
-            h = (float)SAFETY * hh * powf_pos(errmax, (float)PGROW);
+            h = (float)SAFETY * hh * pw;
             rho = 0.2f * (nu * rcp(errmax) + kErrRound) + kHRound;
         } else {
             h = (float)ADAPTIVE * hh;
             rho += kU32;
         }
     } else {
-        h = fminf((float)SAFETY * hh * powf_pos(errmax, (float)PSHRNK), 0.1f * hh);
+        h = fminf((float)SAFETY * hh * pw, 0.1f * hh);
         rho = 0.25f * (rho + nu * rcp(errmax) + kErrRound) + kHRound;
     }
     return true;
