@@ -768,6 +768,14 @@ def main():
     out["roofline"]["fp64_path"]["frac"] = out["roofline"]["fp64_path"]["achieved"] / PEAK_FP64_TFLOPS
     progress("nine precision poses")
     out["precision_poses"] = precision_poses(bt, ctx)
+    dflt = [p["default"] for p in out["precision_poses"]]
+    out["gate"]["all_poses"] = {
+        "max_arcsec": max(d["max_arcsec"] for d in dflt),
+        "above_1arcsec": sum(d["above_1arcsec"] for d in dflt),
+        "capture_flags_equal": all(d["capture_flags_equal"] for d in dflt),
+        "passed": all(d["max_arcsec"] <= 1.0 and d["capture_flags_equal"] for d in dflt),
+        "note": "the same gate over every ray of each of the nine precision poses (a 0.5/0.9/0.999 x r 3/5/20M, "
+                "2049x4096 sky, default critical-curve width)"}
     if not args.no_batch:
         progress("64-pose batch")
         out["batch"] = batch(bt, ctx)

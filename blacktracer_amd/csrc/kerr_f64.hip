@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void ic_capture(Src S, ParamsK prm, int64_t n,
 // FP32 estimate would only see rounding noise; after its first attempt with
 // errmax > ERRCON its state goes to cont.  cont.nstp[ray] = -1 marks a ray
 // that ended here.
-// The exact attempt fits 2 waves per SIMD (248-254 VGPRs).
+// The exact attempt fits 2 waves per SIMD (about 250 VGPRs).
 template <int MODE, class Src>
 __global__ __launch_bounds__(256, 2) void rk_f64(Src S, ParamsK prm, QueueK Q, StatsK* st,
                                                  unsigned chunk, ContK cont, int prio) {
