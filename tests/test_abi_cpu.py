@@ -42,6 +42,9 @@ def test_version_and_defaults():
     p = _lib.BtIntegParams()
     L.bt_integ_params_default(C.byref(p))
     assert (p.eps, p.h1, p.z_end, p.max_steps, p.apriori_capture) == (1e-5, 0.01, -1e7, 1000, 1)
+    # the mixed path's routing defaults
+    assert (p.crit_eps, p.hmin_fp64, p.noise_budget, p.decision_margin, p.drift_budget, p.polar_b) == \
+        (0.03, 1e-6, 0.13, 4.0, 1.2e-6, 1e-3)
     g = _lib.BtGridParams()
     L.bt_grid_params_default(C.byref(g))
     assert g.refine_thresh == 0.015 and g.fix_tvertices == 1
