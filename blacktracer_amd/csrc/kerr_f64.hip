@@ -4,8 +4,8 @@
 // Two kernels per batch:
 //   ic_capture_f64  one lane per ray: initial conditions + a-priori capture
 //                   test; captured rays are written out, escaping rays are
-//                   appended (wave-aggregated) to a work queue with their
-//                   constants of motion.
+//                   appended (staged per wave in LDS) to a work queue with
+//                   their constants of motion.
 //   rk_f64          persistent wavefronts; one lane traces one ray with
 //                   (b, q) and the 5-vector state in registers.  Lanes whose
 //                   ray finished are refilled from the queue after a wave
@@ -29,36 +29,30 @@ __global__ void zero_counters(unsigned* c, int n) {
     if (i < n) c[i] = 0;
 }
 
-// Block-aggregated append to two queues: every thread of the block calls
-// it (uniformly); take0/take1 select the queue(s) ray k goes to.  One
-// atomic per queue per block and round.
-__device__ __forceinline__ void block_push2(const QueueK& Q0, bool take0, const QueueK& Q1,
-                                            bool take1, uint32_t k) {
-    __shared__ unsigned s_cnt[2][4];
-    __shared__ unsigned s_base[2];
-    const unsigned lane = lane_id(), wave = threadIdx.x >> 6;
-    const unsigned long long lt = (1ull << lane) - 1;
-    unsigned long long m0 = __ballot(take0), m1 = __ballot(take1);
-    if (lane == 0) {
-        s_cnt[0][wave] = (unsigned)__popcll(m0);
-        s_cnt[1][wave] = (unsigned)__popcll(m1);
+// Wave-staged append: the wave's lanes with `take` write ray k into the
+// wave's LDS staging list (nbuf entries so far, wave-uniform); the list goes
+// to the queue with one fetch-and-add once it may not hold another round
+// (or, with `flush`, whatever it holds).  Every lane of the wave calls it.
+// One atomic per CAP - 63 or more rays and no block barrier: a shared
+// queue head takes one fetch-and-add every few nanoseconds at best, and a
+// block-wide append would hold every wave of the block at each round.
+template <unsigned CAP>
+__device__ __forceinline__ void wave_append(const QueueK& Qq, uint32_t* buf, unsigned& nbuf, bool take,
+                                            uint32_t k, unsigned lane, bool flush) {
+    const unsigned long long m = __ballot(take);
+    if (take) buf[nbuf + (unsigned)__popcll(m & ((1ull << lane) - 1))] = k;
+    nbuf += (unsigned)__popcll(m);
+    if (nbuf > CAP - 64 || (flush && nbuf)) {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(&Qq.counters[0], nbuf);
+        base = __shfl(base, 0);
+        for (unsigned i = lane; i < nbuf; i += 64) Qq.slots[base + i] = buf[i];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        nbuf = 0;
     }
-    __syncthreads();
-    if (threadIdx.x < 2) {
-        unsigned t = 0;
-        for (unsigned w = 0; w < blockDim.x / 64; w++) t += s_cnt[threadIdx.x][w];
-        const QueueK& Qq = threadIdx.x ? Q1 : Q0;
-        s_base[threadIdx.x] = t ? atomicAdd(&Qq.counters[0], t) : 0;
-    }
-    __syncthreads();
-    unsigned off0 = s_base[0], off1 = s_base[1];
-    for (unsigned w = 0; w < wave; w++) {
-        off0 += s_cnt[0][w];
-        off1 += s_cnt[1][w];
-    }
-    if (take0) Q0.slots[off0 + (unsigned)__popcll(m0 & lt)] = k;
-    if (take1) Q1.slots[off1 + (unsigned)__popcll(m1 & lt)] = k;
-    __syncthreads();   // s_cnt/s_base are reused by the next round
 }
 
 // Initial conditions and a-priori capture test, grid-stride over the rays
@@ -66,11 +60,19 @@ __device__ __forceinline__ void block_push2(const QueueK& Q0, bool take0, const 
 // escaping ray goes to Q.  SPLIT = true (mixed precision): rays whose
 // (b, q) lie within crit_eps of the critical curve (Euclidean distance in
 // the (b, q) plane) go to Q64 and are traced in FP64 throughout; the rest
-// go to Q.  Queue entries and per-ray arrays use the global ray index.
+// go to Q.  Queue entries and per-ray arrays use the global ray index;
+// each wave stages its queue entries in LDS (wave_append), so the order of
+// the entries is not the order of the rays.
+constexpr unsigned kStageQ = 1024, kStageC = 256;
+
 template <bool SPLIT, class Src>
 __global__ __launch_bounds__(256) void ic_capture(Src S, ParamsK prm, int64_t n, QueueK Q,
                                                   QueueK Q64, StatsK* st) {
+    __shared__ uint32_t s_stage_q[4][kStageQ];
+    __shared__ uint32_t s_stage_c[SPLIT ? 4 : 1][SPLIT ? kStageC : 1];
     const unsigned lane = lane_id();
+    const unsigned wave = (threadIdx.x >> 6) & 3;
+    unsigned nq = 0, nc = 0;
     unsigned long long c_rays = 0, c_int = 0, c_nw = 0, c_gd = 0, c_cr = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x; k0 < n; k0 += stride) {
@@ -104,21 +106,37 @@ __global__ __launch_bounds__(256) void ic_capture(Src S, ParamsK prm, int64_t n,
             Q.b[k] = b;
             Q.q[k] = q;
         }
-        block_push2(Q, escapes && !crit, Q64, SPLIT && crit, (uint32_t)k);
+        const bool last = k0 + stride >= n;
+        wave_append<kStageQ>(Q, s_stage_q[wave], nq, escapes && !crit, (uint32_t)k, lane, last);
+        if (SPLIT) wave_append<kStageC>(Q64, s_stage_c[wave], nc, crit, (uint32_t)k, lane, last);
         c_rays += live;
         c_int += escapes;
         c_nw += newton;
         c_gd += golden;
         c_cr += crit;
     }
-    unsigned long long s_rays = wave_sum(c_rays), s_int = wave_sum(c_int), s_nw = wave_sum(c_nw),
-                       s_gd = wave_sum(c_gd), s_cr = wave_sum(c_cr);
-    if (lane == 0 && s_rays) {
-        atomicAdd(&st->rays, s_rays);
-        atomicAdd(&st->integrated, s_int);
-        atomicAdd(&st->newton, s_nw);
-        atomicAdd(&st->golden, s_gd);
-        if (s_cr) atomicAdd(&st->fallback, s_cr);
+    // the block's counts, then at most five atomics per block: the counters
+    // share one cache line, where atomics from every wave would queue
+    __shared__ unsigned long long s_stat[4][5];
+    const unsigned long long s_rays = wave_sum(c_rays), s_int = wave_sum(c_int), s_nw = wave_sum(c_nw),
+                             s_gd = wave_sum(c_gd), s_cr = wave_sum(c_cr);
+    if (lane == 0) {
+        s_stat[wave][0] = s_rays;
+        s_stat[wave][1] = s_int;
+        s_stat[wave][2] = s_nw;
+        s_stat[wave][3] = s_gd;
+        s_stat[wave][4] = s_cr;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        unsigned long long t = 0;
+        for (unsigned w = 0; w < blockDim.x / 64; w++) t += s_stat[w][threadIdx.x];
+        unsigned long long* dst = threadIdx.x == 0   ? &st->rays
+                                  : threadIdx.x == 1 ? &st->integrated
+                                  : threadIdx.x == 2 ? &st->newton
+                                  : threadIdx.x == 3 ? &st->golden
+                                                     : &st->fallback;
+        if (t) atomicAdd(dst, t);
     }
 }
 
@@ -300,8 +318,14 @@ int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const Q
                 const QueueK* Q64, StatsK* d_stats) {
     zero_counters<<<1, 64, 0, c->stream>>>(Q.counters, 16);   // the 64-byte q_ctr block
     KTimer t(c, Q64 ? "ic_split" : "ic_capture_f64");
+    // one resident generation of blocks, grid-stride: each block and wave
+    // then makes its few queue and statistics atomics once
+    int occ = 0;
+    const hipError_t oe = Q64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ic_capture<true, Src>, 256, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ic_capture<false, Src>, 256, 0);
+    if (oe != hipSuccess || occ < 1) occ = 1;
     int64_t blocks = (n + 255) / 256;
-    if (blocks > (int64_t)c->num_cus * 16) blocks = (int64_t)c->num_cus * 16;
+    if (blocks > (int64_t)c->num_cus * occ) blocks = (int64_t)c->num_cus * occ;
     if (Q64)
         ic_capture<true, Src><<<(unsigned)blocks, 256, 0, c->stream>>>(S, prm, n, Q, *Q64, d_stats);
     else
