@@ -232,6 +232,15 @@ struct QueueK {
     unsigned* counters; // [0] = queue length, [1] = head
 };
 
+// The queue counters of one trace: kCtrLines lines of kCtrLine words (128
+// bytes), one per queue or head that kernels running at the same time
+// update with atomics (the mixed path: its warm-up queue, the bulk's head
+// on it, the near-critical queue, the restart queue), so that their
+// atomics do not queue behind each other at one cache line.
+constexpr int kCtrLine = 32;
+constexpr int kCtrLines = 4;
+constexpr int kQueueCtrWords = kCtrLine * kCtrLines;
+
 // Per-ray state handed from one integrator kernel to another (indexed by
 // ray): y holds six arrays of `stride` doubles (r, theta, phi, pR, pTheta, z).
 struct ContK {

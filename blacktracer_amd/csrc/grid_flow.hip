@@ -525,19 +525,10 @@ __global__ __launch_bounds__(256, 2) void grid_flow(FlowArgs A) {
             cm &= cm - 1;
         }
     }
-    const unsigned long long s_att = wave_sum(ws.attempts), s_rays = wave_sum(ws.rays),
-                             s_int = wave_sum(ws.integrated), s_nw = wave_sum(ws.newton),
-                             s_gd = wave_sum(ws.golden);
-    if (lane == 0) {
-        if (s_att) {
-            atomicAdd(&A.stats->attempts, s_att);
-            atomicAdd(&A.stats->attempts_fp64, s_att);
-        }
-        if (s_rays) atomicAdd(&A.stats->rays, s_rays);
-        if (s_int) atomicAdd(&A.stats->integrated, s_int);
-        if (s_nw) atomicAdd(&A.stats->newton, s_nw);
-        if (s_gd) atomicAdd(&A.stats->golden, s_gd);
-    }
+    const unsigned long long cnt[6] = {ws.attempts, ws.attempts, ws.rays, ws.integrated, ws.newton, ws.golden};
+    unsigned long long* const dst[6] = {&A.stats->attempts, &A.stats->attempts_fp64, &A.stats->rays,
+                                        &A.stats->integrated, &A.stats->newton, &A.stats->golden};
+    block_add_stats<6>(cnt, dst);
 }
 
 __global__ void zero_u64(unsigned long long* p, size_t n) {

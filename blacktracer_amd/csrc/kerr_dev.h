@@ -525,6 +525,31 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
+// Adds this thread's counts c[i] into *dst[i], summed over the block first:
+// at most one atomic per counter and block, none for a zero sum.  A launch's
+// statistics share one cache line, where an atomic from every wave would
+// queue behind the others (one every few nanoseconds).  Every thread of the
+// block (at most 256) calls it, once, at the end of the kernel.
+template <int N>
+__device__ __forceinline__ void block_add_stats(const unsigned long long (&c)[N],
+                                                unsigned long long* const (&dst)[N]) {
+    __shared__ unsigned long long s_part[4][N];
+    const unsigned wave = (threadIdx.x >> 6) & 3;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const unsigned long long v = wave_sum(c[i]);
+        if (lane_id() == 0) s_part[wave][i] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        if (threadIdx.x == (unsigned)i) {
+            unsigned long long t = 0;
+            for (unsigned w = 0; w < (blockDim.x + 63) / 64; w++) t += s_part[w][i];
+            if (t) atomicAdd(dst[i], t);
+        }
+}
+
 // Wave-cooperative claim of queue positions for the lanes in ballot mask m
 // (every lane of the wave calls it with the same m).  A wave reserves
 // `chunk` (>= 64) consecutive positions per atomic on *head and hands them

@@ -115,29 +115,9 @@ __global__ __launch_bounds__(256) void ic_capture(Src S, ParamsK prm, int64_t n,
         c_gd += golden;
         c_cr += crit;
     }
-    // the block's counts, then at most five atomics per block: the counters
-    // share one cache line, where atomics from every wave would queue
-    __shared__ unsigned long long s_stat[4][5];
-    const unsigned long long s_rays = wave_sum(c_rays), s_int = wave_sum(c_int), s_nw = wave_sum(c_nw),
-                             s_gd = wave_sum(c_gd), s_cr = wave_sum(c_cr);
-    if (lane == 0) {
-        s_stat[wave][0] = s_rays;
-        s_stat[wave][1] = s_int;
-        s_stat[wave][2] = s_nw;
-        s_stat[wave][3] = s_gd;
-        s_stat[wave][4] = s_cr;
-    }
-    __syncthreads();
-    if (threadIdx.x < 5) {
-        unsigned long long t = 0;
-        for (unsigned w = 0; w < blockDim.x / 64; w++) t += s_stat[w][threadIdx.x];
-        unsigned long long* dst = threadIdx.x == 0   ? &st->rays
-                                  : threadIdx.x == 1 ? &st->integrated
-                                  : threadIdx.x == 2 ? &st->newton
-                                  : threadIdx.x == 3 ? &st->golden
-                                                     : &st->fallback;
-        if (t) atomicAdd(dst, t);
-    }
+    const unsigned long long cnt[5] = {c_rays, c_int, c_nw, c_gd, c_cr};
+    unsigned long long* const dst[5] = {&st->rays, &st->integrated, &st->newton, &st->golden, &st->fallback};
+    block_add_stats<5>(cnt, dst);
 }
 
 // WARM = false: every ray runs to its end (the plain FP64 path).  WARM =
@@ -239,11 +219,9 @@ __global__ __launch_bounds__(256, 2) void rk_f64(Src S, ParamsK prm, QueueK Q, S
             }
         }
     }
-    unsigned long long s = wave_sum(attempts);
-    if (lane == 0 && s) {
-        atomicAdd(&st->attempts, s);
-        atomicAdd(&st->attempts_fp64, s);
-    }
+    const unsigned long long cnt[2] = {attempts, attempts};
+    unsigned long long* const dst[2] = {&st->attempts, &st->attempts_fp64};
+    block_add_stats<2>(cnt, dst);
 }
 
 // Operand pair i of the shared-division self-check: signs, exponents
@@ -308,7 +286,7 @@ int queue_alloc(Context* c, int64_t n, QueueK* Q) {
     Q->q = (double*)p;
     if ((rc = c->buffer("q_slots", sizeof(uint32_t) * n, &p))) return rc;
     Q->slots = (uint32_t*)p;
-    if ((rc = c->buffer("q_ctr", 64, &p))) return rc;
+    if ((rc = c->buffer("q_ctr", sizeof(unsigned) * kQueueCtrWords, &p))) return rc;
     Q->counters = (unsigned*)p;
     return BT_OK;
 }
@@ -316,7 +294,7 @@ int queue_alloc(Context* c, int64_t n, QueueK* Q) {
 template <class Src>
 int trace_setup(Context* c, const Src& S, const ParamsK& prm, int64_t n, const QueueK& Q,
                 const QueueK* Q64, StatsK* d_stats) {
-    zero_counters<<<1, 64, 0, c->stream>>>(Q.counters, 16);   // the 64-byte q_ctr block
+    if (int rc = zero_words(Q.counters, kQueueCtrWords, c->stream)) return rc;   // the q_ctr block
     KTimer t(c, Q64 ? "ic_split" : "ic_capture_f64");
     // one resident generation of blocks, grid-stride: each block and wave
     // then makes its few queue and statistics atomics once

@@ -349,12 +349,9 @@ __global__ __launch_bounds__(256, 5) void rk_mixed(Src S, ParamsK prm, QueueK Q,
         if (restart || flagged) restarts++;
         if (done) slot = -1;
     }
-    unsigned long long s = wave_sum(attempts);
-    unsigned long long r = wave_sum(restarts);
-    if (lane == 0) {
-        if (s) atomicAdd(&st->attempts, s);
-        if (r) atomicAdd(&st->fallback, r);
-    }
+    const unsigned long long cnt[2] = {attempts, restarts};
+    unsigned long long* const dst[2] = {&st->attempts, &st->fallback};
+    block_add_stats<2>(cnt, dst);
 }
 
 }  // namespace
@@ -411,11 +408,11 @@ int trace_mixed_src(Context* c, const Src& S, const ParamsK& prm, int64_t n, Sta
     Qc = Qr = Qw;
     if ((rc = c->buffer("q_slots_crit", sizeof(uint32_t) * n, &p))) return rc;
     Qc.slots = (uint32_t*)p;
-    Qc.counters = Qw.counters + 2;   // [2] = length, [3] = head
+    Qc.counters = Qw.counters + 2 * kCtrLine;   // its own line: length, head
     if ((rc = c->buffer("q_slots_restart", sizeof(uint32_t) * n, &p))) return rc;
     Qr.slots = (uint32_t*)p;
-    Qr.counters = Qw.counters + 6;   // [6] = length, [7] = head
-    unsigned* head32 = Qw.counters + 4;
+    Qr.counters = Qw.counters + 3 * kCtrLine;
+    unsigned* head32 = Qw.counters + kCtrLine;   // the bulk's head on Qw
     ContK cont;
     cont.stride = n;
     if ((rc = c->buffer("cont_y", sizeof(double) * 6 * n, &p))) return rc;

@@ -23,7 +23,7 @@ namespace bt {
 namespace {
 
 // Layout of the array for BT_NOALLOC_MAX_RAYS rays: the queue's four FP64
-// arrays (pR, pTheta, b, q), its slot list, its 16 counter words and the
+// arrays (pR, pTheta, b, q), its slot list, its counter lines and the
 // launch's statistics block, each at a 256-byte boundary.
 constexpr int64_t kMaxRays = BT_NOALLOC_MAX_RAYS;
 constexpr size_t kAlign = 256;
@@ -34,7 +34,7 @@ constexpr size_t kOffB = kOffPt + align_up(sizeof(double) * kMaxRays);
 constexpr size_t kOffQ = kOffB + align_up(sizeof(double) * kMaxRays);
 constexpr size_t kOffSlots = kOffQ + align_up(sizeof(double) * kMaxRays);
 constexpr size_t kOffCounters = kOffSlots + align_up(sizeof(uint32_t) * kMaxRays);
-constexpr size_t kOffStats = kOffCounters + align_up(sizeof(unsigned) * 16);
+constexpr size_t kOffStats = kOffCounters + align_up(sizeof(unsigned) * kQueueCtrWords);
 constexpr size_t kScratchBytes = kOffStats + align_up(sizeof(StatsK));
 static_assert(kScratchBytes < (size_t(40) << 20), "the no-allocation scratch stays near 36 MiB");
 
