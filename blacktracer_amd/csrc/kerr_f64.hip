@@ -126,10 +126,13 @@ __global__ __launch_bounds__(256) void ic_capture(Src S, ParamsK prm, int64_t n,
 // FP32 estimate would only see rounding noise; after its first attempt with
 // errmax > ERRCON its state goes to cont.  cont.nstp[ray] = -1 marks a ray
 // that ended here.
-// The exact attempt fits 2 waves per SIMD (about 250 VGPRs).
+// The exact attempt fits 2 waves per SIMD (about 250 VGPRs).  The warm-up
+// attempt of one pose needs about 170 and is held to 168, which fits 3 with
+// no spills; the multi-pose form (its pose in registers as well) would
+// spill there and keeps 2.
 template <int MODE, class Src>
-__global__ __launch_bounds__(256, 2) void rk_f64(Src S, ParamsK prm, QueueK Q, StatsK* st,
-                                                 unsigned chunk, ContK cont, int prio) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == kWarm && !Src::kMulti ? 3 : 2))) void rk_f64(
+    Src S, ParamsK prm, QueueK Q, StatsK* st, unsigned chunk, ContK cont, int prio) {
     constexpr bool WARM = MODE == kWarm;
     const unsigned lane = lane_id();
     int cur_prio = 0;
