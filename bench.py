@@ -64,7 +64,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes of one rk_mixed launch over the 8.4M-ray sky: rocprofv3 FETCH_SIZE
 # and WRITE_SIZE of that dispatch, each scaled by the same counter on a 1 GiB
 # device copy (profiles/integrator_precision_config_counters.md).
-TRAFFIC_RK_MIXED_BYTES = 982.6e6
+TRAFFIC_RK_MIXED_BYTES = 984.7e6
 ARCSEC = math.pi / 180 / 3600
 
 POSE = dict(a=0.999, theta=math.pi / 2, phi=0.0, r=5.0)
